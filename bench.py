@@ -1,0 +1,220 @@
+"""Headline benchmark: C = A*A for R-MAT scale 20 (1,048,576 rows,
+a/b/c/d = 0.57/0.19/0.19/0.05, 16 edges per row drawn, duplicates removed,
+FP64 values U[-1,1], seed 0) with BRMerge-Precise on one MI355X.
+
+A "step" is one full SpGEMM (validation, product counts, binning, symbolic,
+row pointer, allocation of C, numeric merge) on operands already resident in
+HBM; C stays in HBM and is released after the step (the caching allocator
+keeps the memory).  Prints one JSON line:
+
+  value      = 2 * intermediate_products * steps / elapsed   (GFLOP/s)
+  roofline   = the dominant kernel's algorithmic bytes / its time, from
+               hipEvents recorded around every launch during the timed steps
+  cpu_baseline = the CPU oracle (oracle/, an OpenMP restatement of the
+               reference's BRMerge-Precise) on a bounded row panel of A
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def row_stats(A, B):
+    """Per-row list count L and product count ip (numpy)."""
+    blen = np.diff(np.asarray(B.rpt, np.int64))
+    alen = blen[A.col]
+    cs = np.concatenate([[0], np.cumsum(alen)])
+    rpt = np.asarray(A.rpt, np.int64)
+    ip = cs[rpt[1:]] - cs[rpt[:-1]]
+    L = np.diff(rpt)
+    return L, ip
+
+
+def tiers(L, ip, med_cap, force_long=False):
+    t = np.full(len(ip), 3, np.int8)
+    t[ip == 0] = 0
+    if not force_long:
+        t[(ip > 0) & (ip <= 64)] = 1
+        t[(ip > 64) & (ip <= med_cap) & (L <= 512)] = 2
+    return t
+
+
+def kernel_bytes(name, A, L, ip, nnz_row, tier, st):
+    """Algorithmic bytes moved by all launches of kernel `name` in one SpGEMM.
+
+    Per tier: A entries (col 4 + val 8 + product offset 8 = 20 B), one B
+    gather per product (col 4 + val 8 = 12 B; symbolic kernels read columns
+    only, 4 B), C written once (12 B per nnz).  Upper-level merges read and
+    write 12 B per element.  Ping-pong traffic inside LDS is not counted."""
+    sel = {"k_short_numeric": 1, "k_short_count": 1, "k_medium": 2, "k_sym_hash": 2, "k_sym_bitmap": 3}
+    if name in sel:
+        m = tier == sel[name]
+        nA, nip, nc = int(L[m].sum()), int(ip[m].sum()), int(nnz_row[m].sum())
+        if name in ("k_short_numeric", "k_medium"):
+            return 20 * nA + 12 * nip + 12 * nc + 16 * int(m.sum())
+        return 12 * nA + 4 * nip + 8 * int(m.sum())
+    if name == "k_leaf":
+        return 12 * st["leaf_ip"] + 12 * st["leaf_out"] + 20 * st["leaf_tasks"]
+    if name == "k_lvl_write":
+        return 12 * st["level_in"] + 12 * st["level_out"]
+    if name == "k_lvl_count":
+        return 4 * st["level_in"]
+    return None
+
+
+def cpu_baseline(A, target_s=20.0):
+    """Time the oracle's BRMerge-Precise on a row panel of A*A."""
+    import oracle
+    from brmerge_amd import CSR
+
+    cores = min(16, os.cpu_count() or 1)
+    stride = 64
+    rows = np.arange(0, A.M, stride)
+    rpt = np.asarray(A.rpt, np.int64)
+    lens = rpt[rows + 1] - rpt[rows]
+    prpt = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    idx = np.concatenate([np.arange(rpt[r], rpt[r + 1]) for r in rows]) if len(rows) else np.zeros(0, np.int64)
+    P = (prpt, A.col[idx], A.val[idx], A.N)
+    B = (np.asarray(A.rpt, np.int32), A.col, A.val, A.N)
+    L, ip = row_stats(CSR(prpt, A.col[idx], A.val[idx], len(rows), A.N), A)
+    ip_panel = int(ip.sum())
+    t0 = time.perf_counter()
+    oracle.brmerge_precise(P, B, nthreads=cores)  # warm-up (reference protocol: 1 + mean of runs)
+    t1 = time.perf_counter() - t0
+    reps = int(max(1, min(10, round(target_s / max(t1, 1e-3)))))
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.brmerge_precise(P, B, nthreads=cores)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.mean(ts))
+    return {
+        "value": round(2.0 * ip_panel / t / 1e9, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+        "sample": (f"oracle BRMerge-Precise (OpenMP C restatement of spgemm.h:468-735) on rows i%{stride}==0 "
+                   f"of the same A ({len(rows)} rows, {ip_panel} products) times A; 1 warm-up + mean of {reps}"),
+        "seconds_per_run": round(t, 4),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat20", help="rmatS | er16k | amgN")
+    ap.add_argument("--upper", action="store_true", help="also time BRMerge-Upper")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-profile", action="store_true", help="do not bracket launches with hipEvents")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); multi-GPU sharding is "
+              f"described in DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+
+    import brmerge_amd as bm
+    if bm.lib().brm_init(0) != 0:
+        raise RuntimeError(bm._lib.last_error())
+
+    t_gen = time.perf_counter()
+    A, B = bm.gen.workload(args.workload)
+    t_gen = time.perf_counter() - t_gen
+    L, ip = row_stats(A, B)
+    total_ip = int(ip.sum())
+    dA = bm.DeviceCSR.upload(A)
+    dB = dA if B is A else bm.DeviceCSR.upload(B)
+
+    for _ in range(args.warmup):
+        C = bm.spgemm_dev(dA, dB)
+        C.free()
+
+    bm.set_param("profile", 0 if args.no_profile else 1)
+    bm.prof_reset()
+    timings = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = bm.Precise_Timing()
+        C = bm.spgemm_dev(dA, dB, timing=t)
+        timings.append(t.as_dict())
+        nnz_c = C.nnz
+        last = C
+        if _ < args.steps - 1:
+            C.free()
+    elapsed = time.perf_counter() - t0
+    prof = bm.prof_report()
+    st = bm.last_stats()
+    bm.set_param("profile", 0)
+    crpt = last.rpt.to_host()
+    last.free()
+    nnz_row = np.diff(crpt)
+
+    tier = tiers(L, ip, bm.get_param("med_cap"))
+    ms_per_step = 1000.0 * elapsed / args.steps
+    gflops = 2.0 * total_ip * args.steps / elapsed / 1e9
+    bytes_alg = 4 * (A.M + 1) + 20 * A.nnz + 12 * total_ip + 8 * (A.M + 1) + 12 * nnz_c
+    kernels = {}
+    for k, (n, ms) in prof.items():
+        kernels[k] = {"launches": n, "ms": round(ms, 3)}
+    dom = max(prof.items(), key=lambda kv: kv[1][1])[0] if prof else None
+    roof = None
+    if dom is not None:
+        n, ms = prof[dom]
+        by = kernel_bytes(dom, A, L, ip, nnz_row, tier, st)
+        if by is not None:
+            by_per_launch = by * args.steps / n
+            avg_s = ms / n / 1e3
+            ach = by_per_launch / avg_s / 1e9
+            roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_ms": round(ms / n, 4), "alg_bytes_per_launch": int(by_per_launch)}
+
+    out = {
+        "metric": "SpGEMM GFLOP/s (2*intermediate products/s), BRMerge-Precise, R-MAT A*A",
+        "value": round(gflops, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload} C=A*A" if B is A else args.workload,
+                   "rows": A.M, "nnz_a": A.nnz, "intermediate_products": total_ip, "nnz_c": int(nnz_c),
+                   "variant": "precise", "inputs": "resident in HBM", "generator_s": round(t_gen, 2)},
+        "hbm_gbs_algorithmic": round(bytes_alg / (elapsed / args.steps) / 1e9, 2),
+        "hbm_frac_algorithmic": round(bytes_alg / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": roof,
+        "phases_ms": {k: round(1000 * v, 3) for k, v in timings[-1].items()},
+        "kernels": kernels,
+        "stats": st,
+    }
+    if args.upper:
+        t = bm.Upper_Timing()
+        ts = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            C = bm.spgemm_dev(dA, dB, upper=True, timing=t)
+            ts.append(time.perf_counter() - t0)
+            C.free()
+        out["upper"] = {"value": round(2.0 * total_ip / np.mean(ts) / 1e9, 3), "unit": "GFLOP/s",
+                        "ms_per_step": round(1000 * float(np.mean(ts)), 3)}
+    dA.free()
+    if dB is not dA:
+        dB.free()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(A) if B is A else None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

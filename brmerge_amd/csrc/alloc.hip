@@ -1,0 +1,189 @@
+// Caching device allocator.  hipMalloc/hipFree of the multi-GB outputs and
+// workspaces of a large SpGEMM cost milliseconds each; blocks are therefore
+// kept after release and reused by later calls of the same or smaller size.
+// On allocation failure every cached block is returned to the driver and the
+// request retried once.  All library calls synchronise their stream before
+// returning, so a released block is never still in use by the device.
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "brm_internal.h"
+
+namespace brm {
+namespace {
+
+std::mutex g_mu;
+std::multimap<size_t, void*> g_free;         // size -> block
+std::unordered_map<void*, size_t> g_used;    // block -> size
+size_t g_cached = 0;
+int64_t g_fail_after = -1;
+
+size_t round_size(size_t b) {
+    if (b < 256) b = 256;
+    if (b < (1u << 20)) return (b + 511) & ~size_t(511);
+    const size_t g = size_t(2) << 20;
+    return (b + g - 1) / g * g;
+}
+
+void release_cached_locked() {
+    for (auto& kv : g_free) (void)hipFree(kv.second);
+    g_free.clear();
+    g_cached = 0;
+}
+
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_fail_after >= 0) {
+        if (g_fail_after == 0) { g_fail_after = -1; return nullptr; }
+        g_fail_after--;
+    }
+    size_t sz = round_size(bytes);
+    // Reuse the smallest cached block that is not more than 25% larger.
+    auto it = g_free.lower_bound(sz);
+    if (it != g_free.end() && it->first <= sz + sz / 4) {
+        void* p = it->second;
+        size_t bsz = it->first;
+        g_free.erase(it);
+        g_cached -= bsz;
+        g_used[p] = bsz;
+        return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        release_cached_locked();
+        if (hipMalloc(&p, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    g_used[p] = sz;
+    return p;
+}
+
+void dev_free(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_used.find(p);
+    if (it == g_used.end()) return;  // not ours
+    g_free.emplace(it->second, p);
+    g_cached += it->second;
+    g_used.erase(it);
+}
+
+namespace {
+thread_local std::vector<void*> t_deferred;
+}
+
+void dev_defer_free(void* p) {
+    if (p) t_deferred.push_back(p);
+}
+
+void dev_flush_deferred() {
+    for (void* p : t_deferred) dev_free(p);
+    t_deferred.clear();
+}
+
+void dev_trim() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    release_cached_locked();
+}
+
+size_t dev_cached_bytes() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_cached;
+}
+
+// ---------------------------------------------------------------- profiling
+namespace {
+struct PendingLaunch {
+    const char* name;
+    hipEvent_t a, b;
+};
+std::mutex p_mu;
+thread_local const char* t_open_name = nullptr;
+thread_local hipEvent_t t_open_ev = nullptr;
+std::vector<PendingLaunch> p_pending;
+std::map<std::string, std::pair<int64_t, double>> p_totals;
+
+void prof_drain_locked() {
+    for (auto& pl : p_pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(pl.b) == hipSuccess && hipEventElapsedTime(&ms, pl.a, pl.b) == hipSuccess) {
+            auto& t = p_totals[pl.name];
+            t.first += 1;
+            t.second += ms;
+        }
+        (void)hipEventDestroy(pl.a);
+        (void)hipEventDestroy(pl.b);
+    }
+    p_pending.clear();
+}
+}  // namespace
+
+void prof_begin(const char* name, hipStream_t s) {
+    if (!params().profile) return;
+    hipEvent_t a;
+    if (hipEventCreate(&a) != hipSuccess) return;
+    (void)hipEventRecord(a, s);
+    t_open_name = name;
+    t_open_ev = a;
+}
+
+void prof_end(hipStream_t s) {
+    if (!t_open_ev) return;
+    hipEvent_t b;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(t_open_ev);
+        t_open_ev = nullptr;
+        return;
+    }
+    (void)hipEventRecord(b, s);
+    std::lock_guard<std::mutex> lk(p_mu);
+    p_pending.push_back({t_open_name, t_open_ev, b});
+    t_open_ev = nullptr;
+    if (p_pending.size() > 4096) prof_drain_locked();
+}
+
+void dev_set_fail_after(int64_t n) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_fail_after = n;
+}
+
+}  // namespace brm
+
+extern "C" {
+
+void brm_prof_reset(void) {
+    std::lock_guard<std::mutex> lk(brm::p_mu);
+    brm::prof_drain_locked();
+    brm::p_totals.clear();
+}
+
+int brm_prof_report(char* buf, uint64_t cap) {
+    std::lock_guard<std::mutex> lk(brm::p_mu);
+    brm::prof_drain_locked();
+    std::string out = "{";
+    bool first = true;
+    for (auto& kv : brm::p_totals) {
+        char item[256];
+        snprintf(item, sizeof(item), "%s\"%s\": [%lld, %.6f]", first ? "" : ", ", kv.first.c_str(),
+                 (long long)kv.second.first, kv.second.second);
+        out += item;
+        first = false;
+    }
+    out += "}";
+    if (!buf || out.size() + 1 > cap) return BRM_E_INVALID;
+    memcpy(buf, out.c_str(), out.size() + 1);
+    return BRM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,870 @@
+// Host orchestration and the C ABI (include/brmerge.h).
+//
+// One SpGEMM call runs these phases on one HIP stream (field names follow
+// Precise_Timing / Upper_Timing, BRMerge-SpGEMM/inc/Timings.h:4-46):
+//   pre_allocate     scratch buffers, input validation
+//   compute_flop     per-A-entry product counts + their prefix (one scan)
+//   load_balance     per-row product counts and tier bins
+//   symbolic         exact per-row nnz (Precise only)
+//   prefix_sum_nnz   row pointer of C
+//   allocate_C       C (Precise) / C-bar sized by the product bound (Upper)
+//   compute          numeric merge, all tiers
+//   copy             C-bar -> C compaction (Upper only)
+//   cleanup          scratch release
+// The host reads back only small counters (bin sizes, nnz(C), planner task
+// counts, tile counts) between phases.
+#include <stdio.h>
+#include <stdarg.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "brm_device.h"
+#include "kernels.h"
+
+namespace brm {
+
+Params& params() {
+    static Params p{0, MED_CAP, 0, MED_CAP, 0};
+    return p;
+}
+
+namespace {
+
+thread_local std::string t_err;
+thread_local brm_stats t_stats;
+thread_local int t_device = 0;
+
+void set_err(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    t_err = buf;
+}
+
+int hip_status(hipError_t e, const char* what) {
+    if (e == hipSuccess) return BRM_OK;
+    if (e == hipErrorOutOfMemory) {
+        set_err("%s: out of device memory", what);
+        return BRM_E_NOMEM;
+    }
+    set_err("%s: %s", what, hipGetErrorString(e));
+    return BRM_E_DEVICE;
+}
+
+#define HIPC(expr)                                   \
+    do {                                             \
+        hipError_t e_ = (expr);                      \
+        if (e_ != hipSuccess) return hip_status(e_, #expr); \
+    } while (0)
+
+#define BRMC(expr)                  \
+    do {                            \
+        int st_ = (expr);           \
+        if (st_ != BRM_OK) return st_; \
+    } while (0)
+
+// Device allocations owned by one call; freed on scope exit unless released.
+struct Scratch {
+    std::vector<void*> bufs;
+    ~Scratch() {
+        for (void* p : bufs) dev_free(p);
+    }
+    template <typename T>
+    int alloc(T** p, int64_t n, const char* what) {
+        *p = (T*)dev_alloc(sizeof(T) * (size_t)(n > 0 ? n : 1));
+        if (!*p) {
+            set_err("device allocation of %s (%lld elements) failed", what, (long long)n);
+            return BRM_E_NOMEM;
+        }
+        bufs.push_back(*p);
+        return BRM_OK;
+    }
+    void release(void* p) {
+        auto it = std::find(bufs.begin(), bufs.end(), p);
+        if (it != bufs.end()) bufs.erase(it);
+    }
+    void free_now(void* p) {
+        release(p);
+        dev_free(p);
+    }
+};
+
+enum Phase { P_PRE, P_FLOP, P_PFLOP, P_LB, P_SYM, P_PNNZ, P_ALLOC, P_COMPUTE, P_COPY, P_CLEAN, P_N };
+
+// hipEvent pairs around phases, summed per phase.
+struct PhaseClock {
+    hipStream_t s;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    int open_phase = -1;
+    hipEvent_t open_ev = nullptr;
+    explicit PhaseClock(hipStream_t st) : s(st) {}
+    ~PhaseClock() {
+        for (auto& m : marks) {
+            (void)hipEventDestroy(m.second.first);
+            (void)hipEventDestroy(m.second.second);
+        }
+        if (open_ev) (void)hipEventDestroy(open_ev);
+    }
+    void begin(int p) {
+        end();
+        if (hipEventCreate(&open_ev) != hipSuccess) { open_ev = nullptr; return; }
+        (void)hipEventRecord(open_ev, s);
+        open_phase = p;
+    }
+    void end() {
+        if (open_phase < 0 || !open_ev) return;
+        hipEvent_t e1;
+        if (hipEventCreate(&e1) == hipSuccess) {
+            (void)hipEventRecord(e1, s);
+            marks.push_back({open_phase, {open_ev, e1}});
+        } else {
+            (void)hipEventDestroy(open_ev);
+        }
+        open_ev = nullptr;
+        open_phase = -1;
+    }
+    void collect(double* out) {
+        end();
+        for (int i = 0; i < P_N; ++i) out[i] = 0.0;
+        for (auto& m : marks) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, m.second.first, m.second.second) == hipSuccess) out[m.first] += ms * 1e-3;
+        }
+    }
+};
+
+int d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    return BRM_OK;
+}
+
+int check_dev_err(const int* derr, hipStream_t s, const char* stage) {
+    int e = 0;
+    BRMC(d2h(&e, derr, sizeof(int), s));
+    if (e & DERR_INVALID) {
+        set_err("%s: malformed CSR input (row pointers, column range or unsorted/duplicate B columns)", stage);
+        return BRM_E_INVALID;
+    }
+    if (e) {
+        set_err("%s: internal consistency check failed (flags 0x%x)", stage, e);
+        return BRM_E_INTERNAL;
+    }
+    return BRM_OK;
+}
+
+int ceil_log2_host(int x) {
+    int h = 0;
+    while ((1LL << h) < (long long)x) ++h;
+    return h;
+}
+
+// ---------------------------------------------------------------- long rows
+int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, const int64_t* final_off,
+             hipStream_t s, Scratch& sc) {
+    if (nlong <= 0) return BRM_OK;
+    int64_t* ip_d;
+    int* len_d;
+    BRMC(sc.alloc(&ip_d, nlong, "long-row ip"));
+    BRMC(sc.alloc(&len_d, nlong, "long-row len"));
+    HIPC(launch_long_info(rows_d, nlong, a, ip_d, len_d, s));
+    std::vector<int64_t> ip(nlong);
+    std::vector<int> len(nlong), rows(nlong);
+    BRMC(d2h(ip.data(), ip_d, sizeof(int64_t) * nlong, s));
+    BRMC(d2h(len.data(), len_d, sizeof(int) * nlong, s));
+    BRMC(d2h(rows.data(), rows_d, sizeof(int) * nlong, s));
+    sc.free_now(ip_d);
+    sc.free_now(len_d);
+
+    // Wave partition by workspace budget.
+    int64_t budget = params().ws_budget_bytes;
+    if (budget <= 0) {
+        size_t fr = 0, tot = 0;
+        HIPC(hipMemGetInfo(&fr, &tot));
+        budget = (int64_t)(fr + dev_cached_bytes()) / 10 * 7;
+        if (budget < (int64_t)(256 << 20)) budget = 256 << 20;
+    }
+    auto row_bytes = [&](int k) {
+        const int H = ceil_log2_host(len[k]);
+        return 24 * ip[k] + 8 * ((int64_t)2 << H) + (int64_t)(sizeof(PairTask) + sizeof(LeafTask)) * (len[k] + 34);
+    };
+
+    int k0 = 0;
+    int* derr = o.err;
+    while (k0 < nlong) {
+        int k1 = k0;
+        int64_t used = 0;
+        while (k1 < nlong && (k1 == k0 || used + row_bytes(k1) <= budget)) used += row_bytes(k1++);
+        const int nw = k1 - k0;
+
+        std::vector<int64_t> wsb(nw), slb(nw);
+        int64_t ws_tot = 0, sl_tot = 0, leaf_cap = 0;
+        int maxH = 1;
+        std::vector<int64_t> capH(MAX_HEIGHT + 1, 0);
+        for (int k = 0; k < nw; ++k) {
+            wsb[k] = ws_tot;
+            slb[k] = sl_tot;
+            ws_tot += ip[k0 + k];
+            const int H = ceil_log2_host(len[k0 + k]);
+            sl_tot += (int64_t)2 << H;
+            leaf_cap += len[k0 + k];
+            maxH = std::max(maxH, H);
+            for (int h = 1; h <= std::max(H, 1); ++h) capH[h] += (int64_t)(len[k0 + k] >> h) + 2;
+        }
+        if (maxH > MAX_HEIGHT) {
+            set_err("long row with 2^%d lists exceeds the supported tree height", maxH);
+            return BRM_E_INVALID;
+        }
+        std::vector<int64_t> seg2(MAX_HEIGHT + 2, 0);
+        {
+            int64_t acc = 0;
+            for (int h = 0; h <= MAX_HEIGHT; ++h) {
+                seg2[h] = acc;
+                acc += capH[h];
+            }
+            seg2[MAX_HEIGHT + 1] = acc;
+        }
+        const int64_t ntask_cap = seg2[MAX_HEIGHT + 1];
+
+        Scratch wsc;
+        int *rows_w, *wcol0, *wcol1;
+        double *wval0, *wval1;
+        int64_t *wsb_d, *slb_d, *seg_d, *slots, *counters;
+        LeafTask* leaves;
+        PairTask* tasks;
+        BRMC(wsc.alloc(&rows_w, nw, "wave rows"));
+        BRMC(wsc.alloc(&wsb_d, nw, "wave ws_base"));
+        BRMC(wsc.alloc(&slb_d, nw, "wave slot_base"));
+        BRMC(wsc.alloc(&seg_d, MAX_HEIGHT + 2, "task segments"));
+        BRMC(wsc.alloc(&wcol0, ws_tot, "workspace"));
+        BRMC(wsc.alloc(&wcol1, ws_tot, "workspace"));
+        BRMC(wsc.alloc(&wval0, ws_tot, "workspace"));
+        BRMC(wsc.alloc(&wval1, ws_tot, "workspace"));
+        BRMC(wsc.alloc(&slots, sl_tot, "length slots"));
+        BRMC(wsc.alloc(&counters, CNT_N, "planner counters"));
+        BRMC(wsc.alloc(&leaves, leaf_cap, "leaf tasks"));
+        BRMC(wsc.alloc(&tasks, ntask_cap, "pair tasks"));
+        HIPC(hipMemcpyAsync(rows_w, rows.data() + k0, sizeof(int) * nw, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(wsb_d, wsb.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(slb_d, slb.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(seg_d, seg2.data(), sizeof(int64_t) * (MAX_HEIGHT + 2), hipMemcpyHostToDevice, s));
+        HIPC(launch_fill_i64(counters, CNT_N, 0, s));
+        HIPC(launch_fill_i64(slots, sl_tot, 0, s));
+
+        LongArgs la;
+        la.rows = rows_w;
+        la.nrows = nw;
+        la.ws_base = wsb_d;
+        la.slot_base = slb_d;
+        la.arpt = a.arpt;
+        la.acol = a.acol;
+        la.aval = a.aval;
+        la.brpt = a.brpt;
+        la.ipA = a.ipA;
+        la.final_off = final_off;
+        la.counters = counters;
+        la.leaves = leaves;
+        la.tasks = tasks;
+        la.task_seg = seg_d;
+        HIPC(launch_plan(la, params().leaf_cap, derr, s));
+        std::vector<int64_t> cnt(CNT_N);
+        BRMC(d2h(cnt.data(), counters, sizeof(int64_t) * CNT_N, s));
+        BRMC(check_dev_err(derr, s, "long-row planner"));
+
+        t_stats.long_waves += 1;
+        t_stats.leaf_tasks += cnt[CNT_LEAF];
+        t_stats.leaf_ip += cnt[CNT_LEAF_IP];
+        int* const wcol[2] = {wcol0, wcol1};
+        double* const wval[2] = {wval0, wval1};
+        HIPC(launch_leaf(leaves, cnt[CNT_LEAF], a, wcol, wval, slots, counters + CNT_LEAF_OUT, derr, s));
+
+        LevelBufs wb;
+        wb.ws_col[0] = wcol0;
+        wb.ws_col[1] = wcol1;
+        wb.ws_val[0] = wval0;
+        wb.ws_val[1] = wval1;
+        wb.slots = slots;
+        const int hmax = (int)cnt[CNT_MAXH];
+        for (int h = 1; h <= hmax; ++h) {
+            const int64_t n = cnt[CNT_TASK0 + h];
+            if (n <= 0) continue;
+            const PairTask* th = tasks + seg2[h];
+            Scratch lsc;
+            int64_t *lens, *ntile, *toff;
+            BRMC(lsc.alloc(&lens, 2 * n, "level lens"));
+            BRMC(lsc.alloc(&ntile, n, "level tiles"));
+            BRMC(lsc.alloc(&toff, n + 1, "level tile offsets"));
+            HIPC(launch_level_len(th, n, wb, lens, ntile, s));
+            HIPC(scan_i64(ntile, n, toff, s));
+            int64_t ntiles = 0;
+            BRMC(d2h(&ntiles, toff + n, sizeof(int64_t), s));
+            int64_t *split, *tcnt, *tout;
+            BRMC(lsc.alloc(&split, 2 * ntiles, "level splits"));
+            BRMC(lsc.alloc(&tcnt, ntiles, "tile counts"));
+            BRMC(lsc.alloc(&tout, ntiles + 1, "tile offsets"));
+            HIPC(launch_level_split(th, n, toff, ntiles, lens, a, wb, split, s));
+            HIPC(launch_level_count(th, n, toff, ntiles, lens, split, a, wb, tcnt, s));
+            HIPC(scan_i64(tcnt, ntiles, tout, s));
+            HIPC(launch_level_write(th, n, toff, ntiles, lens, split, tout, a, wb, o, s));
+            t_stats.level_tasks += n;
+            t_stats.level_tiles += ntiles;
+            if (params().profile) {
+                std::vector<int64_t> hl(2 * n);
+                int64_t outn = 0;
+                BRMC(d2h(hl.data(), lens, sizeof(int64_t) * 2 * n, s));
+                BRMC(d2h(&outn, tout + ntiles, sizeof(int64_t), s));
+                for (int64_t x : hl) t_stats.level_in += x;
+                t_stats.level_out += outn;
+            }
+            HIPC(hipStreamSynchronize(s));
+            dev_flush_deferred();
+        }
+        {
+            int64_t lo = 0;
+            BRMC(d2h(&lo, counters + CNT_LEAF_OUT, sizeof(int64_t), s));
+            t_stats.leaf_out += lo;
+        }
+        dev_flush_deferred();
+        k0 = k1;
+    }
+    return BRM_OK;
+}
+
+// ---------------------------------------------------------------- pipeline
+struct DevIn {
+    const int* arpt;
+    const int* acol;
+    const double* aval;
+    int64_t nnzA;
+    const int* brpt;
+    const int* bcol;
+    const double* bval;
+    int64_t nnzB;
+    int M, K, N;
+};
+
+struct DevOut {
+    int64_t* crpt;
+    int* ccol;
+    double* cval;
+    int64_t nnz;
+};
+
+int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph) {
+    if (in.M < 0 || in.K < 0 || in.N < 0 || in.nnzA < 0 || in.nnzB < 0) {
+        set_err("negative dimension");
+        return BRM_E_INVALID;
+    }
+    if (in.nnzA > INT32_MAX || in.nnzB > INT32_MAX) {
+        set_err("nnz(A) and nnz(B) must fit int32 row pointers");
+        return BRM_E_INVALID;
+    }
+    PhaseClock clk(s);
+    Scratch sc;
+    const int M = in.M;
+    memset(&t_stats, 0, sizeof(t_stats));
+
+    clk.begin(P_PRE);
+    int* derr;
+    int* bin_cnt;
+    int64_t *ipA, *iprow, *rowcnt, *crpt;
+    int* bins;
+    BRMC(sc.alloc(&derr, 1, "error flag"));
+    BRMC(sc.alloc(&bin_cnt, BIN_COUNT, "bin counters"));
+    BRMC(sc.alloc(&ipA, in.nnzA + 1, "ipA"));
+    BRMC(sc.alloc(&iprow, M, "row ip"));
+    BRMC(sc.alloc(&rowcnt, M, "row nnz"));
+    BRMC(sc.alloc(&crpt, (int64_t)M + 1, "crpt"));
+    BRMC(sc.alloc(&bins, 3 * (int64_t)M, "row bins"));
+    HIPC(launch_fill_i32(derr, 1, 0, s));
+    HIPC(launch_fill_i32(bin_cnt, BIN_COUNT, 0, s));
+
+    RowArgs a;
+    a.arpt = in.arpt;
+    a.acol = in.acol;
+    a.aval = in.aval;
+    a.brpt = in.brpt;
+    a.bcol = in.bcol;
+    a.bval = in.bval;
+    a.ipA = ipA;
+    a.M = in.M;
+    a.K = in.K;
+    a.N = in.N;
+    HIPC(launch_validate(a, in.nnzA, in.nnzB, derr, s));
+    BRMC(check_dev_err(derr, s, "validation"));
+    HIPC(launch_validate_sorted(in.brpt, in.bcol, in.K, derr, s));
+    BRMC(check_dev_err(derr, s, "validation"));
+
+    clk.begin(P_FLOP);
+    HIPC(scan_alen(in.acol, in.brpt, in.nnzA, ipA, s));
+
+    clk.begin(P_LB);
+    HIPC(launch_rowstats(a, params().med_cap, params().force_long, iprow, rowcnt, bins, bin_cnt, s));
+    int bc[BIN_COUNT];
+    int64_t total_ip = 0;
+    BRMC(d2h(bc, bin_cnt, sizeof(bc), s));
+    BRMC(d2h(&total_ip, ipA + in.nnzA, sizeof(int64_t), s));
+    for (int b = 0; b < BIN_COUNT; ++b) t_stats.rows_by_tier[b] = bc[b];
+    t_stats.total_ip = total_ip;
+    const int* rows_short = bins;
+    const int* rows_med = bins + M;
+    const int* rows_long = bins + 2 * (int64_t)M;
+
+    OutArgs o{};
+    o.err = derr;
+    int* ccol = nullptr;
+    double* cval = nullptr;
+    int64_t nnzC = 0;
+
+    if (!upper) {
+        clk.begin(P_SYM);
+        HIPC(launch_short_count(rows_short, bc[BIN_SHORT], a, rowcnt, s));
+        HIPC(launch_medium_count(rows_med, bc[BIN_MEDIUM], a, rowcnt, s));
+        HIPC(launch_long_count(rows_long, bc[BIN_LONG], a, rowcnt, s));
+
+        clk.begin(P_PNNZ);
+        HIPC(scan_i64(rowcnt, M, crpt, s));
+        BRMC(d2h(&nnzC, crpt + M, sizeof(int64_t), s));
+        if (nnzC > max_nnz) {
+            set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
+            return BRM_E_OVERFLOW;
+        }
+
+        clk.begin(P_ALLOC);
+        BRMC(sc.alloc(&ccol, nnzC, "C col"));
+        BRMC(sc.alloc(&cval, nnzC, "C val"));
+
+        clk.begin(P_COMPUTE);
+        o.upper = 0;
+        o.crpt = crpt;
+        o.ccol = ccol;
+        o.cval = cval;
+        HIPC(launch_short_numeric(rows_short, bc[BIN_SHORT], a, o, s));
+        HIPC(launch_medium_numeric(rows_med, bc[BIN_MEDIUM], a, o, s));
+        BRMC(run_long(a, rows_long, bc[BIN_LONG], o, crpt, s, sc));
+        BRMC(check_dev_err(derr, s, "numeric"));
+    } else {
+        clk.begin(P_ALLOC);
+        int* bcol_bar;
+        double* bval_bar;
+        BRMC(sc.alloc(&bcol_bar, total_ip, "C-bar col"));
+        BRMC(sc.alloc(&bval_bar, total_ip, "C-bar val"));
+
+        clk.begin(P_COMPUTE);
+        o.upper = 1;
+        o.ccol = bcol_bar;
+        o.cval = bval_bar;
+        o.rowlen = rowcnt;
+        HIPC(launch_short_numeric(rows_short, bc[BIN_SHORT], a, o, s));
+        HIPC(launch_medium_numeric(rows_med, bc[BIN_MEDIUM], a, o, s));
+        BRMC(run_long(a, rows_long, bc[BIN_LONG], o, nullptr, s, sc));
+        BRMC(check_dev_err(derr, s, "numeric"));
+
+        clk.begin(P_PNNZ);
+        HIPC(scan_i64(rowcnt, M, crpt, s));
+        BRMC(d2h(&nnzC, crpt + M, sizeof(int64_t), s));
+        if (nnzC > max_nnz) {
+            set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
+            return BRM_E_OVERFLOW;
+        }
+        clk.begin(P_ALLOC);
+        BRMC(sc.alloc(&ccol, nnzC, "C col"));
+        BRMC(sc.alloc(&cval, nnzC, "C val"));
+        clk.begin(P_COPY);
+        HIPC(launch_compact(crpt, M, a, bcol_bar, bval_bar, ccol, cval, nnzC, s));
+    }
+    clk.begin(P_CLEAN);
+    HIPC(hipStreamSynchronize(s));
+    clk.end();
+    HIPC(hipStreamSynchronize(s));
+    if (ph) clk.collect(ph);
+
+    t_stats.nnz_c = nnzC;
+    sc.release(crpt);
+    sc.release(ccol);
+    sc.release(cval);
+    out->crpt = crpt;
+    out->ccol = ccol;
+    out->cval = cval;
+    out->nnz = nnzC;
+    dev_flush_deferred();
+    return BRM_OK;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void fill_precise(brm_precise_timing* t, const double* ph, double total, double h2d, double d2h_) {
+    if (!t) return;
+    t->pre_allocate = ph[P_PRE];
+    t->compute_flop = ph[P_FLOP];
+    t->prefix_sum_flop = ph[P_PFLOP];
+    t->load_balance = ph[P_LB];
+    t->symbolic = ph[P_SYM];
+    t->prefix_sum_nnz = ph[P_PNNZ];
+    t->allocate_C = ph[P_ALLOC];
+    t->compute = ph[P_COMPUTE];
+    t->cleanup = ph[P_CLEAN];
+    t->total = total;
+    t->h2d = h2d;
+    t->d2h = d2h_;
+}
+
+void fill_upper(brm_upper_timing* t, const double* ph, double total, double h2d, double d2h_) {
+    if (!t) return;
+    t->thread = 0;
+    t->pre_allocate = ph[P_PRE];
+    t->compute_flop = ph[P_FLOP];
+    t->prefix_sum_flop = ph[P_PFLOP];
+    t->load_balance = ph[P_LB];
+    t->allocate_hook = 0;
+    t->compute = ph[P_COMPUTE];
+    t->prefix_sum_nnz = ph[P_PNNZ];
+    t->allocate_C = ph[P_ALLOC];
+    t->copy = ph[P_COPY];
+    t->cleanup = ph[P_CLEAN];
+    t->total = total;
+    t->h2d = h2d;
+    t->d2h = d2h_;
+}
+
+int use_device() {
+    HIPC(hipSetDevice(t_device));
+    return BRM_OK;
+}
+
+// Host-pointer entry shared by the four host variants.
+template <typename RptT>
+int host_entry(const int* arpt, const int* acol, const double* aval, const int* brpt, const int* bcol,
+               const double* bval, RptT** crpt_, int** ccol_, double** cval_, int M, int K, int N, RptT* cnnz_,
+               bool upper, double* ph, double* total, double* h2d, double* d2h_) {
+    if (!crpt_ || !ccol_ || !cval_ || !cnnz_) {
+        set_err("null output pointer");
+        return BRM_E_INVALID;
+    }
+    if (M < 0 || K < 0 || N < 0 || (M > 0 && !arpt) || (K > 0 && !brpt)) {
+        set_err("invalid dimensions or null row pointer");
+        return BRM_E_INVALID;
+    }
+    const double t0 = now_s();
+    BRMC(use_device());
+    const int64_t nnzA = M > 0 ? arpt[M] : 0;
+    const int64_t nnzB = K > 0 ? brpt[K] : 0;
+    if (nnzA < 0 || nnzB < 0 || (nnzA > 0 && (!acol || !aval)) || (nnzB > 0 && (!bcol || !bval))) {
+        set_err("invalid nnz or null column/value pointer");
+        return BRM_E_INVALID;
+    }
+    hipStream_t s = nullptr;
+    Scratch sc;
+    int *darpt, *dacol, *dbrpt, *dbcol;
+    double *daval, *dbval;
+    BRMC(sc.alloc(&darpt, (int64_t)M + 1, "A rpt"));
+    BRMC(sc.alloc(&dacol, nnzA, "A col"));
+    BRMC(sc.alloc(&daval, nnzA, "A val"));
+    BRMC(sc.alloc(&dbrpt, (int64_t)K + 1, "B rpt"));
+    BRMC(sc.alloc(&dbcol, nnzB, "B col"));
+    BRMC(sc.alloc(&dbval, nnzB, "B val"));
+    if (M == 0) {
+        int z = 0;
+        HIPC(hipMemcpyAsync(darpt, &z, sizeof(int), hipMemcpyHostToDevice, s));
+    } else {
+        HIPC(hipMemcpyAsync(darpt, arpt, sizeof(int) * ((size_t)M + 1), hipMemcpyHostToDevice, s));
+    }
+    if (K == 0) {
+        int z = 0;
+        HIPC(hipMemcpyAsync(dbrpt, &z, sizeof(int), hipMemcpyHostToDevice, s));
+    } else {
+        HIPC(hipMemcpyAsync(dbrpt, brpt, sizeof(int) * ((size_t)K + 1), hipMemcpyHostToDevice, s));
+    }
+    if (nnzA > 0) {
+        HIPC(hipMemcpyAsync(dacol, acol, sizeof(int) * nnzA, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(daval, aval, sizeof(double) * nnzA, hipMemcpyHostToDevice, s));
+    }
+    if (nnzB > 0) {
+        HIPC(hipMemcpyAsync(dbcol, bcol, sizeof(int) * nnzB, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(dbval, bval, sizeof(double) * nnzB, hipMemcpyHostToDevice, s));
+    }
+    HIPC(hipStreamSynchronize(s));
+    const double t1 = now_s();
+
+    DevIn in{darpt, dacol, daval, nnzA, dbrpt, dbcol, dbval, nnzB, M, K, N};
+    DevOut out{};
+    const int64_t max_nnz = sizeof(RptT) == 4 ? (int64_t)INT32_MAX : INT64_MAX;
+    BRMC(run_spgemm(in, upper, max_nnz, s, &out, ph));
+    Scratch outs;
+    outs.bufs = {out.crpt, out.ccol, out.cval};
+
+    const double t2 = now_s();
+    RptT* hrpt = new (std::nothrow) RptT[(size_t)M + 1];
+    int* hcol = new (std::nothrow) int[(size_t)(out.nnz > 0 ? out.nnz : 1)];
+    double* hval = new (std::nothrow) double[(size_t)(out.nnz > 0 ? out.nnz : 1)];
+    if (!hrpt || !hcol || !hval) {
+        delete[] hrpt;
+        delete[] hcol;
+        delete[] hval;
+        set_err("host allocation of C failed");
+        return BRM_E_NOMEM;
+    }
+    int st = BRM_OK;
+    if (sizeof(RptT) == 8) {
+        st = d2h(hrpt, out.crpt, sizeof(int64_t) * ((size_t)M + 1), s);
+    } else {
+        int* tmp;
+        st = sc.alloc(&tmp, (int64_t)M + 1, "crpt32");
+        if (st == BRM_OK) st = hip_status(launch_rpt_to_i32(out.crpt, (int64_t)M + 1, tmp, s), "crpt32");
+        if (st == BRM_OK) st = d2h(hrpt, tmp, sizeof(int) * ((size_t)M + 1), s);
+    }
+    if (st == BRM_OK && out.nnz > 0) {
+        st = d2h(hcol, out.ccol, sizeof(int) * out.nnz, s);
+        if (st == BRM_OK) st = d2h(hval, out.cval, sizeof(double) * out.nnz, s);
+    }
+    if (st != BRM_OK) {
+        delete[] hrpt;
+        delete[] hcol;
+        delete[] hval;
+        return st;
+    }
+    const double t3 = now_s();
+    *crpt_ = hrpt;
+    *ccol_ = hcol;
+    *cval_ = hval;
+    *cnnz_ = (RptT)out.nnz;
+    *h2d = t1 - t0;
+    *d2h_ = t3 - t2;
+    *total = t3 - t0;
+    dev_flush_deferred();
+    return BRM_OK;
+}
+
+template <typename Tm>
+int dev_entry(const int* arpt, const int* acol, const double* aval, int64_t nnzA, const int* brpt, const int* bcol,
+              const double* bval, int64_t nnzB, int M, int K, int N, int64_t** crpt, int** ccol, double** cval,
+              int64_t* cnnz, bool upper, Tm* timing, void* stream) {
+    if (!crpt || !ccol || !cval || !cnnz || (M > 0 && !arpt) || (K > 0 && !brpt)) {
+        set_err("null pointer argument");
+        return BRM_E_INVALID;
+    }
+    const double t0 = now_s();
+    BRMC(use_device());
+    DevIn in{arpt, acol, aval, nnzA, brpt, bcol, bval, nnzB, M, K, N};
+    DevOut out{};
+    double ph[P_N] = {0};
+    BRMC(run_spgemm(in, upper, INT64_MAX, (hipStream_t)stream, &out, ph));
+    *crpt = out.crpt;
+    *ccol = out.ccol;
+    *cval = out.cval;
+    *cnnz = out.nnz;
+    const double total = now_s() - t0;
+    if constexpr (std::is_same<Tm, brm_precise_timing>::value) fill_precise(timing, ph, total, 0, 0);
+    else fill_upper(timing, ph, total, 0, 0);
+    return BRM_OK;
+}
+
+}  // namespace
+}  // namespace brm
+
+using namespace brm;
+
+extern "C" {
+
+int brm_init(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        (void)hipGetLastError();
+        set_err("no HIP device %d", device);
+        return BRM_E_DEVICE;
+    }
+    t_device = device;
+    return hip_status(hipSetDevice(device), "hipSetDevice");
+}
+
+const char* brm_last_error(void) { return t_err.c_str(); }
+
+const char* brm_version(void) { return "brmerge-mi355x 0.1 (gfx950)"; }
+
+int brm_spgemm_precise_i32(const int* arpt, const int* acol, const double* aval, const int* brpt, const int* bcol,
+                           const double* bval, int** crpt, int** ccol, double** cval, int M, int K, int N, int* cnnz,
+                           brm_precise_timing* timing) {
+    double ph[P_N] = {0}, tot = 0, h2d = 0, d2h_ = 0;
+    int st = host_entry<int>(arpt, acol, aval, brpt, bcol, bval, crpt, ccol, cval, M, K, N, cnnz, false, ph, &tot,
+                             &h2d, &d2h_);
+    if (st == BRM_OK) fill_precise(timing, ph, tot, h2d, d2h_);
+    dev_flush_deferred();
+    return st;
+}
+
+int brm_spgemm_upper_i32(const int* arpt, const int* acol, const double* aval, const int* brpt, const int* bcol,
+                         const double* bval, int** crpt, int** ccol, double** cval, int M, int K, int N, int* cnnz,
+                         brm_upper_timing* timing) {
+    double ph[P_N] = {0}, tot = 0, h2d = 0, d2h_ = 0;
+    int st = host_entry<int>(arpt, acol, aval, brpt, bcol, bval, crpt, ccol, cval, M, K, N, cnnz, true, ph, &tot,
+                             &h2d, &d2h_);
+    if (st == BRM_OK) fill_upper(timing, ph, tot, h2d, d2h_);
+    dev_flush_deferred();
+    return st;
+}
+
+int brm_spgemm_precise_i64(const int* arpt, const int* acol, const double* aval, const int* brpt, const int* bcol,
+                           const double* bval, int64_t** crpt, int** ccol, double** cval, int M, int K, int N,
+                           int64_t* cnnz, brm_precise_timing* timing) {
+    double ph[P_N] = {0}, tot = 0, h2d = 0, d2h_ = 0;
+    int st = host_entry<int64_t>(arpt, acol, aval, brpt, bcol, bval, crpt, ccol, cval, M, K, N, cnnz, false, ph,
+                                 &tot, &h2d, &d2h_);
+    if (st == BRM_OK) fill_precise(timing, ph, tot, h2d, d2h_);
+    dev_flush_deferred();
+    return st;
+}
+
+int brm_spgemm_upper_i64(const int* arpt, const int* acol, const double* aval, const int* brpt, const int* bcol,
+                         const double* bval, int64_t** crpt, int** ccol, double** cval, int M, int K, int N,
+                         int64_t* cnnz, brm_upper_timing* timing) {
+    double ph[P_N] = {0}, tot = 0, h2d = 0, d2h_ = 0;
+    int st = host_entry<int64_t>(arpt, acol, aval, brpt, bcol, bval, crpt, ccol, cval, M, K, N, cnnz, true, ph,
+                                 &tot, &h2d, &d2h_);
+    if (st == BRM_OK) fill_upper(timing, ph, tot, h2d, d2h_);
+    dev_flush_deferred();
+    return st;
+}
+
+void brm_host_free_i32(int* p) { delete[] p; }
+void brm_host_free_i64(int64_t* p) { delete[] p; }
+void brm_host_free_f64(double* p) { delete[] p; }
+
+int brm_spgemm_precise_dev(const int* arpt, const int* acol, const double* aval, int64_t nnzA, const int* brpt,
+                           const int* bcol, const double* bval, int64_t nnzB, int M, int K, int N, int64_t** crpt,
+                           int** ccol, double** cval, int64_t* cnnz, brm_precise_timing* timing, void* stream) {
+    int st = dev_entry(arpt, acol, aval, nnzA, brpt, bcol, bval, nnzB, M, K, N, crpt, ccol, cval, cnnz, false,
+                       timing, stream);
+    dev_flush_deferred();
+    return st;
+}
+
+int brm_spgemm_upper_dev(const int* arpt, const int* acol, const double* aval, int64_t nnzA, const int* brpt,
+                         const int* bcol, const double* bval, int64_t nnzB, int M, int K, int N, int64_t** crpt,
+                         int** ccol, double** cval, int64_t* cnnz, brm_upper_timing* timing, void* stream) {
+    int st = dev_entry(arpt, acol, aval, nnzA, brpt, bcol, bval, nnzB, M, K, N, crpt, ccol, cval, cnnz, true,
+                       timing, stream);
+    dev_flush_deferred();
+    return st;
+}
+
+void* brm_dev_malloc(uint64_t bytes) {
+    if (use_device() != BRM_OK) return nullptr;
+    return dev_alloc((size_t)bytes);
+}
+void brm_dev_free(void* p) { dev_free(p); }
+void brm_dev_trim(void) { dev_trim(); }
+
+int brm_flop_count_dev(const int* arpt, const int* acol, int64_t nnzA, const int* brpt, int M, int K,
+                       int64_t* row_ip, int64_t* total, void* stream) {
+    if (!total || (M > 0 && !arpt) || (K > 0 && !brpt) || M < 0 || nnzA < 0) {
+        set_err("invalid argument");
+        return BRM_E_INVALID;
+    }
+    BRMC(use_device());
+    hipStream_t s = (hipStream_t)stream;
+    Scratch sc;
+    int64_t* ipA;
+    BRMC(sc.alloc(&ipA, nnzA + 1, "ipA"));
+    HIPC(scan_alen(acol, brpt, nnzA, ipA, s));
+    if (row_ip && M > 0) {
+        RowArgs a{};
+        a.arpt = arpt;
+        a.ipA = ipA;
+        a.M = M;
+        // row_ip[i] = ipA[arpt[i+1]] - ipA[arpt[i]] via the row-stat kernel's
+        // helper: reuse the long-info kernel over all rows.
+        int* rows;
+        int* lens;
+        BRMC(sc.alloc(&rows, M, "rows"));
+        BRMC(sc.alloc(&lens, M, "lens"));
+        std::vector<int> ids(M);
+        for (int i = 0; i < M; ++i) ids[i] = i;
+        HIPC(hipMemcpyAsync(rows, ids.data(), sizeof(int) * M, hipMemcpyHostToDevice, s));
+        HIPC(launch_long_info(rows, M, a, row_ip, lens, s));
+    }
+    int st = d2h(total, ipA + nnzA, sizeof(int64_t), s);
+    dev_flush_deferred();
+    return st;
+}
+
+int brm_flop_count(const int* arpt, const int* acol, const int* brpt, int M, int K, int64_t* total) {
+    if (!total || M < 0 || K < 0 || (M > 0 && !arpt) || (K > 0 && !brpt)) {
+        set_err("invalid argument");
+        return BRM_E_INVALID;
+    }
+    BRMC(use_device());
+    const int64_t nnzA = M > 0 ? arpt[M] : 0;
+    Scratch sc;
+    int *dacol, *dbrpt;
+    BRMC(sc.alloc(&dacol, nnzA, "A col"));
+    BRMC(sc.alloc(&dbrpt, (int64_t)K + 1, "B rpt"));
+    hipStream_t s = nullptr;
+    if (nnzA > 0) HIPC(hipMemcpyAsync(dacol, acol, sizeof(int) * nnzA, hipMemcpyHostToDevice, s));
+    if (K > 0) HIPC(hipMemcpyAsync(dbrpt, brpt, sizeof(int) * ((size_t)K + 1), hipMemcpyHostToDevice, s));
+    for (int64_t j = 0; j < nnzA; ++j) {
+        if (acol[j] < 0 || acol[j] >= K) {
+            set_err("A column index out of range");
+            return BRM_E_INVALID;
+        }
+    }
+    return brm_flop_count_dev(nullptr, dacol, nnzA, dbrpt, 0, K, nullptr, total, s);
+}
+
+int brm_set_param(const char* name, int64_t value) {
+    if (!name) return BRM_E_INVALID;
+    Params& p = params();
+    if (!strcmp(name, "ws_budget_bytes")) { if (value < 0) return BRM_E_INVALID; p.ws_budget_bytes = value; }
+    else if (!strcmp(name, "med_cap")) { if (value < SHORT_MAX || value > MED_CAP) return BRM_E_INVALID; p.med_cap = (int)value; }
+    else if (!strcmp(name, "leaf_cap")) { if (value < 1 || value > MED_CAP) return BRM_E_INVALID; p.leaf_cap = (int)value; }
+    else if (!strcmp(name, "force_long")) { p.force_long = value ? 1 : 0; }
+    else if (!strcmp(name, "profile")) { p.profile = value ? 1 : 0; }
+    else return BRM_E_INVALID;
+    return BRM_OK;
+}
+
+int64_t brm_get_param(const char* name) {
+    if (!name) return -1;
+    Params& p = params();
+    if (!strcmp(name, "ws_budget_bytes")) return p.ws_budget_bytes;
+    if (!strcmp(name, "med_cap")) return p.med_cap;
+    if (!strcmp(name, "leaf_cap")) return p.leaf_cap;
+    if (!strcmp(name, "force_long")) return p.force_long;
+    if (!strcmp(name, "profile")) return p.profile;
+    return -1;
+}
+
+void brm_debug_fail_alloc_after(int64_t n) { dev_set_fail_after(n); }
+
+int brm_last_stats(brm_stats* out) {
+    if (!out) return BRM_E_INVALID;
+    *out = t_stats;
+    return BRM_OK;
+}
+
+int brm_memcpy_h2d(void* dst_dev, const void* src_host, uint64_t bytes) {
+    if (bytes == 0) return BRM_OK;
+    if (!dst_dev || !src_host) return BRM_E_INVALID;
+    BRMC(use_device());
+    HIPC(hipMemcpy(dst_dev, src_host, bytes, hipMemcpyHostToDevice));
+    return BRM_OK;
+}
+
+int brm_memcpy_d2h(void* dst_host, const void* src_dev, uint64_t bytes) {
+    if (bytes == 0) return BRM_OK;
+    if (!dst_host || !src_dev) return BRM_E_INVALID;
+    BRMC(use_device());
+    HIPC(hipMemcpy(dst_host, src_dev, bytes, hipMemcpyDeviceToHost));
+    return BRM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// CSR assembly helpers.
+//
+// Upper compaction: rows merged into C-bar at their product offsets
+// (ipA[arpt[row]], the reference's scan_floprC, spgemm.h:399-400) are packed
+// into C at crpt[row] (spgemm.h:427-445).  Each workgroup owns a contiguous
+// range of C, locates the row of its first element by binary search on crpt
+// and walks rows forward, so reads and writes are both coalesced.
+#include "brm_device.h"
+#include "kernels.h"
+
+namespace brm {
+namespace {
+
+constexpr int CP_NT = 256;
+constexpr int CP_ITEMS = 16;
+constexpr int64_t CP_TILE = CP_NT * CP_ITEMS;
+
+__global__ __launch_bounds__(CP_NT) void k_compact(const int64_t* __restrict__ crpt, int M, RowArgs a,
+                                                   const int* __restrict__ cbar_col,
+                                                   const double* __restrict__ cbar_val, int* __restrict__ ccol,
+                                                   double* __restrict__ cval, int64_t nnzC) {
+    __shared__ int r0s;
+    const int64_t q0 = (int64_t)blockIdx.x * CP_TILE;
+    const int64_t q1 = min(nnzC, q0 + CP_TILE);
+    if (threadIdx.x == 0) r0s = upper_idx(crpt, 0, M, q0);
+    __syncthreads();
+    int r = r0s;
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += CP_NT) {
+        r = gallop_idx(crpt, r, M, q);
+        const int64_t src = a.ipA[a.arpt[r]] + (q - crpt[r]);
+        ccol[q] = cbar_col[src];
+        cval[q] = cbar_val[src];
+    }
+}
+
+__global__ void k_rpt_to_i32(const int64_t* __restrict__ in, int64_t n, int* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = (int)in[i];
+}
+
+__global__ void k_fill_i64(int64_t* p, int64_t n, int64_t v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_i32(int* p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+hipError_t launch_compact(const int64_t* crpt, int M, const RowArgs& a, const int* cbar_col, const double* cbar_val,
+                          int* ccol, double* cval, int64_t nnzC, hipStream_t s) {
+    if (nnzC <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_compact", s);
+        k_compact<<<nblk(nnzC, (int)CP_TILE), CP_NT, 0, s>>>(crpt, M, a, cbar_col, cbar_val, ccol, cval, nnzC);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rpt_to_i32(const int64_t* in, int64_t n, int* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_rpt_to_i32", s);
+        k_rpt_to_i32<<<nblk(n, 256), 256, 0, s>>>(in, n, out);
+    }
+    return hipGetLastError();
+}
+
+// Fills use kernels rather than hipMemsetAsync so that the sequence stays
+// valid inside captured HIP graphs.
+hipError_t launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_fill_i64", s);
+        k_fill_i64<<<nblk(n, 256), 256, 0, s>>>(p, n, v);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int* p, int64_t n, int v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_fill_i32", s);
+        k_fill_i32<<<nblk(n, 256), 256, 0, s>>>(p, n, v);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace brm

@@ -1,0 +1,133 @@
+// Device-side building blocks shared by the BRMerge kernels (gfx950, wave64).
+//
+// Everything here is plain HIP C++ for CDNA4: 64-lane wavefronts, LDS scans,
+// ds_bpermute/ds_permute lane exchange.  Value arithmetic uses explicit
+// round-to-nearest multiply and add (no contraction into FMA), because the
+// reference stores each product aval*bval before any add
+// (BRMerge-SpGEMM/inc/spgemm.h:621,653).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define BRM_WAVE 64
+
+namespace brm {
+
+__device__ __forceinline__ double fmul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ double fadd(double a, double b) { return __dadd_rn(a, b); }
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// ---------------------------------------------------------------- wave scans
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < BRM_WAVE; d <<= 1) {
+        T y = __shfl_up(x, d, BRM_WAVE);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+    for (int d = BRM_WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, BRM_WAVE);
+    return x;
+}
+
+// ---------------------------------------------------------------- block scans
+// Exclusive scan over the block; returns this thread's exclusive prefix and
+// the block total in *total.  `sm` must hold NT/64 + 1 elements of T.
+template <int NT, typename T>
+__device__ __forceinline__ T block_excl_scan(T x, T* sm, T* total) {
+    constexpr int NW = NT / BRM_WAVE;
+    const int lane = lane_id();
+    const int w = threadIdx.x / BRM_WAVE;
+    T inc = wave_incl_scan(x);
+    if (lane == BRM_WAVE - 1) sm[w] = inc;
+    __syncthreads();
+    if (threadIdx.x < BRM_WAVE) {
+        T v = (lane < NW) ? sm[lane] : T(0);
+        T vi = wave_incl_scan(v);
+        if (lane < NW) sm[lane] = vi - v;
+        if (lane == NW - 1) sm[NW] = vi;
+    }
+    __syncthreads();
+    T res = sm[w] + inc - x;
+    *total = sm[NW];
+    __syncthreads();
+    return res;
+}
+
+template <int NT, typename T>
+__device__ __forceinline__ T block_sum(T x, T* sm) {
+    T tot;
+    (void)block_excl_scan<NT, T>(x, sm, &tot);
+    return tot;
+}
+
+// ---------------------------------------------------------------- searches
+// Largest k in [lo, hi) with a[k] <= key (a nondecreasing, a[lo] <= key).
+template <typename T, typename K>
+__device__ __forceinline__ int upper_idx(const T* a, int lo, int hi, K key) {
+    // invariant: a[lo] <= key, answer in [lo, hi)
+    int n = hi - lo;
+    while (n > 1) {
+        int half = n >> 1;
+        int mid = lo + half;
+        if ((K)a[mid] <= key) { lo = mid; n -= half; } else { n = half; }
+    }
+    return lo;
+}
+
+// Galloping variant: start from hint (a[hint] <= key), search forward.
+template <typename T, typename K>
+__device__ __forceinline__ int gallop_idx(const T* a, int hint, int hi, K key) {
+    int lo = hint;
+    int step = 1;
+    int probe = lo + step;
+    while (probe < hi && (K)a[probe] <= key) {
+        lo = probe;
+        step <<= 1;
+        probe = lo + step;
+    }
+    int end = probe < hi ? probe : hi;
+    return upper_idx(a, lo, end, key);
+}
+
+// Merge-path split of the stable (left-first on ties) merge of two sorted,
+// duplicate-free column lists L[0..nL) and R[0..nR) at diagonal d: returns
+// i (taken from L); j = d - i.  If L[i-1] == R[j] the pair would straddle the
+// cut, so j is advanced by one and both copies fall in the earlier part.
+template <typename CL, typename CR>
+__device__ __forceinline__ void merge_split(CL colL, int nL, CR colR, int nR, int d, int* oi, int* oj) {
+    int lo = d - nR; if (lo < 0) lo = 0;
+    int hi = d < nL ? d : nL;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (colL(mid) <= colR(d - 1 - mid)) lo = mid + 1; else hi = mid;
+    }
+    int i = lo, j = d - lo;
+    if (i > 0 && j < nR && colL(i - 1) == colR(j)) j++;
+    *oi = i;
+    *oj = j;
+}
+
+template <typename CL, typename CR>
+__device__ __forceinline__ void merge_split64(CL colL, int64_t nL, CR colR, int64_t nR, int64_t d,
+                                              int64_t* oi, int64_t* oj) {
+    int64_t lo = d - nR; if (lo < 0) lo = 0;
+    int64_t hi = d < nL ? d : nL;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (colL(mid) <= colR(d - 1 - mid)) lo = mid + 1; else hi = mid;
+    }
+    int64_t i = lo, j = d - lo;
+    if (i > 0 && j < nR && colL(i - 1) == colR(j)) j++;
+    *oi = i;
+    *oj = j;
+}
+
+}  // namespace brm

@@ -1,0 +1,145 @@
+// Input validation, per-row intermediate-product counts and row binning.
+//
+// The per-A-entry product counts and their prefix (scan_alen in scan.hip)
+// replace compute_flop (BRMerge-SpGEMM/inc/spgemm.h:31-56); the row counts
+// are differences of that prefix.  Binning replaces the reference's
+// flop-balanced static_section partition (spgemm.h:247-256, 494-503): instead
+// of contiguous ranges per thread, rows are sorted into tiers by product
+// count, and each tier runs the kernel shape that suits it.  Binning never
+// changes results (each row is merged by the same tree whatever its tier).
+//
+// Validation replaces the checks the reference's reader performs on load
+// (src/CSR.cpp:454-477) and guards the kernels against out-of-range indices.
+#include "brm_device.h"
+#include "kernels.h"
+
+namespace brm {
+namespace {
+
+constexpr int NT = 256;
+
+__global__ void k_validate_rpt(const int* __restrict__ rpt, int n, int64_t nnz, int* err) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i > n) return;
+    const int v = rpt[i];
+    bool bad = (i == 0 && v != 0) || (i == n && (int64_t)v != nnz) || (i < n && v > rpt[i + 1]);
+    if (bad) atomicOr(err, (int)DERR_INVALID);
+}
+
+__global__ void k_validate_range(const int* __restrict__ col, int64_t nnz, int limit, int* err) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= nnz) return;
+    const int v = col[i];
+    if (v < 0 || v >= limit) atomicOr(err, (int)DERR_INVALID);
+}
+
+// Strictly increasing columns within each row; one wavefront per row.
+__global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __restrict__ col, int n, int* err) {
+    const int r = blockIdx.x * (NT / BRM_WAVE) + threadIdx.x / BRM_WAVE;
+    if (r >= n) return;
+    const int b = rpt[r], e = rpt[r + 1];
+    bool bad = false;
+    for (int k = b + lane_id(); k + 1 < e; k += BRM_WAVE) bad |= col[k] >= col[k + 1];
+    if (__ballot(bad) != 0ull && lane_id() == 0) atomicOr(err, (int)DERR_INVALID);
+}
+
+__global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int force_long, int64_t* __restrict__ iprow,
+                                                 int64_t* __restrict__ rowcnt, int* __restrict__ bins,
+                                                 int* __restrict__ bin_cnt) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    int bin = -1;
+    if (i < a.M) {
+        const int a0 = a.arpt[i], a1 = a.arpt[i + 1];
+        const int L = a1 - a0;
+        const int64_t ip = a.ipA[a1] - a.ipA[a0];
+        iprow[i] = ip;
+        if (ip == 0) {
+            bin = BIN_EMPTY;
+            rowcnt[i] = 0;
+        } else if (force_long) {
+            bin = BIN_LONG;
+        } else if (ip <= SHORT_MAX) {
+            bin = BIN_SHORT;
+        } else if (ip <= med_cap && L <= MED_LMAX) {
+            bin = BIN_MEDIUM;
+        } else {
+            bin = BIN_LONG;
+        }
+    }
+    const int lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int b = 0; b < BIN_COUNT; ++b) {
+        const uint64_t m = __ballot(bin == b);
+        if (m == 0ull) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        int pos = 0;
+        if (lane == leader) pos = atomicAdd(&bin_cnt[b], __popcll(m));
+        pos = __shfl(pos, leader, BRM_WAVE);
+        if (b > 0 && bin == b) bins[(int64_t)(b - 1) * a.M + pos + __popcll(m & below)] = i;
+    }
+}
+
+__global__ void k_long_info(const int* __restrict__ rows, int n, RowArgs a, int64_t* ip_out, int* len_out) {
+    const int k = blockIdx.x * NT + threadIdx.x;
+    if (k >= n) return;
+    const int r = rows[k];
+    const int a0 = a.arpt[r], a1 = a.arpt[r + 1];
+    ip_out[k] = a.ipA[a1] - a.ipA[a0];
+    len_out[k] = a1 - a0;
+}
+
+inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+hipError_t launch_validate(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* err, hipStream_t s) {
+    // Row pointers first: the sortedness check indexes columns through them.
+    {
+        ProfScope ps_("k_validate_rpt", s);
+        k_validate_rpt<<<nblk((int64_t)a.M + 1, NT), NT, 0, s>>>(a.arpt, a.M, nnzA, err);
+    }
+    {
+        ProfScope ps_("k_validate_rpt", s);
+        k_validate_rpt<<<nblk((int64_t)a.K + 1, NT), NT, 0, s>>>(a.brpt, a.K, nnzB, err);
+    }
+    if (nnzA > 0) {
+        ProfScope ps_("k_validate_range", s);
+        k_validate_range<<<nblk(nnzA, NT), NT, 0, s>>>(a.acol, nnzA, a.K, err);
+    }
+    if (nnzB > 0) {
+        ProfScope ps_("k_validate_range", s);
+        k_validate_range<<<nblk(nnzB, NT), NT, 0, s>>>(a.bcol, nnzB, a.N, err);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* err, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_validate_sorted", s);
+        k_validate_sorted<<<nblk(n, NT / BRM_WAVE), NT, 0, s>>>(rpt, col, n, err);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rowstats(const RowArgs& a, int med_cap, int force_long, int64_t* iprow, int64_t* rowcnt,
+                           int* bins, int* bin_cnt, hipStream_t s) {
+    if (a.M <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_rowstats", s);
+        k_rowstats<<<nblk(a.M, NT), NT, 0, s>>>(a, med_cap, force_long, iprow, rowcnt, bins, bin_cnt);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_long_info(const int* rows, int n, const RowArgs& a, int64_t* ip_out, int* len_out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_long_info", s);
+        k_long_info<<<nblk(n, NT), NT, 0, s>>>(rows, n, a, ip_out, len_out);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace brm

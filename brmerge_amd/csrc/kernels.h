@@ -1,0 +1,75 @@
+// Kernel argument blocks and launcher declarations shared between the
+// kernel translation units and the host orchestration (api.hip).
+#pragma once
+#include "brm_internal.h"
+
+namespace brm {
+
+// Read-only view of A and B (device pointers).
+struct RowArgs {
+    const int* arpt;
+    const int* acol;
+    const double* aval;
+    const int* brpt;
+    const int* bcol;
+    const double* bval;
+    const int64_t* ipA;  // [nnzA+1]: global index of each A entry's first product
+    int M, K, N;
+};
+
+// Where finished rows go.
+struct OutArgs {
+    int upper;            // 0: Precise (C at crpt), 1: Upper (C-bar at ipA[arpt[row]], lengths to rowlen)
+    const int64_t* crpt;  // Precise
+    int* ccol;            // Precise: C; Upper: C-bar
+    double* cval;
+    int64_t* rowlen;      // Upper
+    int* err;             // DevErr bits
+};
+
+// flop.hip
+hipError_t launch_validate(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* err, hipStream_t s);
+hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* err, hipStream_t s);
+hipError_t launch_rowstats(const RowArgs& a, int med_cap, int force_long, int64_t* iprow, int64_t* nnzrow,
+                           int* bins /*[3][M]*/, int* bin_cnt /*[4]*/, hipStream_t s);
+
+// short.hip
+hipError_t launch_short_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
+hipError_t launch_short_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s);
+
+// lds.hip
+hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
+hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s);
+hipError_t launch_long_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
+hipError_t launch_leaf(const LeafTask* tasks, int64_t n, const RowArgs& a, int* const ws_col[2],
+                       double* const ws_val[2], int64_t* slots, int64_t* out_ctr, int* err, hipStream_t s);
+
+// long.hip
+struct LevelBufs {
+    int* ws_col[2];
+    double* ws_val[2];
+    int64_t* slots;
+};
+hipError_t launch_plan(const LongArgs& la, int leaf_cap, int* err, hipStream_t s);
+hipError_t launch_level_len(const PairTask* t, int64_t n, const LevelBufs& wb, int64_t* lens /*[2n]*/,
+                            int64_t* ntile, hipStream_t s);
+hipError_t launch_level_split(const PairTask* t, int64_t n, const int64_t* toff, int64_t ntiles,
+                              const int64_t* lens, const RowArgs& a, const LevelBufs& wb,
+                              int64_t* split /*[2*ntiles]*/, hipStream_t s);
+hipError_t launch_level_count(const PairTask* t, int64_t n, const int64_t* toff, int64_t ntiles,
+                              const int64_t* lens, const int64_t* split, const RowArgs& a,
+                              const LevelBufs& wb, int64_t* tcnt, hipStream_t s);
+hipError_t launch_level_write(const PairTask* t, int64_t n, const int64_t* toff, int64_t ntiles,
+                              const int64_t* lens, const int64_t* split, const int64_t* tout,
+                              const RowArgs& a, const LevelBufs& wb, const OutArgs& o, hipStream_t s);
+
+// assemble.hip
+hipError_t launch_compact(const int64_t* crpt, int M, const RowArgs& a, const int* cbar_col,
+                          const double* cbar_val, int* ccol, double* cval, int64_t nnzC, hipStream_t s);
+hipError_t launch_rpt_to_i32(const int64_t* in, int64_t n, int* out, hipStream_t s);
+hipError_t launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);
+hipError_t launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);
+hipError_t launch_long_info(const int* rows, int n, const RowArgs& a, int64_t* ip_out, int* len_out,
+                            hipStream_t s);
+
+}  // namespace brm

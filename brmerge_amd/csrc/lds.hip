@@ -1,0 +1,426 @@
+// LDS tier: one workgroup merges one row (or one aligned subtree of a long
+// row's lists) entirely in LDS, and the symbolic counters for medium and
+// long rows.
+//
+// Tree merge (BRMerge-SpGEMM/inc/spgemm.h:610-707 restated for a workgroup):
+//   EXPAND   every non-empty list p (B row acol[p] scaled by aval[p]) is laid
+//            out contiguously in LDS buffer 0 (coalesced B gathers);
+//   MERGE    level by level, lists whose node ids share a parent
+//            (id >> 1 equal) form a pair, others a single that is carried;
+//            all pairs of a level are merged at once with merge-path
+//            partitioning of the level's raw element range over the
+//            workgroup's threads (a count pass, a block scan, a write pass),
+//            from one LDS buffer into the other (ping-pong);
+//   OUTPUT   the last list is streamed to C / C-bar / the workspace.
+// Empty lists are never materialised but their positions are kept in the
+// node ids, so the pairing -- and therefore the bracketing of every sum --
+// is that of the reference's aligned tree.
+#include <limits.h>
+
+#include "brm_device.h"
+#include "kernels.h"
+
+namespace brm {
+namespace {
+
+template <int NT, int CAP, int LMAX>
+struct MergeSmem {
+    double val[2][CAP];
+    int col[2][CAP];
+    int lid[2][LMAX];
+    int loff[2][LMAX + 1];
+    int gs[LMAX + 1];
+    int sg[NT + 1];
+    int si[NT + 1];
+    int sj[NT + 1];
+    int scan[NT / BRM_WAVE + 1];
+};
+
+// Merge the non-empty lists of A entries [ab, ae) of one row.  Positions are
+// taken relative to ab, which must be aligned to the subtree size.  The
+// result (length returned, uniform) is written to dcol/dval.
+template <int NT, int CAP, int LMAX>
+__device__ int64_t lds_tree_merge(MergeSmem<NT, CAP, LMAX>& sm, const RowArgs& a, int ab, int ae,
+                                  int* __restrict__ dcol, double* __restrict__ dval, int64_t dcap, int* err) {
+    const int t = threadIdx.x;
+    const int64_t ibase = a.ipA[ab];
+    const int S = (int)(a.ipA[ae] - ibase);
+
+    // ---- list descriptors of the non-empty lists
+    int n = 0;
+    for (int cs = ab; cs < ae; cs += NT) {
+        const int ai = cs + t;
+        int len = 0;
+        if (ai < ae) len = (int)(a.ipA[ai + 1] - a.ipA[ai]);
+        const int flag = len > 0;
+        int tot;
+        const int ex = block_excl_scan<NT>(flag, sm.scan, &tot);
+        if (flag && n + ex < LMAX) {
+            sm.lid[0][n + ex] = ai - ab;
+            sm.loff[0][n + ex] = (int)(a.ipA[ai] - ibase);
+        }
+        n += tot;
+    }
+    if (n > LMAX || S > CAP) {  // planner guarantees otherwise
+        if (t == 0) atomicOr(err, (int)DERR_CAPACITY);
+        return 0;
+    }
+    if (t == 0) sm.loff[0][n] = S;
+    __syncthreads();
+
+    // ---- EXPAND (spgemm.h:618-624): products rounded before any add
+    {
+        int k = 0;
+        for (int e = t; e < S; e += NT) {
+            k = gallop_idx(sm.loff[0], k, n, e);
+            const int ai = ab + sm.lid[0][k];
+            const int bi = a.brpt[a.acol[ai]] + (e - sm.loff[0][k]);
+            sm.col[0][e] = a.bcol[bi];
+            sm.val[0][e] = fmul(a.aval[ai], a.bval[bi]);
+        }
+    }
+    __syncthreads();
+
+    // ---- MERGE levels
+    int cur = 0;
+    while (n > 1) {
+        const int nxt = cur ^ 1;
+        int G = 0;
+        for (int cs = 0; cs < n; cs += NT) {
+            const int k = cs + t;
+            int flag = 0;
+            if (k < n) flag = (k == 0) || ((sm.lid[cur][k - 1] >> 1) != (sm.lid[cur][k] >> 1));
+            int tot;
+            const int ex = block_excl_scan<NT>(flag, sm.scan, &tot);
+            if (flag) {
+                sm.gs[G + ex] = k;
+                sm.lid[nxt][G + ex] = sm.lid[cur][k] >> 1;
+            }
+            G += tot;
+        }
+        if (t == 0) sm.gs[G] = n;
+        __syncthreads();
+        if (G == n) {  // no pair on this level: only the node ids move up
+            for (int k = t; k < n; k += NT) sm.lid[cur][k] = sm.lid[nxt][k];
+            __syncthreads();
+            continue;
+        }
+        const int* __restrict__ lo_ = sm.loff[cur];
+        const int* __restrict__ cc = sm.col[cur];
+        const double* __restrict__ cv = sm.val[cur];
+        const int Sl = lo_[n];
+        const int c = (Sl + NT - 1) / NT;
+
+        // split points of threads t (and NT)
+        for (int tt = t; tt <= NT; tt += NT) {
+            const int d = min(tt * c, Sl);
+            // largest g in [0, G] with goff(g) <= d, goff(G) = Sl
+            int glo = 0, glen = G + 1;
+            while (glen > 1) {
+                const int half = glen >> 1;
+                const int mid = glo + half;
+                const int gm = (mid == G) ? Sl : lo_[sm.gs[mid]];
+                if (gm <= d) { glo = mid; glen -= half; } else { glen = half; }
+            }
+            int i = 0, j = 0;
+            if (glo < G) {
+                const int k = sm.gs[glo];
+                const int L0 = lo_[k], R0 = lo_[k + 1];
+                const int nL = R0 - L0;
+                const bool pair = (sm.gs[glo + 1] - k) == 2;
+                const int nR = pair ? lo_[k + 2] - R0 : 0;
+                const int dd = d - L0;
+                if (!pair) { i = dd; j = 0; }
+                else {
+                    merge_split([&](int x) { return cc[L0 + x]; }, nL, [&](int y) { return cc[R0 + y]; }, nR,
+                                dd, &i, &j);
+                }
+            }
+            sm.sg[tt] = glo;
+            sm.si[tt] = i;
+            sm.sj[tt] = j;
+        }
+        __syncthreads();
+
+        const int g0 = sm.sg[t], i0 = sm.si[t], j0 = sm.sj[t];
+        const int ge = sm.sg[t + 1], ie = sm.si[t + 1], je = sm.sj[t + 1];
+
+        // count pass
+        int cnt = 0;
+        {
+            int g = g0, i = i0, j = j0;
+            while (g < ge || (g == ge && (i < ie || j < je))) {
+                const int k = sm.gs[g];
+                const int L0 = lo_[k], R0 = lo_[k + 1];
+                const bool pair = (sm.gs[g + 1] - k) == 2;
+                const int nL = R0 - L0;
+                const int nR = pair ? lo_[k + 2] - R0 : 0;
+                const int iend = (g == ge) ? ie : nL;
+                const int jend = (g == ge) ? je : nR;
+                while (i < iend && j < jend) {
+                    const int x = cc[L0 + i], y = cc[R0 + j];
+                    i += (x <= y);
+                    j += (y <= x);
+                    cnt++;
+                }
+                cnt += (iend - i) + (jend - j);
+                g++;
+                i = 0;
+                j = 0;
+            }
+        }
+        int total;
+        const int obase = block_excl_scan<NT>(cnt, sm.scan, &total);
+
+        // write pass
+        {
+            int* __restrict__ oc = sm.col[nxt];
+            double* __restrict__ ov = sm.val[nxt];
+            int o = obase;
+            int g = g0, i = i0, j = j0;
+            while (g < ge || (g == ge && (i < ie || j < je))) {
+                const int k = sm.gs[g];
+                const int L0 = lo_[k], R0 = lo_[k + 1];
+                const bool pair = (sm.gs[g + 1] - k) == 2;
+                const int nL = R0 - L0;
+                const int nR = pair ? lo_[k + 2] - R0 : 0;
+                const int iend = (g == ge) ? ie : nL;
+                const int jend = (g == ge) ? je : nR;
+                if (i == 0 && j == 0) sm.loff[nxt][g] = o;
+                while (i < iend && j < jend) {
+                    const int x = cc[L0 + i], y = cc[R0 + j];
+                    if (x == y) {
+                        oc[o] = x;
+                        ov[o] = fadd(cv[L0 + i], cv[R0 + j]);
+                        i++;
+                        j++;
+                    } else if (x < y) {
+                        oc[o] = x;
+                        ov[o] = cv[L0 + i];
+                        i++;
+                    } else {
+                        oc[o] = y;
+                        ov[o] = cv[R0 + j];
+                        j++;
+                    }
+                    o++;
+                }
+                for (; i < iend; ++i, ++o) { oc[o] = cc[L0 + i]; ov[o] = cv[L0 + i]; }
+                for (; j < jend; ++j, ++o) { oc[o] = cc[R0 + j]; ov[o] = cv[R0 + j]; }
+                g++;
+                i = 0;
+                j = 0;
+            }
+        }
+        if (t == 0) sm.loff[nxt][G] = total;
+        __syncthreads();
+        cur = nxt;
+        n = G;
+    }
+
+    // ---- OUTPUT
+    const int len = (n == 1) ? sm.loff[cur][1] : 0;
+    const int wlen = (int)min((int64_t)len, dcap);  // never write past the destination
+    for (int e = t; e < wlen; e += NT) {
+        dcol[e] = sm.col[cur][e];
+        dval[e] = sm.val[cur][e];
+    }
+    return len;
+}
+
+template <int NT, int CAP, int LMAX>
+__global__ __launch_bounds__(NT) void k_medium(const int* __restrict__ rows, int nrows, RowArgs a, OutArgs o) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    auto& sm = *reinterpret_cast<MergeSmem<NT, CAP, LMAX>*>(smraw);
+    const int row = rows[blockIdx.x];
+    const int ab = a.arpt[row], ae = a.arpt[row + 1];
+    if (!o.upper) {
+        const int64_t cb = o.crpt[row];
+        const int64_t cap = o.crpt[row + 1] - cb;
+        const int64_t len = lds_tree_merge<NT, CAP, LMAX>(sm, a, ab, ae, o.ccol + cb, o.cval + cb, cap, o.err);
+        if (threadIdx.x == 0 && len != cap) atomicOr(o.err, (int)DERR_COUNT_MISMATCH);
+    } else {
+        const int64_t cb = a.ipA[ab];
+        const int64_t len =
+            lds_tree_merge<NT, CAP, LMAX>(sm, a, ab, ae, o.ccol + cb, o.cval + cb, a.ipA[ae] - cb, o.err);
+        if (threadIdx.x == 0) o.rowlen[row] = len;
+    }
+}
+
+struct WsPtrs {
+    int* col0;
+    int* col1;
+    double* val0;
+    double* val1;
+};
+
+template <int NT, int CAP, int LMAX>
+__global__ __launch_bounds__(NT) void k_leaf(const LeafTask* __restrict__ tasks, RowArgs a, WsPtrs ws,
+                                             int64_t* __restrict__ slots, int64_t* out_ctr, int* err) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    auto& sm = *reinterpret_cast<MergeSmem<NT, CAP, LMAX>*>(smraw);
+    const LeafTask tk = tasks[blockIdx.x];
+    int* dc = (tk.buf ? ws.col1 : ws.col0) + tk.dst_off;
+    double* dv = (tk.buf ? ws.val1 : ws.val0) + tk.dst_off;
+    const int64_t len = lds_tree_merge<NT, CAP, LMAX>(sm, a, tk.ab, tk.ae, dc, dv, a.ipA[tk.ae] - a.ipA[tk.ab], err);
+    if (threadIdx.x == 0) {
+        slots[tk.slot] = len;
+        atomicAdd((unsigned long long*)out_ctr, (unsigned long long)len);
+    }
+}
+
+// ---------------------------------------------------------------- symbolic
+// Medium rows: distinct columns counted with an LDS open-addressing table
+// (the reference uses a per-thread linear-probing table of upper_log(ip)
+// slots, spgemm.h:528-556; any table gives the same count).
+constexpr int HS_NT = 256;
+constexpr int HS_CAP = 2 * MED_CAP;
+
+__global__ __launch_bounds__(HS_NT) void k_sym_hash(const int* __restrict__ rows, int nrows, RowArgs a,
+                                                    int64_t* __restrict__ nnzrow) {
+    __shared__ int table[HS_CAP];
+    __shared__ int loff[MED_LMAX + 1];
+    __shared__ int sm[HS_NT / BRM_WAVE + 1];
+    const int t = threadIdx.x;
+    const int row = rows[blockIdx.x];
+    const int ab = a.arpt[row];
+    const int L = a.arpt[row + 1] - ab;
+    const int64_t ib = a.ipA[ab];
+    const int S = (int)(a.ipA[ab + L] - ib);
+    int hs = 64;
+    while (hs < 2 * S) hs <<= 1;
+    for (int k = t; k < hs; k += HS_NT) table[k] = -1;
+    for (int k = t; k <= L; k += HS_NT) loff[k] = (int)(a.ipA[ab + k] - ib);
+    __syncthreads();
+    int cnt = 0;
+    int hint = 0;
+    for (int e = t; e < S; e += HS_NT) {
+        hint = gallop_idx(loff, hint, L, e);
+        const int ai = ab + hint;
+        const int key = a.bcol[a.brpt[a.acol[ai]] + (e - loff[hint])];
+        unsigned h = ((unsigned)key * 2654435761u) & (unsigned)(hs - 1);
+        for (;;) {
+            const int old = atomicCAS(&table[h], -1, key);
+            if (old == -1) { cnt++; break; }
+            if (old == key) break;
+            h = (h + 1) & (unsigned)(hs - 1);
+        }
+    }
+    const int tot = block_sum<HS_NT>(cnt, sm);
+    if (t == 0) nnzrow[row] = tot;
+}
+
+// Long rows: distinct columns counted with an LDS bitmap of up to 2^20
+// columns per pass (128 KB); wider matrices take several column passes.
+constexpr int BM_NT = 1024;
+constexpr int BM_BITS = 1 << 20;
+constexpr int BM_WORDS = BM_BITS / 32;
+constexpr int BM_CHUNK = 4096;
+
+__global__ __launch_bounds__(BM_NT) void k_sym_bitmap(const int* __restrict__ rows, int nrows, RowArgs a,
+                                                      int64_t* __restrict__ nnzrow) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    unsigned* bm = reinterpret_cast<unsigned*>(smraw);
+    int* loff = reinterpret_cast<int*>(smraw + sizeof(unsigned) * BM_WORDS);
+    int64_t* red = reinterpret_cast<int64_t*>(smraw + sizeof(unsigned) * BM_WORDS + sizeof(int) * (BM_CHUNK + 8));
+    const int t = threadIdx.x;
+    const int row = rows[blockIdx.x];
+    const int ab = a.arpt[row], ae = a.arpt[row + 1];
+    int64_t total = 0;
+    for (int64_t c0 = 0; c0 < a.N; c0 += BM_BITS) {
+        const int64_t span = min((int64_t)BM_BITS, (int64_t)a.N - c0);
+        const int words = (int)((span + 31) / 32);
+        for (int w = t; w < words; w += BM_NT) bm[w] = 0u;
+        __syncthreads();
+        for (int cs = ab; cs < ae; cs += BM_CHUNK) {
+            const int nl = min(BM_CHUNK, ae - cs);
+            const int64_t cb = a.ipA[cs];
+            for (int k = t; k <= nl; k += BM_NT) loff[k] = (int)(a.ipA[cs + k] - cb);
+            __syncthreads();
+            const int Sc = loff[nl];
+            int hint = 0;
+            for (int e = t; e < Sc; e += BM_NT) {
+                hint = gallop_idx(loff, hint, nl, e);
+                const int ai = cs + hint;
+                const int col = a.bcol[a.brpt[a.acol[ai]] + (e - loff[hint])];
+                const int64_t rel = (int64_t)col - c0;
+                if (rel >= 0 && rel < span) atomicOr(&bm[rel >> 5], 1u << (rel & 31));
+            }
+            __syncthreads();
+        }
+        int64_t cnt = 0;
+        for (int w = t; w < words; w += BM_NT) cnt += __popc(bm[w]);
+        total += block_sum<BM_NT>(cnt, red);
+        __syncthreads();
+    }
+    if (t == 0) nnzrow[row] = total;
+}
+
+constexpr size_t bitmap_smem() {
+    return sizeof(unsigned) * BM_WORDS + sizeof(int) * (BM_CHUNK + 8) + sizeof(int64_t) * (BM_NT / BRM_WAVE + 2);
+}
+
+}  // namespace
+
+hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    using SM = MergeSmem<MED_NT, MED_CAP, MED_LMAX>;
+    auto kern = k_medium<MED_NT, MED_CAP, MED_LMAX>;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(SM));
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    {
+        ProfScope ps_("k_medium", s);
+        kern<<<n, MED_NT, sizeof(SM), s>>>(rows, n, a, o);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf(const LeafTask* tasks, int64_t n, const RowArgs& a, int* const ws_col[2],
+                       double* const ws_val[2], int64_t* slots, int64_t* out_ctr, int* err, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    using SM = MergeSmem<MED_NT, MED_CAP, MED_LMAX>;
+    auto kern = k_leaf<MED_NT, MED_CAP, MED_LMAX>;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(SM));
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    WsPtrs ws{ws_col[0], ws_col[1], ws_val[0], ws_val[1]};
+    {
+        ProfScope ps_("k_leaf", s);
+        kern<<<(unsigned)n, MED_NT, sizeof(SM), s>>>(tasks, a, ws, slots, out_ctr, err);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_sym_hash", s);
+        k_sym_hash<<<n, HS_NT, 0, s>>>(rows, n, a, nnzrow);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_long_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_sym_bitmap, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)bitmap_smem());
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    {
+        ProfScope ps_("k_sym_bitmap", s);
+        k_sym_bitmap<<<n, BM_NT, bitmap_smem(), s>>>(rows, n, a, nnzrow);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace brm

@@ -1,0 +1,189 @@
+/*
+ * brmerge.h -- C ABI of the MI355X (gfx950) BRMerge SpGEMM library
+ * (libbrmerge.so).  C = A * B for CSR matrices with int32 column indices and
+ * FP64 values, computed by binary-row merging: every A entry of a row selects
+ * one B row scaled by the A value, and the row's lists are merged pairwise in
+ * an aligned binary tree.  Results are bit-identical to
+ * BRMerge-SpGEMM/inc/spgemm.h (same tree bracketing, products rounded before
+ * any add).
+ *
+ * Conventions shared by every entry point:
+ *   - 0-based CSR; columns of each B row strictly increasing (the reference
+ *     guarantees this at read time, src/CSR.cpp:454-477; here it is checked
+ *     and rejected with BRM_E_INVALID);
+ *   - K and N are the inner and output dimensions (the reference accepts and
+ *     ignores them, spgemm.h:468-471; here they bound the index checks);
+ *   - every call is synchronous and returns a BRM_* status; the message of
+ *     the last failure is available from brm_last_error().
+ */
+#ifndef BRMERGE_H_
+#define BRMERGE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define BRM_OK 0
+#define BRM_E_INVALID 1   /* malformed CSR or argument */
+#define BRM_E_NOMEM 2     /* device or host allocation failed; no output is returned */
+#define BRM_E_OVERFLOW 3  /* nnz(C) does not fit the int32 row pointer of an _i32 entry */
+#define BRM_E_DEVICE 4    /* HIP runtime error */
+#define BRM_E_INTERNAL 5  /* internal consistency check failed */
+
+/* Phase times in seconds.  Field names follow Precise_Timing
+ * (BRMerge-SpGEMM/inc/Timings.h:27-46); h2d/d2h are the host<->device copies
+ * of the host-pointer entries (zero for device entries). */
+typedef struct brm_precise_timing {
+    double pre_allocate;
+    double compute_flop;
+    double prefix_sum_flop;
+    double load_balance;
+    double symbolic;
+    double prefix_sum_nnz;
+    double allocate_C;
+    double compute;
+    double cleanup;
+    double total;
+    double h2d;
+    double d2h;
+} brm_precise_timing;
+
+/* Field names follow Upper_Timing (BRMerge-SpGEMM/inc/Timings.h:4-25). */
+typedef struct brm_upper_timing {
+    double thread;
+    double pre_allocate;
+    double compute_flop;
+    double prefix_sum_flop;
+    double load_balance;
+    double allocate_hook;
+    double compute;
+    double prefix_sum_nnz;
+    double allocate_C;
+    double copy;
+    double cleanup;
+    double total;
+    double h2d;
+    double d2h;
+} brm_upper_timing;
+
+/* Select the HIP device used by this thread's calls (default 0). */
+int brm_init(int device);
+const char *brm_last_error(void);
+const char *brm_version(void);
+
+/*
+ * Host-pointer entries.  Replace
+ *   brmerge_precise(mint*,mint*,mdouble*,mint*,mint*,mdouble*,mint**,mint**,mdouble**,
+ *                   mint M,mint K,mint N,mint* cnnz,Precise_Timing&)   spgemm.h:468-471
+ *   brmerge_upper(... , Upper_Timing&)                                  spgemm.h:220-223
+ * with the same argument order.  *crpt, *ccol and *cval are allocated with
+ * C++ new[] (as the reference does, spgemm.h:479,574-575) and owned by the
+ * caller: release them with delete[] (CSR::release, CSR.cpp:102-109) or with
+ * brm_host_free_i32 / brm_host_free_f64.  On error nothing is allocated.
+ * The timing pointer may be NULL.
+ */
+int brm_spgemm_precise_i32(const int *arpt, const int *acol, const double *aval,
+                           const int *brpt, const int *bcol, const double *bval,
+                           int **crpt, int **ccol, double **cval,
+                           int M, int K, int N, int *cnnz, brm_precise_timing *timing);
+int brm_spgemm_upper_i32(const int *arpt, const int *acol, const double *aval,
+                         const int *brpt, const int *bcol, const double *bval,
+                         int **crpt, int **ccol, double **cval,
+                         int M, int K, int N, int *cnnz, brm_upper_timing *timing);
+
+/* As above with an int64 row pointer for C (nnz(C) may exceed INT32_MAX;
+ * the reference's int32 crpt cannot represent such products, spgemm.h:479,571).
+ * Free *crpt with brm_host_free_i64. */
+int brm_spgemm_precise_i64(const int *arpt, const int *acol, const double *aval,
+                           const int *brpt, const int *bcol, const double *bval,
+                           int64_t **crpt, int **ccol, double **cval,
+                           int M, int K, int N, int64_t *cnnz, brm_precise_timing *timing);
+int brm_spgemm_upper_i64(const int *arpt, const int *acol, const double *aval,
+                         const int *brpt, const int *bcol, const double *bval,
+                         int64_t **crpt, int **ccol, double **cval,
+                         int M, int K, int N, int64_t *cnnz, brm_upper_timing *timing);
+
+void brm_host_free_i32(int *p);
+void brm_host_free_i64(int64_t *p);
+void brm_host_free_f64(double *p);
+
+/*
+ * Device-pointer entries: A and B already in HBM (device pointers), C
+ * returned in HBM.  `stream` is a hipStream_t (NULL = default stream); the
+ * call returns after the stream has completed the product.  Output buffers
+ * come from the library's caching device allocator: release each with
+ * brm_dev_free.  nnzA / nnzB are arpt[M] / brpt[K] (passed to avoid a
+ * device read).
+ */
+int brm_spgemm_precise_dev(const int *arpt, const int *acol, const double *aval, int64_t nnzA,
+                           const int *brpt, const int *bcol, const double *bval, int64_t nnzB,
+                           int M, int K, int N,
+                           int64_t **crpt, int **ccol, double **cval, int64_t *cnnz,
+                           brm_precise_timing *timing, void *stream);
+int brm_spgemm_upper_dev(const int *arpt, const int *acol, const double *aval, int64_t nnzA,
+                         const int *brpt, const int *bcol, const double *bval, int64_t nnzB,
+                         int M, int K, int N,
+                         int64_t **crpt, int **ccol, double **cval, int64_t *cnnz,
+                         brm_upper_timing *timing, void *stream);
+
+void *brm_dev_malloc(uint64_t bytes);  /* caching device allocator */
+void brm_dev_free(void *p);
+void brm_dev_trim(void);               /* return cached device memory to the driver */
+
+/*
+ * Intermediate-product ("flop") count, the reference's
+ *   long compute_flop(const CSR& A, const CSR& B)     spgemm.h:59-66
+ * (sum over A entries of the selected B row's length) for host pointers.
+ * Returns BRM_OK and the total in *total.  The _dev form also fills the
+ * per-row counts (device int64[M]) when row_ip is non-NULL.
+ */
+int brm_flop_count(const int *arpt, const int *acol, const int *brpt, int M, int K, int64_t *total);
+int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int *brpt, int M, int K,
+                       int64_t *row_ip, int64_t *total, void *stream);
+
+/* Tuning knobs (names: "ws_budget_bytes", "med_cap", "leaf_cap",
+ * "force_long", "profile"); returns BRM_E_INVALID for unknown names or values. */
+int brm_set_param(const char *name, int64_t value);
+int64_t brm_get_param(const char *name);
+
+/* Synchronous copies between host memory and memory from brm_dev_malloc
+ * (for callers without a HIP binding of their own). */
+int brm_memcpy_h2d(void *dst_dev, const void *src_host, uint64_t bytes);
+int brm_memcpy_d2h(void *dst_host, const void *src_dev, uint64_t bytes);
+
+/* Per-kernel device timing.  With brm_set_param("profile", 1) every kernel
+ * launch of the library is bracketed by a pair of hipEvents on its stream;
+ * brm_prof_report writes {"kernel_name": [launches, total_ms], ...} as JSON
+ * into buf (returns BRM_E_INVALID if cap is too small); brm_prof_reset
+ * clears the totals. */
+void brm_prof_reset(void);
+int brm_prof_report(char *buf, uint64_t cap);
+
+/* Work counters of the calling thread's last SpGEMM (filled always; the
+ * per-level element counts only when "profile" is set).  Used to turn kernel
+ * times into algorithmic bandwidth. */
+typedef struct brm_stats {
+    int64_t rows_by_tier[4];   /* empty, short (<= 64 products), medium (LDS), long */
+    int64_t total_ip;          /* intermediate products */
+    int64_t nnz_c;
+    int64_t long_waves;        /* workspace waves of the long-row tier */
+    int64_t leaf_tasks;        /* LDS-merged aligned subtrees of long rows */
+    int64_t leaf_ip;           /* products inside those subtrees */
+    int64_t leaf_out;          /* merged elements they produce */
+    int64_t level_tasks;       /* pair merges above the subtrees */
+    int64_t level_tiles;
+    int64_t level_in;          /* merge input elements (profile only) */
+    int64_t level_out;         /* merge output elements (profile only) */
+} brm_stats;
+int brm_last_stats(brm_stats *out);
+
+/* Test hook: make the n-th next device allocation fail (n < 0 disables). */
+void brm_debug_fail_alloc_after(int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BRMERGE_H_ */

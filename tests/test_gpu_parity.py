@@ -1,0 +1,246 @@
+"""GPU parity: libbrmerge.so (HIP, gfx950) against the CPU oracle.
+
+Structure (row pointer, columns) must match exactly; values are required to
+be within 1e-12 max-abs-normalised relative error and are checked
+bit-for-bit as well, since the GPU keeps the reference's tree bracketing and
+rounds every product before adding it.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests._util import as_tuple, assert_csr_equal, csr_from_tuple, rand_csr_tuple
+from tests.test_oracle import KATS
+
+pytestmark = pytest.mark.gpu
+
+TIER_SETTINGS = {
+    "default": {},
+    "all_long": {"force_long": 1},
+    "tiny_leaves": {"force_long": 1, "leaf_cap": 8},
+    "small_medium": {"med_cap": 64, "leaf_cap": 100},
+}
+
+
+@pytest.fixture
+def tier(request, gpu):
+    name = request.param
+    saved = {k: gpu.get_param(k) for k in ("force_long", "leaf_cap", "med_cap")}
+    for k, v in TIER_SETTINGS[name].items():
+        gpu.set_param(k, v)
+    yield name
+    for k, v in saved.items():
+        gpu.set_param(k, v)
+
+
+def run_both(bm, A, B, i64=False):
+    Ah, Bh = csr_from_tuple(A), csr_from_tuple(B)
+    Cp = bm.brmerge_precise(Ah, Bh, i64=i64)
+    Cu = bm.brmerge_upper(Ah, Bh, i64=i64)
+    return Cp, Cu
+
+
+@pytest.mark.parametrize("case", KATS, ids=[k[0] for k in KATS])
+@pytest.mark.parametrize("tier", list(TIER_SETTINGS), indirect=True)
+def test_kat_gpu(gpu, case, tier):
+    _, A, B, C = case
+    Cp, Cu = run_both(gpu, A, B)
+    from oracle import OracleCSR
+    R = OracleCSR(len(C[0]) - 1, B[3], C[0], C[1], C[2])
+    assert_csr_equal(Cp, R)
+    assert_csr_equal(Cu, R)
+
+
+@pytest.mark.parametrize("seed", range(16))
+@pytest.mark.parametrize("tier", list(TIER_SETTINGS), indirect=True)
+def test_random_small(gpu, seed, tier):
+    rng = np.random.default_rng(1000 + seed)
+    M, K, N = [int(x) for x in rng.integers(1, 300, 3)]
+    A = rand_csr_tuple(rng, M, K, int(rng.integers(1, 80)), empty_frac=0.3)
+    B = rand_csr_tuple(rng, K, N, int(rng.integers(1, 60)), empty_frac=0.3)
+    R = oracle.brmerge_precise(A, B, nthreads=4)
+    Cp, Cu = run_both(gpu, A, B)
+    assert_csr_equal(Cp, R)
+    assert_csr_equal(Cu, R)
+    Cp64, Cu64 = run_both(gpu, A, B, i64=True)
+    assert_csr_equal(Cp64, R)
+    assert_csr_equal(Cu64, R)
+
+
+def test_er16384_config(gpu):
+    A = gpu.gen.er(16384, 16384, 8, 0)
+    R = oracle.brmerge_precise(as_tuple(A), as_tuple(A))
+    assert gpu.compute_flop(A, A) == 8 * 8 * 16384
+    assert_csr_equal(gpu.brmerge_precise(A, A), R)
+    assert_csr_equal(gpu.brmerge_upper(A, A), R)
+
+
+@pytest.mark.parametrize("scale", [10, 13, 15])
+def test_rmat(gpu, scale):
+    A = gpu.gen.rmat(scale)
+    R = oracle.brmerge_precise(as_tuple(A), as_tuple(A))
+    assert_csr_equal(gpu.brmerge_precise(A, A), R)
+    assert_csr_equal(gpu.brmerge_upper(A, A), R)
+
+
+def test_amg_chain(gpu):
+    n = 21
+    S, P = gpu.gen.stencil27(n), gpu.gen.aggregation(n)
+    AP = gpu.brmerge_precise(S, P)
+    R1 = oracle.brmerge_precise(as_tuple(S), as_tuple(P))
+    assert_csr_equal(AP, R1)
+    assert gpu.compute_flop(S, P) == S.nnz
+    PT = P.transpose()
+    C = gpu.brmerge_precise(PT, AP)
+    R2 = oracle.brmerge_precise(as_tuple(PT), (np.asarray(R1.rpt, np.int32), R1.col, R1.val, P.N))
+    assert_csr_equal(C, R2)
+    assert gpu.compute_flop(PT, AP) == AP.nnz
+
+
+def test_banded(gpu):
+    A = gpu.gen.banded(5000, 64, 4, 32, 3)
+    B = gpu.gen.banded(5000, 64, 4, 32, 4)
+    R = oracle.brmerge_precise(as_tuple(A), as_tuple(B))
+    Cp = gpu.brmerge_precise(A, B)
+    assert_csr_equal(Cp, R)
+
+
+def test_edge_shapes(gpu):
+    bm = gpu
+    # empty A (M = 0)
+    A = bm.CSR(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0), 0, 5)
+    B = bm.gen.er(5, 7, 2, 1)
+    C = bm.brmerge_precise(A, B)
+    assert C.M == 0 and C.nnz == 0
+    # A without entries, B without entries
+    A = bm.CSR(np.zeros(4, np.int32), np.zeros(0, np.int32), np.zeros(0), 3, 5)
+    C = bm.brmerge_upper(A, B)
+    assert C.nnz == 0 and list(C.rpt) == [0, 0, 0, 0]
+    Bz = bm.CSR(np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0), 5, 7)
+    A = bm.gen.er(3, 5, 4, 2)
+    C = bm.brmerge_precise(A, Bz)
+    assert C.nnz == 0
+
+
+def test_row_with_many_empty_lists(gpu):
+    """Rows with thousands of A entries, almost all pointing at empty B rows."""
+    rng = np.random.default_rng(5)
+    K, N = 20000, 3000
+    brows = [np.sort(rng.choice(N, size=(int(rng.integers(1, 40)) if r % 97 == 0 else 0), replace=False))
+             for r in range(K)]
+    brpt = np.concatenate([[0], np.cumsum([len(b) for b in brows])]).astype(np.int32)
+    B = (brpt, np.concatenate(brows).astype(np.int32), rng.uniform(-1, 1, brpt[-1]), N)
+    acols = [np.sort(rng.choice(K, size=s, replace=False)) for s in (5000, 700, 64, 1, 12000)]
+    arpt = np.concatenate([[0], np.cumsum([len(a) for a in acols])]).astype(np.int32)
+    A = (arpt, np.concatenate(acols).astype(np.int32), rng.uniform(-1, 1, arpt[-1]), K)
+    R = oracle.brmerge_precise(A, B)
+    Cp, Cu = run_both(gpu, A, B)
+    assert_csr_equal(Cp, R)
+    assert_csr_equal(Cu, R)
+
+
+def test_single_long_list(gpu):
+    """A row selecting one B row longer than any LDS tile (a copied list)."""
+    rng = np.random.default_rng(6)
+    N = 100000
+    bc = np.sort(rng.choice(N, size=30000, replace=False)).astype(np.int32)
+    B = (np.array([0, 30000, 30000], np.int32), bc, rng.uniform(-1, 1, 30000), N)
+    A = (np.array([0, 1, 3, 4], np.int32), np.array([0, 0, 1, 1], np.int32), np.array([0.5, 3.0, 2.0, 1.0]), 2)
+    R = oracle.brmerge_precise(A, B)
+    Cp, Cu = run_both(gpu, A, B)
+    assert_csr_equal(Cp, R)
+    assert_csr_equal(Cu, R)
+
+
+def test_long_rows_many_levels(gpu):
+    """Rows far beyond one tile with deep trees, dense column overlap."""
+    rng = np.random.default_rng(7)
+    K, N = 3000, 6000
+    B = rand_csr_tuple(rng, K, N, 300, empty_frac=0.1)
+    acols = [np.sort(rng.choice(K, size=s, replace=False)) for s in (2900, 1500, 257, 513, 100)]
+    arpt = np.concatenate([[0], np.cumsum([len(a) for a in acols])]).astype(np.int32)
+    A = (arpt, np.concatenate(acols).astype(np.int32), rng.uniform(-2, 2, arpt[-1]), K)
+    R = oracle.brmerge_precise(A, B)
+    Cp, Cu = run_both(gpu, A, B)
+    assert_csr_equal(Cp, R)
+    assert_csr_equal(Cu, R)
+
+
+def test_device_entry_and_determinism(gpu):
+    bm = gpu
+    A = bm.gen.rmat(12, seed=3)
+    dA = bm.DeviceCSR.upload(A)
+    try:
+        t = bm.Precise_Timing()
+        C1 = bm.spgemm_dev(dA, dA, timing=t)
+        C2 = bm.spgemm_dev(dA, dA)
+        C3 = bm.spgemm_dev(dA, dA, upper=True)
+        h1, h2, h3 = C1.download(), C2.download(), C3.download()
+        for C in (C1, C2, C3):
+            C.free()
+    finally:
+        dA.free()
+    R = oracle.brmerge_precise(as_tuple(A), as_tuple(A))
+    for h in (h1, h2, h3):
+        assert_csr_equal(h, R)
+    assert t.total > 0 and t.compute > 0
+
+
+def test_invalid_inputs(gpu):
+    bm = gpu
+    good = bm.gen.er(50, 50, 5, 0)
+    # unsorted B row
+    B = bm.gen.er(50, 50, 5, 1)
+    B.col = B.col.copy()
+    B.col[0], B.col[1] = B.col[1], B.col[0]
+    with pytest.raises(bm.BrmError) as e:
+        bm.brmerge_precise(good, B)
+    assert e.value.status == 1
+    # column out of range
+    B = bm.gen.er(50, 50, 5, 1)
+    B.col = B.col.copy()
+    B.col[7] = 50
+    with pytest.raises(bm.BrmError):
+        bm.brmerge_upper(good, B)
+    # A column beyond K
+    A = bm.gen.er(50, 50, 5, 2)
+    A.col = A.col.copy()
+    A.col[-1] = 999
+    with pytest.raises(bm.BrmError):
+        bm.brmerge_precise(A, good)
+    # non-monotone row pointer
+    A = bm.gen.er(50, 50, 5, 2)
+    A.rpt = A.rpt.copy()
+    A.rpt[10] = A.rpt[11] + 2
+    with pytest.raises(bm.BrmError):
+        bm.brmerge_precise(A, good)
+    # the library is still usable afterwards
+    R = oracle.brmerge_precise(as_tuple(good), as_tuple(good))
+    assert_csr_equal(bm.brmerge_precise(good, good), R)
+
+
+def test_int32_overflow(gpu):
+    """nnz(C) = 65,536 * 64 * 1,024 > INT32_MAX must be refused by the _i32 entry."""
+    bm = gpu
+    rows, lists, width = 65536, 64, 1024
+    B = bm.CSR(np.arange(lists + 1, dtype=np.int32) * width, np.arange(lists * width, dtype=np.int32),
+               np.ones(lists * width), lists, lists * width)
+    A = bm.CSR(np.arange(rows + 1, dtype=np.int32) * lists, np.tile(np.arange(lists, dtype=np.int32), rows),
+               np.ones(rows * lists), rows, lists)
+    with pytest.raises(bm.BrmError) as e:
+        bm.brmerge_precise(A, B)
+    assert e.value.status == 3
+
+
+def test_allocation_failure_reports_nomem(gpu):
+    bm = gpu
+    A = bm.gen.rmat(11, seed=9)
+    L = bm.lib()
+    for n in (0, 3, 7):
+        L.brm_debug_fail_alloc_after(n)
+        with pytest.raises(bm.BrmError) as e:
+            bm.brmerge_precise(A, A)
+        assert e.value.status == 2
+    L.brm_debug_fail_alloc_after(-1)
+    R = oracle.brmerge_precise(as_tuple(A), as_tuple(A))
+    assert_csr_equal(bm.brmerge_precise(A, A), R)
