@@ -131,9 +131,13 @@ def main():
     dA = bm.DeviceCSR.upload(A)
     dB = dA if B is A else bm.DeviceCSR.upload(B)
 
-    for _ in range(args.warmup):
+    print(f"[bench] {args.workload}: M={A.M} nnzA={A.nnz} ip={total_ip} (generated in {t_gen:.1f}s)",
+          file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        tw = time.perf_counter()
         C = bm.spgemm_dev(dA, dB)
         C.free()
+        print(f"[bench] warmup {w}: {time.perf_counter() - tw:.3f}s", file=sys.stderr, flush=True)
 
     bm.set_param("profile", 0 if args.no_profile else 1)
     bm.prof_reset()
@@ -144,6 +148,7 @@ def main():
         C = bm.spgemm_dev(dA, dB, timing=t)
         timings.append(t.as_dict())
         nnz_c = C.nnz
+        print(f"[bench] step {_}: {t.total:.3f}s nnz(C)={nnz_c}", file=sys.stderr, flush=True)
         last = C
         if _ < args.steps - 1:
             C.free()

@@ -42,6 +42,7 @@ def rand_csr_tuple(rng, M, K, per_row_max, empty_frac=0.2, adversarial=True):
     return (np.array(rpt, np.int32), np.array(col, np.int32), np.array(val, np.float64), K)
 
 
-def csr_from_tuple(T):
+def csr_from_tuple(T, ncols=None):
     from brmerge_amd import CSR
-    return CSR(T[0], T[1], T[2], len(T[0]) - 1, T[3])
+    n = T[3] if len(T) > 3 else ncols
+    return CSR(T[0], T[1], T[2], len(T[0]) - 1, n)

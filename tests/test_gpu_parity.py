@@ -34,7 +34,7 @@ def tier(request, gpu):
 
 
 def run_both(bm, A, B, i64=False):
-    Ah, Bh = csr_from_tuple(A), csr_from_tuple(B)
+    Ah, Bh = csr_from_tuple(A, len(B[0]) - 1), csr_from_tuple(B)
     Cp = bm.brmerge_precise(Ah, Bh, i64=i64)
     Cu = bm.brmerge_upper(Ah, Bh, i64=i64)
     return Cp, Cu
