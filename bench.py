@@ -53,7 +53,8 @@ def kernel_bytes(name, A, L, ip, nnz_row, tier, st):
     Per tier: A entries (col 4 + val 8 + product offset 8 = 20 B), one B
     gather per product (col 4 + val 8 = 12 B; symbolic kernels read columns
     only, 4 B), C written once (12 B per nnz).  Upper-level merges read and
-    write 12 B per element.  Ping-pong traffic inside LDS is not counted."""
+    write 12 B per element (k_group: each input element once, each output
+    once).  Ping-pong traffic inside LDS is not counted."""
     sel = {"k_short_numeric": 1, "k_short_count": 1, "k_medium": 2, "k_sym_hash": 2, "k_sym_bitmap": 3}
     if name in sel:
         m = tier == sel[name]
@@ -63,10 +64,8 @@ def kernel_bytes(name, A, L, ip, nnz_row, tier, st):
         return 12 * nA + 4 * nip + 8 * int(m.sum())
     if name == "k_leaf":
         return 12 * st["leaf_ip"] + 12 * st["leaf_out"] + 20 * st["leaf_tasks"]
-    if name == "k_lvl_write":
-        return 12 * st["level_in"] + 12 * st["level_out"]
-    if name == "k_lvl_count":
-        return 4 * st["level_in"]
+    if name == "k_group":
+        return 12 * st["group_in"] + 12 * st["group_out"] + 48 * st["group_tasks"]
     return None
 
 
@@ -113,6 +112,8 @@ def main():
     ap.add_argument("--upper", action="store_true", help="also time BRMerge-Upper")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="do not bracket launches with hipEvents")
+    ap.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
+                    help="library tuning knob (brm_set_param), repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); multi-GPU sharding is "
@@ -122,6 +123,11 @@ def main():
     import brmerge_amd as bm
     if bm.lib().brm_init(0) != 0:
         raise RuntimeError(bm._lib.last_error())
+    params = {}
+    for kv in args.param:
+        k, v = kv.split("=", 1)
+        bm.set_param(k, int(v))
+        params[k] = int(v)
 
     t_gen = time.perf_counter()
     A, B = bm.gen.workload(args.workload)
@@ -195,7 +201,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload} C=A*A" if B is A else args.workload,
                    "rows": A.M, "nnz_a": A.nnz, "intermediate_products": total_ip, "nnz_c": int(nnz_c),
-                   "variant": "precise", "inputs": "resident in HBM", "generator_s": round(t_gen, 2)},
+                   "variant": "precise", "inputs": "resident in HBM", "generator_s": round(t_gen, 2),
+                   "params": params},
         "hbm_gbs_algorithmic": round(bytes_alg / (elapsed / args.steps) / 1e9, 2),
         "hbm_frac_algorithmic": round(bytes_alg / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roof,

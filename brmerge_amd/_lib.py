@@ -40,8 +40,8 @@ class UpperTimingC(ctypes.Structure):
 
 class StatsC(ctypes.Structure):
     _fields_ = [("rows_by_tier", ctypes.c_int64 * 4)] + [(n, ctypes.c_int64) for n in (
-        "total_ip", "nnz_c", "long_waves", "leaf_tasks", "leaf_ip", "leaf_out", "level_tasks", "level_tiles",
-        "level_in", "level_out")]
+        "total_ip", "nnz_c", "long_waves", "leaf_tasks", "leaf_ip", "leaf_out", "passes", "group_tasks",
+        "group_rounds", "group_in", "group_out")]
 
 
 # Every symbol include/brmerge.h and include/brmerge_gen.h declare.

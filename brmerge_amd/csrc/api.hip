@@ -30,7 +30,7 @@
 namespace brm {
 
 Params& params() {
-    static Params p{0, MED_CAP, 0, MED_CAP, 0};
+    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, 0};
     return p;
 }
 
@@ -194,70 +194,91 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     }
     auto row_bytes = [&](int k) {
         const int H = ceil_log2_host(len[k]);
-        return 24 * ip[k] + 8 * ((int64_t)2 << H) + (int64_t)(sizeof(PairTask) + sizeof(LeafTask)) * (len[k] + 34);
+        return 24 * ip[k] + 8 * ((int64_t)2 << H) +
+               (int64_t)(sizeof(GroupTask) + sizeof(LeafTask) + 2 * sizeof(Child)) * (len[k] + 16);
     };
 
-    int k0 = 0;
-    int* derr = o.err;
-    while (k0 < nlong) {
+    // ---- partition into waves and size every buffer for the largest wave
+    struct Wave {
+        int k0, k1;
+        int64_t ws_tot, sl_tot, leaf_cap, child_cap;
+        std::vector<int64_t> wsb, slb, seg;
+    };
+    std::vector<Wave> waves;
+    int64_t mx_ws = 0, mx_sl = 0, mx_leaf = 0, mx_child = 0, mx_task = 0;
+    int mx_rows = 0;
+    for (int k0 = 0; k0 < nlong;) {
         int k1 = k0;
         int64_t used = 0;
         while (k1 < nlong && (k1 == k0 || used + row_bytes(k1) <= budget)) used += row_bytes(k1++);
-        const int nw = k1 - k0;
-
-        std::vector<int64_t> wsb(nw), slb(nw);
-        int64_t ws_tot = 0, sl_tot = 0, leaf_cap = 0;
-        int maxH = 1;
-        std::vector<int64_t> capH(MAX_HEIGHT + 1, 0);
-        for (int k = 0; k < nw; ++k) {
-            wsb[k] = ws_tot;
-            slb[k] = sl_tot;
-            ws_tot += ip[k0 + k];
-            const int H = ceil_log2_host(len[k0 + k]);
-            sl_tot += (int64_t)2 << H;
-            leaf_cap += len[k0 + k];
-            maxH = std::max(maxH, H);
-            for (int h = 1; h <= std::max(H, 1); ++h) capH[h] += (int64_t)(len[k0 + k] >> h) + 2;
-        }
-        if (maxH > MAX_HEIGHT) {
-            set_err("long row with 2^%d lists exceeds the supported tree height", maxH);
-            return BRM_E_INVALID;
-        }
-        std::vector<int64_t> seg2(MAX_HEIGHT + 2, 0);
-        {
-            int64_t acc = 0;
-            for (int h = 0; h <= MAX_HEIGHT; ++h) {
-                seg2[h] = acc;
-                acc += capH[h];
+        Wave w;
+        w.k0 = k0;
+        w.k1 = k1;
+        w.ws_tot = w.sl_tot = w.leaf_cap = w.child_cap = 0;
+        w.wsb.resize(k1 - k0);
+        w.slb.resize(k1 - k0);
+        std::vector<int64_t> capP(MAX_PASSES + 1, 0);
+        for (int k = 0; k < k1 - k0; ++k) {
+            const int Lr = len[k0 + k];
+            const int H = ceil_log2_host(Lr);
+            const int J = H == 0 ? 1 : (H + PASS_BITS - 1) / PASS_BITS;
+            if (J > MAX_PASSES) {
+                set_err("long row with 2^%d lists exceeds the supported tree height", H);
+                return BRM_E_INVALID;
             }
-            seg2[MAX_HEIGHT + 1] = acc;
+            w.wsb[k] = w.ws_tot;
+            w.slb[k] = w.sl_tot;
+            w.ws_tot += ip[k0 + k];
+            w.sl_tot += (int64_t)2 << H;
+            w.leaf_cap += Lr;
+            w.child_cap += 2 * (int64_t)Lr + 2;
+            for (int j = 1; j <= J; ++j) capP[j] += (int64_t)(Lr >> std::min(j * PASS_BITS, H)) + 2;
         }
-        const int64_t ntask_cap = seg2[MAX_HEIGHT + 1];
+        w.seg.assign(MAX_PASSES + 2, 0);
+        for (int j = 0; j <= MAX_PASSES; ++j) w.seg[j + 1] = w.seg[j] + capP[j];
+        mx_ws = std::max(mx_ws, w.ws_tot);
+        mx_sl = std::max(mx_sl, w.sl_tot);
+        mx_leaf = std::max(mx_leaf, w.leaf_cap);
+        mx_child = std::max(mx_child, w.child_cap);
+        mx_task = std::max(mx_task, w.seg[MAX_PASSES + 1]);
+        mx_rows = std::max(mx_rows, k1 - k0);
+        waves.push_back(std::move(w));
+        k0 = k1;
+    }
 
-        Scratch wsc;
-        int *rows_w, *wcol0, *wcol1;
-        double *wval0, *wval1;
-        int64_t *wsb_d, *slb_d, *seg_d, *slots, *counters;
-        LeafTask* leaves;
-        PairTask* tasks;
-        BRMC(wsc.alloc(&rows_w, nw, "wave rows"));
-        BRMC(wsc.alloc(&wsb_d, nw, "wave ws_base"));
-        BRMC(wsc.alloc(&slb_d, nw, "wave slot_base"));
-        BRMC(wsc.alloc(&seg_d, MAX_HEIGHT + 2, "task segments"));
-        BRMC(wsc.alloc(&wcol0, ws_tot, "workspace"));
-        BRMC(wsc.alloc(&wcol1, ws_tot, "workspace"));
-        BRMC(wsc.alloc(&wval0, ws_tot, "workspace"));
-        BRMC(wsc.alloc(&wval1, ws_tot, "workspace"));
-        BRMC(wsc.alloc(&slots, sl_tot, "length slots"));
-        BRMC(wsc.alloc(&counters, CNT_N, "planner counters"));
-        BRMC(wsc.alloc(&leaves, leaf_cap, "leaf tasks"));
-        BRMC(wsc.alloc(&tasks, ntask_cap, "pair tasks"));
-        HIPC(hipMemcpyAsync(rows_w, rows.data() + k0, sizeof(int) * nw, hipMemcpyHostToDevice, s));
-        HIPC(hipMemcpyAsync(wsb_d, wsb.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice, s));
-        HIPC(hipMemcpyAsync(slb_d, slb.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice, s));
-        HIPC(hipMemcpyAsync(seg_d, seg2.data(), sizeof(int64_t) * (MAX_HEIGHT + 2), hipMemcpyHostToDevice, s));
+    Scratch wsc;
+    int *rows_w, *wcol0, *wcol1;
+    double *wval0, *wval1;
+    int64_t *wsb_d, *slb_d, *seg_d, *slots, *counters;
+    LeafTask* leaves;
+    GroupTask* tasks;
+    Child* children;
+    BRMC(wsc.alloc(&rows_w, mx_rows, "wave rows"));
+    BRMC(wsc.alloc(&wsb_d, mx_rows, "wave ws_base"));
+    BRMC(wsc.alloc(&slb_d, mx_rows, "wave slot_base"));
+    BRMC(wsc.alloc(&seg_d, MAX_PASSES + 2, "task segments"));
+    BRMC(wsc.alloc(&wcol0, mx_ws, "workspace"));
+    BRMC(wsc.alloc(&wcol1, mx_ws, "workspace"));
+    BRMC(wsc.alloc(&wval0, mx_ws, "workspace"));
+    BRMC(wsc.alloc(&wval1, mx_ws, "workspace"));
+    BRMC(wsc.alloc(&slots, mx_sl, "length slots"));
+    BRMC(wsc.alloc(&counters, CNT_N, "planner counters"));
+    BRMC(wsc.alloc(&leaves, mx_leaf, "leaf tasks"));
+    BRMC(wsc.alloc(&tasks, mx_task, "group tasks"));
+    BRMC(wsc.alloc(&children, mx_child, "group children"));
+    int* const wcol[2] = {wcol0, wcol1};
+    double* const wval[2] = {wval0, wval1};
+    int* derr = o.err;
+    const LdsShape sh = lds_shape(params().lds_cfg);
+
+    for (const Wave& w : waves) {
+        const int nw = w.k1 - w.k0;
+        HIPC(hipMemcpyAsync(rows_w, rows.data() + w.k0, sizeof(int) * nw, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(wsb_d, w.wsb.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(slb_d, w.slb.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(seg_d, w.seg.data(), sizeof(int64_t) * (MAX_PASSES + 2), hipMemcpyHostToDevice, s));
         HIPC(launch_fill_i64(counters, CNT_N, 0, s));
-        HIPC(launch_fill_i64(slots, sl_tot, 0, s));
+        HIPC(launch_fill_i64(slots, w.sl_tot, 0, s));
 
         LongArgs la;
         la.rows = rows_w;
@@ -272,68 +293,38 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         la.final_off = final_off;
         la.counters = counters;
         la.leaves = leaves;
+        la.leaves_cap = w.leaf_cap;
         la.tasks = tasks;
         la.task_seg = seg_d;
-        HIPC(launch_plan(la, params().leaf_cap, derr, s));
+        la.children = children;
+        la.child_cap = w.child_cap;
+        la.frontier_cap = std::min(params().leaf_cap, sh.cap);
+        la.lmax = sh.lmax;
+        la.log_lmax = sh.log_lmax;
+        HIPC(launch_plan(la, derr, s));
         std::vector<int64_t> cnt(CNT_N);
         BRMC(d2h(cnt.data(), counters, sizeof(int64_t) * CNT_N, s));
         BRMC(check_dev_err(derr, s, "long-row planner"));
 
+        HIPC(launch_leaf(leaves, cnt[CNT_LEAF], a, wcol, wval, slots, counters, o, s));
+        const int passes = (int)cnt[CNT_MAXP];
+        int64_t ntask = 0;
+        for (int j = 1; j <= passes && j <= MAX_PASSES; ++j) {
+            HIPC(launch_group(tasks + w.seg[j], cnt[CNT_TASK0 + j], children, a, wcol, wval, slots, counters, o, s));
+            ntask += cnt[CNT_TASK0 + j];
+        }
+        std::vector<int64_t> c2(CNT_N);
+        BRMC(d2h(c2.data(), counters, sizeof(int64_t) * CNT_N, s));
         t_stats.long_waves += 1;
         t_stats.leaf_tasks += cnt[CNT_LEAF];
         t_stats.leaf_ip += cnt[CNT_LEAF_IP];
-        int* const wcol[2] = {wcol0, wcol1};
-        double* const wval[2] = {wval0, wval1};
-        HIPC(launch_leaf(leaves, cnt[CNT_LEAF], a, wcol, wval, slots, counters + CNT_LEAF_OUT, derr, s));
-
-        LevelBufs wb;
-        wb.ws_col[0] = wcol0;
-        wb.ws_col[1] = wcol1;
-        wb.ws_val[0] = wval0;
-        wb.ws_val[1] = wval1;
-        wb.slots = slots;
-        const int hmax = (int)cnt[CNT_MAXH];
-        for (int h = 1; h <= hmax; ++h) {
-            const int64_t n = cnt[CNT_TASK0 + h];
-            if (n <= 0) continue;
-            const PairTask* th = tasks + seg2[h];
-            Scratch lsc;
-            int64_t *lens, *ntile, *toff;
-            BRMC(lsc.alloc(&lens, 2 * n, "level lens"));
-            BRMC(lsc.alloc(&ntile, n, "level tiles"));
-            BRMC(lsc.alloc(&toff, n + 1, "level tile offsets"));
-            HIPC(launch_level_len(th, n, wb, lens, ntile, s));
-            HIPC(scan_i64(ntile, n, toff, s));
-            int64_t ntiles = 0;
-            BRMC(d2h(&ntiles, toff + n, sizeof(int64_t), s));
-            int64_t *split, *tcnt, *tout;
-            BRMC(lsc.alloc(&split, 2 * ntiles, "level splits"));
-            BRMC(lsc.alloc(&tcnt, ntiles, "tile counts"));
-            BRMC(lsc.alloc(&tout, ntiles + 1, "tile offsets"));
-            HIPC(launch_level_split(th, n, toff, ntiles, lens, a, wb, split, s));
-            HIPC(launch_level_count(th, n, toff, ntiles, lens, split, a, wb, tcnt, s));
-            HIPC(scan_i64(tcnt, ntiles, tout, s));
-            HIPC(launch_level_write(th, n, toff, ntiles, lens, split, tout, a, wb, o, s));
-            t_stats.level_tasks += n;
-            t_stats.level_tiles += ntiles;
-            if (params().profile) {
-                std::vector<int64_t> hl(2 * n);
-                int64_t outn = 0;
-                BRMC(d2h(hl.data(), lens, sizeof(int64_t) * 2 * n, s));
-                BRMC(d2h(&outn, tout + ntiles, sizeof(int64_t), s));
-                for (int64_t x : hl) t_stats.level_in += x;
-                t_stats.level_out += outn;
-            }
-            HIPC(hipStreamSynchronize(s));
-            dev_flush_deferred();
-        }
-        {
-            int64_t lo = 0;
-            BRMC(d2h(&lo, counters + CNT_LEAF_OUT, sizeof(int64_t), s));
-            t_stats.leaf_out += lo;
-        }
+        t_stats.leaf_out += c2[CNT_LEAF_OUT];
+        t_stats.group_tasks += ntask;
+        t_stats.group_rounds += c2[CNT_ROUNDS];
+        t_stats.group_in += c2[CNT_GRP_IN];
+        t_stats.group_out += c2[CNT_GRP_OUT];
+        t_stats.passes = std::max<int64_t>(t_stats.passes, passes);
         dev_flush_deferred();
-        k0 = k1;
     }
     return BRM_OK;
 }
@@ -407,7 +398,9 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     HIPC(scan_alen(in.acol, in.brpt, in.nnzA, ipA, s));
 
     clk.begin(P_LB);
-    HIPC(launch_rowstats(a, params().med_cap, params().force_long, iprow, rowcnt, bins, bin_cnt, s));
+    const LdsShape shape = lds_shape(params().lds_cfg);
+    HIPC(launch_rowstats(a, std::min(params().med_cap, shape.cap), shape.lmax, params().force_long, iprow, rowcnt,
+                         bins, bin_cnt, s));
     int bc[BIN_COUNT];
     int64_t total_ip = 0;
     BRMC(d2h(bc, bin_cnt, sizeof(bc), s));
@@ -828,6 +821,8 @@ int brm_set_param(const char* name, int64_t value) {
     else if (!strcmp(name, "leaf_cap")) { if (value < 1 || value > MED_CAP) return BRM_E_INVALID; p.leaf_cap = (int)value; }
     else if (!strcmp(name, "force_long")) { p.force_long = value ? 1 : 0; }
     else if (!strcmp(name, "profile")) { p.profile = value ? 1 : 0; }
+    else if (!strcmp(name, "lds_cfg")) { if (value < 0 || value >= N_LDS_CFG) return BRM_E_INVALID; p.lds_cfg = (int)value; }
+    else if (!strcmp(name, "group_cfg")) { if (value < 0 || value >= N_GROUP_CFG) return BRM_E_INVALID; p.group_cfg = (int)value; }
     else return BRM_E_INVALID;
     return BRM_OK;
 }
@@ -840,6 +835,8 @@ int64_t brm_get_param(const char* name) {
     if (!strcmp(name, "leaf_cap")) return p.leaf_cap;
     if (!strcmp(name, "force_long")) return p.force_long;
     if (!strcmp(name, "profile")) return p.profile;
+    if (!strcmp(name, "lds_cfg")) return p.lds_cfg;
+    if (!strcmp(name, "group_cfg")) return p.group_cfg;
     return -1;
 }
 

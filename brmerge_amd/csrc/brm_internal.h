@@ -26,10 +26,12 @@ constexpr int MED_CAP = 4096;
 constexpr int MED_LMAX = 512;
 constexpr int MED_LOG_LMAX = 9;
 
-// Upper-level merge tiles (threads x items).
-constexpr int LVL_NT = 256;
-constexpr int LVL_ITEMS = 8;
-constexpr int LVL_TILE = LVL_NT * LVL_ITEMS;
+// Long-row group merge: the tree above the LDS frontier is merged PASS_BITS
+// levels per HBM pass; one workgroup streams up to GROUP_KMAX child lists
+// through an LDS window of GROUP_CAP elements.
+constexpr int PASS_BITS = 5;
+constexpr int GROUP_KMAX = 1 << PASS_BITS;
+constexpr int MAX_PASSES = 8;
 
 constexpr int MAX_HEIGHT = 32;
 
@@ -49,12 +51,21 @@ struct Src {
     double scale;  // VIRTUAL: A value
 };
 
-struct PairTask {
-    Src L, R;
+// Child list of a group merge: its source and its node id (first list
+// position relative to the group's subtree root).
+struct Child {
+    Src src;
+    int lid;
+    int pad;
+};
+
+// One node above the frontier, merged from its children in one pass.
+struct GroupTask {
     int row;
+    int nchild;
     int final_dst;    // 1: writes C (Precise) / C-bar (Upper); 0: workspace
     int dst_buf;
-    int pad;
+    int64_t cbase;    // first child in the child array
     int64_t dst_off;  // element offset in destination
     int64_t slot;     // workspace: length slot of this node
 };
@@ -63,17 +74,23 @@ struct LeafTask {
     int row;
     int ab, ae;       // A-entry range (an aligned subtree of the row's lists)
     int buf;
+    int final_dst;    // the subtree is the whole row: write the destination
+    int pad;
     int64_t dst_off;
     int64_t slot;
 };
 
-// Counters written by the planner.
-constexpr int CNT_LEAF = 0;
-constexpr int CNT_MAXH = 1;
-constexpr int CNT_LEAF_IP = 2;
-constexpr int CNT_LEAF_OUT = 3;
-constexpr int CNT_TASK0 = 8;  // CNT_TASK0 + h: pair tasks at height h
-constexpr int CNT_N = CNT_TASK0 + MAX_HEIGHT + 1;
+// Counters written by the planner and the merge kernels.
+constexpr int CNT_LEAF = 0;      // leaf tasks
+constexpr int CNT_MAXP = 1;      // passes needed by the wave
+constexpr int CNT_LEAF_IP = 2;   // products inside leaves
+constexpr int CNT_LEAF_OUT = 3;  // leaf output elements
+constexpr int CNT_GRP_IN = 4;    // group merge input elements
+constexpr int CNT_GRP_OUT = 5;   // group merge output elements
+constexpr int CNT_CHILD = 6;     // child descriptors used
+constexpr int CNT_ROUNDS = 7;    // streaming rounds of the group merges
+constexpr int CNT_TASK0 = 8;     // CNT_TASK0 + j: group tasks of pass j (1-based)
+constexpr int CNT_N = CNT_TASK0 + MAX_PASSES + 1;
 
 struct LongArgs {
     const int* rows;        // long rows of this wave
@@ -88,8 +105,14 @@ struct LongArgs {
     const int64_t* final_off;  // per row: C offset (Precise: crpt) or nullptr (Upper: ipA[arpt[row]])
     int64_t* counters;
     LeafTask* leaves;
-    PairTask* tasks;            // segmented by height
-    const int64_t* task_seg;    // [MAX_HEIGHT+1] segment start per height
+    int64_t leaves_cap;
+    GroupTask* tasks;           // segmented by pass
+    const int64_t* task_seg;    // [MAX_PASSES+2] segment start per pass
+    Child* children;
+    int64_t child_cap;
+    int frontier_cap; // frontier: products per LDS subtree
+    int lmax;         // frontier: lists per LDS subtree (a power of two)
+    int log_lmax;
 };
 
 // ------------------------------------------------------------------ error flags (device)
@@ -118,7 +141,23 @@ struct Params {
     int force_long;            // test hook: route every non-empty row to the long tier
     int leaf_cap;              // frontier capacity for long-row leaves (<= MED_CAP)
     int profile;               // bracket every kernel launch with hipEvents
+    int lds_cfg;               // LDS tier shape (threads, elements, lists): see lds_shape()
+    int group_cfg;             // group-merge shape (threads, window elements): see group_shape()
 };
+
+// Shapes of the LDS tree-merge kernels, selectable at run time.
+struct LdsShape {
+    int nt, cap, lmax, log_lmax;
+};
+constexpr int N_LDS_CFG = 3;
+inline LdsShape lds_shape(int cfg) {
+    switch (cfg) {
+        case 1: return {256, 2048, 256, 8};
+        case 2: return {128, 2048, 256, 8};
+        default: return {256, 4096, 512, 9};
+    }
+}
+constexpr int N_GROUP_CFG = 4;
 Params& params();
 
 // ------------------------------------------------------------------ profiling

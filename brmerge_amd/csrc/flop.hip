@@ -43,7 +43,8 @@ __global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __rest
     if (__ballot(bad) != 0ull && lane_id() == 0) atomicOr(err, (int)DERR_INVALID);
 }
 
-__global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int force_long, int64_t* __restrict__ iprow,
+__global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int med_lmax, int force_long,
+                                                 int64_t* __restrict__ iprow,
                                                  int64_t* __restrict__ rowcnt, int* __restrict__ bins,
                                                  int* __restrict__ bin_cnt) {
     const int i = blockIdx.x * NT + threadIdx.x;
@@ -60,7 +61,7 @@ __global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int for
             bin = BIN_LONG;
         } else if (ip <= SHORT_MAX) {
             bin = BIN_SHORT;
-        } else if (ip <= med_cap && L <= MED_LMAX) {
+        } else if (ip <= med_cap && L <= med_lmax) {
             bin = BIN_MEDIUM;
         } else {
             bin = BIN_LONG;
@@ -123,12 +124,12 @@ hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* er
     return hipGetLastError();
 }
 
-hipError_t launch_rowstats(const RowArgs& a, int med_cap, int force_long, int64_t* iprow, int64_t* rowcnt,
+hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int force_long, int64_t* iprow, int64_t* rowcnt,
                            int* bins, int* bin_cnt, hipStream_t s) {
     if (a.M <= 0) return hipSuccess;
     {
         ProfScope ps_("k_rowstats", s);
-        k_rowstats<<<nblk(a.M, NT), NT, 0, s>>>(a, med_cap, force_long, iprow, rowcnt, bins, bin_cnt);
+        k_rowstats<<<nblk(a.M, NT), NT, 0, s>>>(a, med_cap, med_lmax, force_long, iprow, rowcnt, bins, bin_cnt);
     }
     return hipGetLastError();
 }

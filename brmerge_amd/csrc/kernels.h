@@ -30,7 +30,7 @@ struct OutArgs {
 // flop.hip
 hipError_t launch_validate(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* err, hipStream_t s);
 hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* err, hipStream_t s);
-hipError_t launch_rowstats(const RowArgs& a, int med_cap, int force_long, int64_t* iprow, int64_t* nnzrow,
+hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int force_long, int64_t* iprow, int64_t* nnzrow,
                            int* bins /*[3][M]*/, int* bin_cnt /*[4]*/, hipStream_t s);
 
 // short.hip
@@ -42,26 +42,14 @@ hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t
 hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s);
 hipError_t launch_long_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
 hipError_t launch_leaf(const LeafTask* tasks, int64_t n, const RowArgs& a, int* const ws_col[2],
-                       double* const ws_val[2], int64_t* slots, int64_t* out_ctr, int* err, hipStream_t s);
+                       double* const ws_val[2], int64_t* slots, int64_t* counters, const OutArgs& o,
+                       hipStream_t s);
 
 // long.hip
-struct LevelBufs {
-    int* ws_col[2];
-    double* ws_val[2];
-    int64_t* slots;
-};
-hipError_t launch_plan(const LongArgs& la, int leaf_cap, int* err, hipStream_t s);
-hipError_t launch_level_len(const PairTask* t, int64_t n, const LevelBufs& wb, int64_t* lens /*[2n]*/,
-                            int64_t* ntile, hipStream_t s);
-hipError_t launch_level_split(const PairTask* t, int64_t n, const int64_t* toff, int64_t ntiles,
-                              const int64_t* lens, const RowArgs& a, const LevelBufs& wb,
-                              int64_t* split /*[2*ntiles]*/, hipStream_t s);
-hipError_t launch_level_count(const PairTask* t, int64_t n, const int64_t* toff, int64_t ntiles,
-                              const int64_t* lens, const int64_t* split, const RowArgs& a,
-                              const LevelBufs& wb, int64_t* tcnt, hipStream_t s);
-hipError_t launch_level_write(const PairTask* t, int64_t n, const int64_t* toff, int64_t ntiles,
-                              const int64_t* lens, const int64_t* split, const int64_t* tout,
-                              const RowArgs& a, const LevelBufs& wb, const OutArgs& o, hipStream_t s);
+hipError_t launch_plan(const LongArgs& la, int* err, hipStream_t s);
+hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
+                        int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
+                        const OutArgs& o, hipStream_t s);
 
 // assemble.hip
 hipError_t launch_compact(const int64_t* crpt, int M, const RowArgs& a, const int* cbar_col,

@@ -19,22 +19,10 @@
 
 #include "brm_device.h"
 #include "kernels.h"
+#include "lds_merge.h"
 
 namespace brm {
 namespace {
-
-template <int NT, int CAP, int LMAX>
-struct MergeSmem {
-    double val[2][CAP];
-    int col[2][CAP];
-    int lid[2][LMAX];
-    int loff[2][LMAX + 1];
-    int gs[LMAX + 1];
-    int sg[NT + 1];
-    int si[NT + 1];
-    int sj[NT + 1];
-    int scan[NT / BRM_WAVE + 1];
-};
 
 // Merge the non-empty lists of A entries [ab, ae) of one row.  Positions are
 // taken relative to ab, which must be aligned to the subtree size.  The
@@ -81,145 +69,10 @@ __device__ int64_t lds_tree_merge(MergeSmem<NT, CAP, LMAX>& sm, const RowArgs& a
     }
     __syncthreads();
 
-    // ---- MERGE levels
     int cur = 0;
-    while (n > 1) {
-        const int nxt = cur ^ 1;
-        int G = 0;
-        for (int cs = 0; cs < n; cs += NT) {
-            const int k = cs + t;
-            int flag = 0;
-            if (k < n) flag = (k == 0) || ((sm.lid[cur][k - 1] >> 1) != (sm.lid[cur][k] >> 1));
-            int tot;
-            const int ex = block_excl_scan<NT>(flag, sm.scan, &tot);
-            if (flag) {
-                sm.gs[G + ex] = k;
-                sm.lid[nxt][G + ex] = sm.lid[cur][k] >> 1;
-            }
-            G += tot;
-        }
-        if (t == 0) sm.gs[G] = n;
-        __syncthreads();
-        if (G == n) {  // no pair on this level: only the node ids move up
-            for (int k = t; k < n; k += NT) sm.lid[cur][k] = sm.lid[nxt][k];
-            __syncthreads();
-            continue;
-        }
-        const int* __restrict__ lo_ = sm.loff[cur];
-        const int* __restrict__ cc = sm.col[cur];
-        const double* __restrict__ cv = sm.val[cur];
-        const int Sl = lo_[n];
-        const int c = (Sl + NT - 1) / NT;
-
-        // split points of threads t (and NT)
-        for (int tt = t; tt <= NT; tt += NT) {
-            const int d = min(tt * c, Sl);
-            // largest g in [0, G] with goff(g) <= d, goff(G) = Sl
-            int glo = 0, glen = G + 1;
-            while (glen > 1) {
-                const int half = glen >> 1;
-                const int mid = glo + half;
-                const int gm = (mid == G) ? Sl : lo_[sm.gs[mid]];
-                if (gm <= d) { glo = mid; glen -= half; } else { glen = half; }
-            }
-            int i = 0, j = 0;
-            if (glo < G) {
-                const int k = sm.gs[glo];
-                const int L0 = lo_[k], R0 = lo_[k + 1];
-                const int nL = R0 - L0;
-                const bool pair = (sm.gs[glo + 1] - k) == 2;
-                const int nR = pair ? lo_[k + 2] - R0 : 0;
-                const int dd = d - L0;
-                if (!pair) { i = dd; j = 0; }
-                else {
-                    merge_split([&](int x) { return cc[L0 + x]; }, nL, [&](int y) { return cc[R0 + y]; }, nR,
-                                dd, &i, &j);
-                }
-            }
-            sm.sg[tt] = glo;
-            sm.si[tt] = i;
-            sm.sj[tt] = j;
-        }
-        __syncthreads();
-
-        const int g0 = sm.sg[t], i0 = sm.si[t], j0 = sm.sj[t];
-        const int ge = sm.sg[t + 1], ie = sm.si[t + 1], je = sm.sj[t + 1];
-
-        // count pass
-        int cnt = 0;
-        {
-            int g = g0, i = i0, j = j0;
-            while (g < ge || (g == ge && (i < ie || j < je))) {
-                const int k = sm.gs[g];
-                const int L0 = lo_[k], R0 = lo_[k + 1];
-                const bool pair = (sm.gs[g + 1] - k) == 2;
-                const int nL = R0 - L0;
-                const int nR = pair ? lo_[k + 2] - R0 : 0;
-                const int iend = (g == ge) ? ie : nL;
-                const int jend = (g == ge) ? je : nR;
-                while (i < iend && j < jend) {
-                    const int x = cc[L0 + i], y = cc[R0 + j];
-                    i += (x <= y);
-                    j += (y <= x);
-                    cnt++;
-                }
-                cnt += (iend - i) + (jend - j);
-                g++;
-                i = 0;
-                j = 0;
-            }
-        }
-        int total;
-        const int obase = block_excl_scan<NT>(cnt, sm.scan, &total);
-
-        // write pass
-        {
-            int* __restrict__ oc = sm.col[nxt];
-            double* __restrict__ ov = sm.val[nxt];
-            int o = obase;
-            int g = g0, i = i0, j = j0;
-            while (g < ge || (g == ge && (i < ie || j < je))) {
-                const int k = sm.gs[g];
-                const int L0 = lo_[k], R0 = lo_[k + 1];
-                const bool pair = (sm.gs[g + 1] - k) == 2;
-                const int nL = R0 - L0;
-                const int nR = pair ? lo_[k + 2] - R0 : 0;
-                const int iend = (g == ge) ? ie : nL;
-                const int jend = (g == ge) ? je : nR;
-                if (i == 0 && j == 0) sm.loff[nxt][g] = o;
-                while (i < iend && j < jend) {
-                    const int x = cc[L0 + i], y = cc[R0 + j];
-                    if (x == y) {
-                        oc[o] = x;
-                        ov[o] = fadd(cv[L0 + i], cv[R0 + j]);
-                        i++;
-                        j++;
-                    } else if (x < y) {
-                        oc[o] = x;
-                        ov[o] = cv[L0 + i];
-                        i++;
-                    } else {
-                        oc[o] = y;
-                        ov[o] = cv[R0 + j];
-                        j++;
-                    }
-                    o++;
-                }
-                for (; i < iend; ++i, ++o) { oc[o] = cc[L0 + i]; ov[o] = cv[L0 + i]; }
-                for (; j < jend; ++j, ++o) { oc[o] = cc[R0 + j]; ov[o] = cv[R0 + j]; }
-                g++;
-                i = 0;
-                j = 0;
-            }
-        }
-        if (t == 0) sm.loff[nxt][G] = total;
-        __syncthreads();
-        cur = nxt;
-        n = G;
-    }
+    int len = merge_levels<NT, CAP, LMAX>(sm, n, &cur);
 
     // ---- OUTPUT
-    const int len = (n == 1) ? sm.loff[cur][1] : 0;
     const int wlen = (int)min((int64_t)len, dcap);  // never write past the destination
     for (int e = t; e < wlen; e += NT) {
         dcol[e] = sm.col[cur][e];
@@ -256,16 +109,31 @@ struct WsPtrs {
 
 template <int NT, int CAP, int LMAX>
 __global__ __launch_bounds__(NT) void k_leaf(const LeafTask* __restrict__ tasks, RowArgs a, WsPtrs ws,
-                                             int64_t* __restrict__ slots, int64_t* out_ctr, int* err) {
+                                             int64_t* __restrict__ slots, int64_t* counters, OutArgs o) {
     extern __shared__ __align__(16) unsigned char smraw[];
     auto& sm = *reinterpret_cast<MergeSmem<NT, CAP, LMAX>*>(smraw);
     const LeafTask tk = tasks[blockIdx.x];
-    int* dc = (tk.buf ? ws.col1 : ws.col0) + tk.dst_off;
-    double* dv = (tk.buf ? ws.val1 : ws.val0) + tk.dst_off;
-    const int64_t len = lds_tree_merge<NT, CAP, LMAX>(sm, a, tk.ab, tk.ae, dc, dv, a.ipA[tk.ae] - a.ipA[tk.ab], err);
+    int* dc;
+    double* dv;
+    int64_t cap;
+    if (tk.final_dst) {
+        dc = o.ccol + tk.dst_off;
+        dv = o.cval + tk.dst_off;
+        cap = o.upper ? a.ipA[tk.ae] - a.ipA[tk.ab] : o.crpt[tk.row + 1] - o.crpt[tk.row];
+    } else {
+        dc = (tk.buf ? ws.col1 : ws.col0) + tk.dst_off;
+        dv = (tk.buf ? ws.val1 : ws.val0) + tk.dst_off;
+        cap = a.ipA[tk.ae] - a.ipA[tk.ab];
+    }
+    const int64_t len = lds_tree_merge<NT, CAP, LMAX>(sm, a, tk.ab, tk.ae, dc, dv, cap, o.err);
     if (threadIdx.x == 0) {
-        slots[tk.slot] = len;
-        atomicAdd((unsigned long long*)out_ctr, (unsigned long long)len);
+        if (tk.final_dst) {
+            if (o.upper) o.rowlen[tk.row] = len;
+            else if (len != cap) atomicOr(o.err, (int)DERR_COUNT_MISMATCH);
+        } else {
+            slots[tk.slot] = len;
+        }
+        atomicAdd((unsigned long long*)&counters[CNT_LEAF_OUT], (unsigned long long)len);
     }
 }
 
@@ -360,12 +228,10 @@ constexpr size_t bitmap_smem() {
     return sizeof(unsigned) * BM_WORDS + sizeof(int) * (BM_CHUNK + 8) + sizeof(int64_t) * (BM_NT / BRM_WAVE + 2);
 }
 
-}  // namespace
-
-hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    using SM = MergeSmem<MED_NT, MED_CAP, MED_LMAX>;
-    auto kern = k_medium<MED_NT, MED_CAP, MED_LMAX>;
+template <int NT, int CAP, int LMAX>
+hipError_t medium_numeric_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
+    using SM = MergeSmem<NT, CAP, LMAX>;
+    auto kern = k_medium<NT, CAP, LMAX>;
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(SM));
@@ -374,28 +240,50 @@ hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const
     }
     {
         ProfScope ps_("k_medium", s);
-        kern<<<n, MED_NT, sizeof(SM), s>>>(rows, n, a, o);
+        kern<<<n, NT, sizeof(SM), s>>>(rows, n, a, o);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_leaf(const LeafTask* tasks, int64_t n, const RowArgs& a, int* const ws_col[2],
-                       double* const ws_val[2], int64_t* slots, int64_t* out_ctr, int* err, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    using SM = MergeSmem<MED_NT, MED_CAP, MED_LMAX>;
-    auto kern = k_leaf<MED_NT, MED_CAP, MED_LMAX>;
+template <int NT, int CAP, int LMAX>
+hipError_t leaf_t(const LeafTask* tasks, int64_t n, const RowArgs& a, const WsPtrs& ws, int64_t* slots,
+                  int64_t* counters, const OutArgs& o, hipStream_t s) {
+    using SM = MergeSmem<NT, CAP, LMAX>;
+    auto kern = k_leaf<NT, CAP, LMAX>;
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(SM));
         if (e != hipSuccess) return e;
         attr = true;
     }
-    WsPtrs ws{ws_col[0], ws_col[1], ws_val[0], ws_val[1]};
     {
         ProfScope ps_("k_leaf", s);
-        kern<<<(unsigned)n, MED_NT, sizeof(SM), s>>>(tasks, a, ws, slots, out_ctr, err);
+        kern<<<(unsigned)n, NT, sizeof(SM), s>>>(tasks, a, ws, slots, counters, o);
     }
     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    switch (params().lds_cfg) {
+        case 1: return medium_numeric_t<256, 2048, 256>(rows, n, a, o, s);
+        case 2: return medium_numeric_t<128, 2048, 256>(rows, n, a, o, s);
+        default: return medium_numeric_t<256, 4096, 512>(rows, n, a, o, s);
+    }
+}
+
+hipError_t launch_leaf(const LeafTask* tasks, int64_t n, const RowArgs& a, int* const ws_col[2],
+                       double* const ws_val[2], int64_t* slots, int64_t* counters, const OutArgs& o,
+                       hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    WsPtrs ws{ws_col[0], ws_col[1], ws_val[0], ws_val[1]};
+    switch (params().lds_cfg) {
+        case 1: return leaf_t<256, 2048, 256>(tasks, n, a, ws, slots, counters, o, s);
+        case 2: return leaf_t<128, 2048, 256>(tasks, n, a, ws, slots, counters, o, s);
+        default: return leaf_t<256, 4096, 512>(tasks, n, a, ws, slots, counters, o, s);
+    }
 }
 
 hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {

@@ -1,24 +1,29 @@
 // Long-row tier: rows whose products do not fit one workgroup's LDS.
 //
-// The row's merge tree (aligned binary tree over A-entry positions, as in
+// The row's merge tree (the aligned binary tree over A-entry positions of
 // BRMerge-SpGEMM/inc/spgemm.h:639-707) is cut at its frontier: the maximal
-// aligned subtrees whose products fit the LDS tile.  Each frontier subtree
-// with at least two non-empty lists is merged in LDS by the leaf kernel
-// (lds.hip) into a workspace buffer; a frontier subtree with one non-empty
-// list is left in place as a "virtual" list (that B row, scaled on the fly).
-// Every node above the frontier that holds two or more non-empty lists is a
-// pair task at its height; heights are processed bottom-up, each height by
-// four kernels over all its tasks:
-//   len    -- input lengths and tile counts per task,
-//   split  -- merge-path split of every 2048-element tile (global binary search),
-//   count  -- per-tile output count (columns only, merged in LDS),
-//   write  -- per-tile merge in LDS, staged and streamed out coalesced.
-// Workspace storage alternates between two buffers by tree height (the
-// reference's ping-pong), each node stored at its product offset in the row.
+// aligned subtrees whose products fit an LDS tile.  A frontier subtree with
+// two or more non-empty lists is merged in LDS by the leaf kernel (lds.hip)
+// into a workspace buffer; one with a single non-empty list stays in place
+// as a "virtual" list (that B row, scaled on the fly).
+//
+// Above the frontier the tree is merged PASS_BITS levels per pass: pass j
+// materialises every node at height h_j = min(j * PASS_BITS, H) from its
+// children -- the highest materialised nodes below it (frontier nodes or
+// pass j-1 nodes), at most 2^PASS_BITS of them.  One workgroup streams a
+// node's children through LDS in rounds: every child contributes a window
+// proportional to what it has left, and all elements up to the smallest
+// window-end column are merged (tree order, in LDS) and written out.  Every
+// contribution to a column <= that bound is present in the round, so each
+// column is summed by the reference's bracketing, bit for bit.
+//
+// Workspace buffers alternate by pass: pass j writes buffer j&1 at the node's
+// product offset in the row and reads buffer (j-1)&1.
 #include <limits.h>
 
 #include "brm_device.h"
 #include "kernels.h"
+#include "lds_merge.h"
 
 namespace brm {
 namespace {
@@ -32,6 +37,7 @@ __device__ __forceinline__ int node_end(int L, int s, int h) {
 __device__ __forceinline__ int64_t heap_idx(int H, int s, int h) {
     return ((int64_t)1 << (H - h)) - 1 + (s >> h);
 }
+__device__ __forceinline__ int pass_height(int j, int H) { return min(j * PASS_BITS, H); }
 
 struct NodeK {
     int kind;
@@ -48,12 +54,28 @@ __device__ NodeK node_kind(const int64_t* ipr, int L, int s, int h) {
     return {SRC_STORED, -1, ip};
 }
 
-__device__ __forceinline__ bool fits(const int64_t* ipr, int L, int s, int h, int cap) {
+struct Fit {
+    int cap;       // products per LDS subtree
+    int lmax;      // lists per LDS subtree
+    int log_lmax;
+};
+
+__device__ __forceinline__ bool fits(const int64_t* ipr, int L, int s, int h, const Fit& f) {
     const int e = node_end(L, s, h);
-    return (e - s) <= MED_LMAX && (ipr[e] - ipr[s]) <= cap;
+    return (e - s) <= f.lmax && (ipr[e] - ipr[s]) <= f.cap;
 }
 
-__device__ Src make_src(const LongArgs& la, int a0, const int64_t* ipr, int L, int H, int s, int h,
+// Height of the frontier node that contains position p.
+__device__ int frontier_height(const int64_t* ipr, int L, int H, int p, const Fit& f) {
+    const int hcap = min(H, f.log_lmax);
+    int hf = 0;
+    if (fits(ipr, L, p, 0, f)) {
+        while (hf < hcap && fits(ipr, L, p & ~((2 << hf) - 1), hf + 1, f)) hf++;
+    }
+    return hf;
+}
+
+__device__ Src make_src(const LongArgs& la, int a0, const int64_t* ipr, int L, int H, int s, int h, int buf,
                         int64_t wsb, int64_t slb) {
     const NodeK nk = node_kind(ipr, L, s, h);
     Src r;
@@ -69,383 +91,340 @@ __device__ Src make_src(const LongArgs& la, int a0, const int64_t* ipr, int L, i
         r.len = nk.ip;
         r.scale = la.aval[ai];
     } else if (nk.kind == SRC_STORED) {
-        r.buf = h & 1;
+        r.buf = buf;
         r.off = wsb + (ipr[s] - ipr[0]);
         r.slot = slb + heap_idx(H, s, h);
     }
     return r;
 }
 
-__device__ void emit_task(const LongArgs& la, int h, const PairTask& T, int* err) {
-    const unsigned long long idx = atomicAdd((unsigned long long*)&la.counters[CNT_TASK0 + h], 1ull);
-    const int64_t cap = la.task_seg[h + 1] - la.task_seg[h];
-    if ((int64_t)idx >= cap) {
+// Emits the group task of node (s, hj), the pass-j node (or the root).
+__device__ void emit_group(const LongArgs& la, int row, int a0, const int64_t* ipr, int L, int H, int j, int s,
+                           int hj, int hprev, const Fit& cap, int64_t wsb, int64_t slb, int64_t final_off, int* err) {
+    const bool root = (hj == H);
+    const NodeK u = node_kind(ipr, L, s, hj);
+    if (u.kind == SRC_EMPTY || (u.kind == SRC_VIRTUAL && !root)) return;
+    const int e = node_end(L, s, hj);
+    // count, then reserve, then write the children
+    int nc = 0;
+    for (int q = s; q < e;) {
+        const int hq = frontier_height(ipr, L, H, q, cap);
+        const int ch = hq >= hprev ? hq : hprev;
+        if (node_kind(ipr, L, q, ch).kind != SRC_EMPTY) nc++;
+        q += 1 << ch;
+    }
+    const unsigned long long cb = atomicAdd((unsigned long long*)&la.counters[CNT_CHILD], (unsigned long long)nc);
+    if ((int64_t)cb + nc > la.child_cap || nc > GROUP_KMAX) {
         atomicOr(err, (int)DERR_CAPACITY);
         return;
     }
-    la.tasks[la.task_seg[h] + (int64_t)idx] = T;
+    int k = 0;
+    for (int q = s; q < e;) {
+        const int hq = frontier_height(ipr, L, H, q, cap);
+        const int ch = hq >= hprev ? hq : hprev;
+        const Src src = make_src(la, a0, ipr, L, H, q, ch, (j - 1) & 1, wsb, slb);
+        if (src.kind != SRC_EMPTY) {
+            Child c;
+            c.src = src;
+            c.lid = q - s;
+            c.pad = 0;
+            la.children[cb + k] = c;
+            k++;
+        }
+        q += 1 << ch;
+    }
+    GroupTask T;
+    T.row = row;
+    T.nchild = nc;
+    T.cbase = (int64_t)cb;
+    if (root) {
+        T.final_dst = 1;
+        T.dst_buf = 0;
+        T.dst_off = final_off;
+        T.slot = -1;
+    } else {
+        T.final_dst = 0;
+        T.dst_buf = j & 1;
+        T.dst_off = wsb + (ipr[s] - ipr[0]);
+        T.slot = slb + heap_idx(H, s, hj);
+    }
+    const unsigned long long idx = atomicAdd((unsigned long long*)&la.counters[CNT_TASK0 + j], 1ull);
+    if ((int64_t)idx >= la.task_seg[j + 1] - la.task_seg[j]) {
+        atomicOr(err, (int)DERR_CAPACITY);
+        return;
+    }
+    la.tasks[la.task_seg[j] + (int64_t)idx] = T;
 }
 
-__global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int cap, int* err) {
+__global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int* err) {
+    const Fit cap{la.frontier_cap, la.lmax, la.log_lmax};
     const int r = blockIdx.x;
     const int row = la.rows[r];
     const int a0 = la.arpt[row];
     const int L = la.arpt[row + 1] - a0;
     const int64_t* ipr = la.ipA + a0;
     const int H = ceil_log2i(L);
+    const int J = H == 0 ? 1 : (H + PASS_BITS - 1) / PASS_BITS;
     const int64_t wsb = la.ws_base[r], slb = la.slot_base[r];
     const int64_t final_off = la.final_off ? la.final_off[row] : ipr[0];
-    const int hcap = min(H, MED_LOG_LMAX);
 
     for (int p = threadIdx.x; p < L; p += PL_NT) {
-        int hf = 0;
-        if (fits(ipr, L, p, 0, cap)) {
-            while (hf < hcap && fits(ipr, L, p & ~((2 << hf) - 1), hf + 1, cap)) hf++;
-        }
+        const int hf = frontier_height(ipr, L, H, p, cap);
         const int s = p & ~((1 << hf) - 1);
         if (s != p) continue;
         const NodeK fk = node_kind(ipr, L, s, hf);
+        if (hf == H) {  // the whole row is one frontier node
+            if (fk.kind == SRC_STORED) {
+                const unsigned long long li = atomicAdd((unsigned long long*)&la.counters[CNT_LEAF], 1ull);
+                atomicAdd((unsigned long long*)&la.counters[CNT_LEAF_IP], (unsigned long long)fk.ip);
+                if ((int64_t)li >= la.leaves_cap) { atomicOr(err, (int)DERR_CAPACITY); continue; }
+                LeafTask lt;
+                lt.row = row;
+                lt.ab = a0;
+                lt.ae = a0 + L;
+                lt.buf = 0;
+                lt.final_dst = 1;
+                lt.pad = 0;
+                lt.dst_off = final_off;
+                lt.slot = -1;
+                la.leaves[li] = lt;
+            } else if (fk.kind == SRC_VIRTUAL) {
+                emit_group(la, row, a0, ipr, L, H, 1, 0, H, 0, cap, wsb, slb, final_off, err);
+            }
+            continue;
+        }
+        const int jf = hf / PASS_BITS + 1;  // pass that consumes this frontier node
         if (fk.kind == SRC_STORED) {
             const unsigned long long li = atomicAdd((unsigned long long*)&la.counters[CNT_LEAF], 1ull);
             atomicAdd((unsigned long long*)&la.counters[CNT_LEAF_IP], (unsigned long long)fk.ip);
+            if ((int64_t)li >= la.leaves_cap) { atomicOr(err, (int)DERR_CAPACITY); continue; }
             LeafTask lt;
             lt.row = row;
             lt.ab = a0 + s;
             lt.ae = a0 + node_end(L, s, hf);
-            lt.buf = hf & 1;
+            lt.buf = (jf - 1) & 1;
+            lt.final_dst = 0;
+            lt.pad = 0;
             lt.dst_off = wsb + (ipr[s] - ipr[0]);
             lt.slot = slb + heap_idx(H, s, hf);
             la.leaves[li] = lt;
         }
-        if (hf == H) {  // the whole row is one frontier node: copy it to the destination
-            PairTask T;
-            T.L = make_src(la, a0, ipr, L, H, s, hf, wsb, slb);
-            T.R = T.L;
-            T.R.kind = SRC_EMPTY;
-            T.row = row;
-            T.final_dst = 1;
-            T.dst_buf = 0;
-            T.pad = 0;
-            T.dst_off = final_off;
-            T.slot = -1;
-            emit_task(la, 1, T, err);
-            continue;
-        }
-        for (int h = hf + 1; h <= H && (s & ((1 << h) - 1)) == 0; ++h) {
-            const bool root = (h == H);
-            const NodeK u = node_kind(ipr, L, s, h);
-            if (u.kind != SRC_STORED && !root) continue;
-            PairTask T;
-            if (u.kind == SRC_STORED) {
-                T.L = make_src(la, a0, ipr, L, H, s, h - 1, wsb, slb);
-                const int s2 = s + (1 << (h - 1));
-                if (s2 < L) {
-                    T.R = make_src(la, a0, ipr, L, H, s2, h - 1, wsb, slb);
-                } else {
-                    T.R = T.L;
-                    T.R.kind = SRC_EMPTY;
-                }
-            } else {  // the root holds a single non-empty list
-                T.L = make_src(la, a0, ipr, L, H, s, h, wsb, slb);
-                T.R = T.L;
-                T.R.kind = SRC_EMPTY;
-            }
-            T.row = row;
-            T.pad = 0;
-            if (root) {
-                T.final_dst = 1;
-                T.dst_buf = 0;
-                T.dst_off = final_off;
-                T.slot = -1;
-            } else {
-                T.final_dst = 0;
-                T.dst_buf = h & 1;
-                T.dst_off = wsb + (ipr[s] - ipr[0]);
-                T.slot = slb + heap_idx(H, s, h);
-            }
-            emit_task(la, h, T, err);
+        // pass nodes whose leftmost frontier node starts at s
+        for (int j = jf; j <= J; ++j) {
+            const int hj = pass_height(j, H);
+            if (s & ((1 << hj) - 1)) break;
+            emit_group(la, row, a0, ipr, L, H, j, s, hj, pass_height(j - 1, H), cap, wsb, slb, final_off, err);
         }
     }
-    if (threadIdx.x == 0)
-        atomicMax((unsigned long long*)&la.counters[CNT_MAXH], (unsigned long long)(H < 1 ? 1 : H));
+    if (threadIdx.x == 0) atomicMax((unsigned long long*)&la.counters[CNT_MAXP], (unsigned long long)J);
 }
 
-// ---------------------------------------------------------------- levels
+// ---------------------------------------------------------------- group merge
+template <int GNT, int GCAP>
+struct GroupSmem {
+    MergeSmem<GNT, GCAP, GROUP_KMAX> m;
+    double scale[GROUP_KMAX];
+    int64_t off[GROUP_KMAX];
+    int64_t len[GROUP_KMAX];
+    int64_t cur[GROUP_KMAX];
+    int kind[GROUP_KMAX];
+    int buf[GROUP_KMAX];
+    int lid[GROUP_KMAX];
+    int w[GROUP_KMAX];
+    int woff[GROUP_KMAX + 1];
+    int take[GROUP_KMAX];
+    int orig[GROUP_KMAX];
+    int nlists;
+    int done;
+};
+
 struct WsView {
     int* col0;
     int* col1;
     double* val0;
     double* val1;
-    int64_t* slots;
 };
 
-__device__ __forceinline__ int64_t src_len(const Src& s, const int64_t* slots) {
-    return s.kind == SRC_EMPTY ? 0 : (s.kind == SRC_VIRTUAL ? s.len : slots[s.slot]);
-}
-__device__ __forceinline__ int src_col(const Src& s, const RowArgs& a, const WsView& w, int64_t x) {
-    if (s.kind == SRC_VIRTUAL) return a.bcol[s.off + x];
-    return (s.buf ? w.col1 : w.col0)[s.off + x];
-}
-__device__ __forceinline__ double src_val(const Src& s, const RowArgs& a, const WsView& w, int64_t x) {
-    if (s.kind == SRC_VIRTUAL) return fmul(s.scale, a.bval[s.off + x]);
-    return (s.buf ? w.val1 : w.val0)[s.off + x];
-}
-
-constexpr int LT_NT = LVL_NT;
-constexpr int LT_CAP = LVL_TILE + 8;
-
-__global__ void k_lvl_len(const PairTask* __restrict__ tasks, int64_t n, WsView w, int64_t* __restrict__ lens,
-                          int64_t* __restrict__ ntile) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= n) return;
-    const PairTask T = tasks[t];
-    const int64_t lL = src_len(T.L, w.slots), lR = src_len(T.R, w.slots);
-    lens[2 * t] = lL;
-    lens[2 * t + 1] = lR;
-    const int64_t raw = lL + lR;
-    ntile[t] = raw <= 0 ? 1 : (raw + LVL_TILE - 1) / LVL_TILE;
-}
-
-__global__ void k_lvl_split(const PairTask* __restrict__ tasks, int64_t n, const int64_t* __restrict__ toff,
-                            int64_t ntiles, const int64_t* __restrict__ lens, RowArgs a, WsView w,
-                            int64_t* __restrict__ split) {
-    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (k >= ntiles) return;
-    const int64_t t = upper_idx(toff, 0, (int)n, k);
-    const int64_t d = (k - toff[t]) * (int64_t)LVL_TILE;
-    const PairTask T = tasks[t];
-    int64_t i = 0, j = 0;
-    if (d > 0) {
-        merge_split64([&](int64_t x) { return src_col(T.L, a, w, x); }, lens[2 * t],
-                      [&](int64_t y) { return src_col(T.R, a, w, y); }, lens[2 * t + 1], d, &i, &j);
+template <int GNT, int GCAP>
+__global__ __launch_bounds__(GNT) void k_group(const GroupTask* __restrict__ tasks,
+                                                    const Child* __restrict__ children, RowArgs a, WsView ws,
+                                                    int64_t* __restrict__ slots, int64_t* counters, OutArgs o) {
+    __shared__ GroupSmem<GNT, GCAP> g;
+    const int t = threadIdx.x;
+    const int lane = lane_id();
+    const GroupTask T = tasks[blockIdx.x];
+    const int k = T.nchild;
+    if (t < k) {
+        const Child c = children[T.cbase + t];
+        g.kind[t] = c.src.kind;
+        g.buf[t] = c.src.buf;
+        g.off[t] = c.src.off;
+        g.scale[t] = c.src.scale;
+        g.lid[t] = c.lid;
+        g.len[t] = c.src.kind == SRC_VIRTUAL ? c.src.len : slots[c.src.slot];
+        g.cur[t] = 0;
     }
-    split[2 * k] = i;
-    split[2 * k + 1] = j;
-}
-
-struct TileSmem {
-    double vl[LT_CAP];
-    double vr[LT_CAP];
-    double ov[LT_CAP];
-    int cl[LT_CAP];
-    int cr[LT_CAP];
-    int oc[LT_CAP];
-    int si[LT_NT + 1];
-    int sj[LT_NT + 1];
-    int64_t red[LT_NT / BRM_WAVE + 1];
-    int64_t misc[4];
-};
-
-// Loads this tile's two input ranges into LDS; returns (nl, nr) and task id.
-template <bool VALS>
-__device__ void load_tile(TileSmem& sm, const PairTask& T, int64_t i0, int64_t i1, int64_t j0, int64_t j1,
-                          const RowArgs& a, const WsView& w) {
-    const int nl = (int)(i1 - i0), nr = (int)(j1 - j0);
-    for (int x = threadIdx.x; x < nl; x += LT_NT) {
-        sm.cl[x] = src_col(T.L, a, w, i0 + x);
-        if (VALS) sm.vl[x] = src_val(T.L, a, w, i0 + x);
-    }
-    for (int y = threadIdx.x; y < nr; y += LT_NT) {
-        sm.cr[y] = src_col(T.R, a, w, j0 + y);
-        if (VALS) sm.vr[y] = src_val(T.R, a, w, j0 + y);
-    }
-}
-
-__device__ void tile_splits(TileSmem& sm, int nl, int nr) {
-    const int tot = nl + nr;
-    const int c = (tot + LT_NT - 1) / LT_NT;
-    for (int tt = threadIdx.x; tt <= LT_NT; tt += LT_NT) {
-        const int d = min(tt * c, tot);
-        int i, j;
-        merge_split([&](int x) { return sm.cl[x]; }, nl, [&](int y) { return sm.cr[y]; }, nr, d, &i, &j);
-        sm.si[tt] = i;
-        sm.sj[tt] = j;
-    }
-}
-
-__device__ __forceinline__ void tile_bounds(const int64_t* toff, int64_t n, const int64_t* lens,
-                                            const int64_t* split, int64_t k, int64_t* t_out, int64_t* i0,
-                                            int64_t* j0, int64_t* i1, int64_t* j1) {
-    const int64_t t = upper_idx(toff, 0, (int)n, k);
-    const int64_t last = toff[t + 1] - 1;
-    *t_out = t;
-    *i0 = split[2 * k];
-    *j0 = split[2 * k + 1];
-    if (k == last) {
-        *i1 = lens[2 * t];
-        *j1 = lens[2 * t + 1];
-    } else {
-        *i1 = split[2 * k + 2];
-        *j1 = split[2 * k + 3];
-    }
-}
-
-__global__ __launch_bounds__(LT_NT) void k_lvl_count(const PairTask* __restrict__ tasks, int64_t n,
-                                                     const int64_t* __restrict__ toff, const int64_t* __restrict__ lens,
-                                                     const int64_t* __restrict__ split, RowArgs a, WsView w,
-                                                     int64_t* __restrict__ tcnt) {
-    __shared__ TileSmem sm;
-    const int64_t k = blockIdx.x;
-    int64_t t, i0, j0, i1, j1;
-    tile_bounds(toff, n, lens, split, k, &t, &i0, &j0, &i1, &j1);
-    const PairTask T = tasks[t];
-    load_tile<false>(sm, T, i0, i1, j0, j1, a, w);
-    const int nl = (int)(i1 - i0), nr = (int)(j1 - j0);
-    __syncthreads();
-    tile_splits(sm, nl, nr);
-    __syncthreads();
-    int i = sm.si[threadIdx.x], j = sm.sj[threadIdx.x];
-    const int ie = sm.si[threadIdx.x + 1], je = sm.sj[threadIdx.x + 1];
-    int64_t cnt = 0;
-    while (i < ie && j < je) {
-        const int x = sm.cl[i], y = sm.cr[j];
-        i += (x <= y);
-        j += (y <= x);
-        cnt++;
-    }
-    cnt += (ie - i) + (je - j);
-    const int64_t tot = block_sum<LT_NT>(cnt, sm.red);
-    if (threadIdx.x == 0) tcnt[k] = tot;
-}
-
-__global__ __launch_bounds__(LT_NT) void k_lvl_write(const PairTask* __restrict__ tasks, int64_t n,
-                                                     const int64_t* __restrict__ toff, const int64_t* __restrict__ lens,
-                                                     const int64_t* __restrict__ split, const int64_t* __restrict__ tout,
-                                                     RowArgs a, WsView w, OutArgs o) {
-    __shared__ TileSmem sm;
-    const int64_t k = blockIdx.x;
-    int64_t t, i0, j0, i1, j1;
-    tile_bounds(toff, n, lens, split, k, &t, &i0, &j0, &i1, &j1);
-    const PairTask T = tasks[t];
-    load_tile<true>(sm, T, i0, i1, j0, j1, a, w);
-    const int nl = (int)(i1 - i0), nr = (int)(j1 - j0);
-    __syncthreads();
-    tile_splits(sm, nl, nr);
-    __syncthreads();
-    int i = sm.si[threadIdx.x], j = sm.sj[threadIdx.x];
-    const int ie = sm.si[threadIdx.x + 1], je = sm.sj[threadIdx.x + 1];
-    // this thread's output position within the tile = merged count before (i, j)
-    int64_t cnt = 0;
-    {
-        int ii = i, jj = j;
-        while (ii < ie && jj < je) {
-            const int x = sm.cl[ii], y = sm.cr[jj];
-            ii += (x <= y);
-            jj += (y <= x);
-            cnt++;
-        }
-        cnt += (ie - ii) + (je - jj);
-    }
-    int64_t tile_tot;
-    int o_ = (int)block_excl_scan<LT_NT>(cnt, sm.red, &tile_tot);
-    while (i < ie && j < je) {
-        const int x = sm.cl[i], y = sm.cr[j];
-        if (x == y) {
-            sm.oc[o_] = x;
-            sm.ov[o_] = fadd(sm.vl[i], sm.vr[j]);
-            i++;
-            j++;
-        } else if (x < y) {
-            sm.oc[o_] = x;
-            sm.ov[o_] = sm.vl[i];
-            i++;
-        } else {
-            sm.oc[o_] = y;
-            sm.ov[o_] = sm.vr[j];
-            j++;
-        }
-        o_++;
-    }
-    for (; i < ie; ++i, ++o_) { sm.oc[o_] = sm.cl[i]; sm.ov[o_] = sm.vl[i]; }
-    for (; j < je; ++j, ++o_) { sm.oc[o_] = sm.cr[j]; sm.ov[o_] = sm.vr[j]; }
     __syncthreads();
 
-    const int64_t first = toff[t];
-    const int64_t pre = tout[k] - tout[first];
     int* dc;
     double* dv;
+    int64_t dcap;
     if (T.final_dst) {
         dc = o.ccol + T.dst_off;
         dv = o.cval + T.dst_off;
+        dcap = o.upper ? INT64_MAX : o.crpt[T.row + 1] - o.crpt[T.row];
     } else {
-        dc = (T.dst_buf ? w.col1 : w.col0) + T.dst_off;
-        dv = (T.dst_buf ? w.val1 : w.val0) + T.dst_off;
+        dc = (T.dst_buf ? ws.col1 : ws.col0) + T.dst_off;
+        dv = (T.dst_buf ? ws.val1 : ws.val0) + T.dst_off;
+        dcap = INT64_MAX;
     }
-    int64_t wlim = tile_tot;
-    if (T.final_dst && !o.upper) {  // Precise: never write past the row of C
-        const int64_t room = o.crpt[T.row + 1] - o.crpt[T.row] - pre;
-        wlim = room < 0 ? 0 : (room < wlim ? room : wlim);
+
+    int64_t in_total = 0;
+    if (t < BRM_WAVE) {
+        const int64_t l = t < k ? g.len[t] : 0;
+        in_total = wave_sum(l);
     }
-    for (int x = threadIdx.x; x < (int)wlim; x += LT_NT) {
-        dc[pre + x] = sm.oc[x];
-        dv[pre + x] = sm.ov[x];
-    }
-    if (threadIdx.x == 0 && k == toff[t + 1] - 1) {
-        const int64_t len = tout[k + 1] - tout[first];
-        if (T.final_dst) {
-            if (!o.upper) {
-                if (len != o.crpt[T.row + 1] - o.crpt[T.row]) atomicOr(o.err, (int)DERR_COUNT_MISMATCH);
-            } else {
-                o.rowlen[T.row] = len;
+    int64_t out_n = 0;
+    int rounds = 0;
+    for (;;) {
+        // ---- window sizes (wavefront 0): proportional to what each child has left
+        if (t < BRM_WAVE) {
+            const int64_t rem = t < k ? g.len[t] - g.cur[t] : 0;
+            const int64_t R = wave_sum(rem);
+            int64_t wv;
+            if (R <= GCAP) wv = rem;
+            else wv = rem ? max((int64_t)1, (int64_t)(GCAP - GROUP_KMAX) * rem / R) : 0;
+            if (wv > rem) wv = rem;
+            const int wi = (int)wv;
+            const int inc = wave_incl_scan(wi);
+            if (t < k) {
+                g.w[t] = wi;
+                g.woff[t] = inc - wi;
             }
-        } else {
-            w.slots[T.slot] = len;
+            if (t == BRM_WAVE - 1) {
+                g.woff[k] = inc;
+                g.done = (R == 0);
+            }
         }
+        __syncthreads();
+        if (g.done) break;
+        rounds++;
+        const int W = g.woff[k];
+        // ---- load the windows into buffer 1
+        {
+            int p = 0;
+            for (int e = t; e < W; e += GNT) {
+                p = gallop_idx(g.woff, p, k, e);
+                const int64_t x = g.cur[p] + (e - g.woff[p]);
+                int c;
+                double v;
+                if (g.kind[p] == SRC_VIRTUAL) {
+                    c = a.bcol[g.off[p] + x];
+                    v = fmul(g.scale[p], a.bval[g.off[p] + x]);
+                } else {
+                    c = (g.buf[p] ? ws.col1 : ws.col0)[g.off[p] + x];
+                    v = (g.buf[p] ? ws.val1 : ws.val0)[g.off[p] + x];
+                }
+                g.m.col[1][e] = c;
+                g.m.val[1][e] = v;
+            }
+        }
+        __syncthreads();
+        // ---- merge bound: smallest last column of a partially loaded child;
+        //      every child contributes its window prefix up to that bound
+        if (t < BRM_WAVE) {
+            int lastc = INT_MAX;
+            if (t < k && g.w[t] < g.len[t] - g.cur[t]) lastc = g.m.col[1][g.woff[t] + g.w[t] - 1];
+#pragma unroll
+            for (int d = BRM_WAVE / 2; d > 0; d >>= 1) lastc = min(lastc, __shfl_xor(lastc, d, BRM_WAVE));
+            int take = 0;
+            if (t < k) {
+                const int* wc = g.m.col[1] + g.woff[t];
+                int lo = 0, hi = g.w[t];
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (wc[mid] <= lastc) lo = mid + 1; else hi = mid;
+                }
+                take = lo;
+            }
+            const uint64_t nz = __ballot(take > 0);
+            const int idx = __popcll(nz & ((1ull << lane) - 1ull));
+            const int ex = wave_incl_scan(take) - take;
+            if (t < k) g.take[t] = take;
+            if (take > 0) {
+                g.m.lid[0][idx] = g.lid[t];
+                g.m.loff[0][idx] = ex;
+                g.orig[idx] = t;
+            }
+            if (t == BRM_WAVE - 1) {
+                g.nlists = __popcll(nz);
+                g.m.loff[0][__popcll(nz)] = ex + take;
+            }
+        }
+        __syncthreads();
+        // ---- gather the taken prefixes into buffer 0, contiguous, in child order
+        const int nl = g.nlists;
+        const int S = g.m.loff[0][nl];
+        {
+            int q = 0;
+            for (int e = t; e < S; e += GNT) {
+                q = gallop_idx(g.m.loff[0], q, nl, e);
+                const int src = g.woff[g.orig[q]] + (e - g.m.loff[0][q]);
+                g.m.col[0][e] = g.m.col[1][src];
+                g.m.val[0][e] = g.m.val[1][src];
+            }
+        }
+        __syncthreads();
+        int cb;
+        const int len = merge_levels<GNT, GCAP, GROUP_KMAX>(g.m, nl, &cb);
+        // ---- stream out
+        for (int e = t; e < len; e += GNT) {
+            const int64_t o_ = out_n + e;
+            if (o_ < dcap) {
+                dc[o_] = g.m.col[cb][e];
+                dv[o_] = g.m.val[cb][e];
+            }
+        }
+        out_n += len;
+        if (t < k) g.cur[t] += g.take[t];
+        __syncthreads();
+    }
+
+    if (t == 0) {
+        if (T.final_dst) {
+            if (o.upper) o.rowlen[T.row] = out_n;
+            else if (out_n != dcap) atomicOr(o.err, (int)DERR_COUNT_MISMATCH);
+        } else {
+            slots[T.slot] = out_n;
+        }
+        atomicAdd((unsigned long long*)&counters[CNT_GRP_IN], (unsigned long long)in_total);
+        atomicAdd((unsigned long long*)&counters[CNT_GRP_OUT], (unsigned long long)out_n);
+        atomicAdd((unsigned long long*)&counters[CNT_ROUNDS], (unsigned long long)rounds);
     }
 }
-
-inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
 }  // namespace
 
-hipError_t launch_plan(const LongArgs& la, int leaf_cap, int* err, hipStream_t s) {
+hipError_t launch_plan(const LongArgs& la, int* err, hipStream_t s) {
     if (la.nrows <= 0) return hipSuccess;
     {
         ProfScope ps_("k_plan", s);
-        k_plan<<<la.nrows, PL_NT, 0, s>>>(la, leaf_cap, err);
+        k_plan<<<la.nrows, PL_NT, 0, s>>>(la, err);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_level_len(const PairTask* t, int64_t n, const LevelBufs& wb, int64_t* lens, int64_t* ntile,
-                            hipStream_t s) {
-    WsView w{wb.ws_col[0], wb.ws_col[1], wb.ws_val[0], wb.ws_val[1], wb.slots};
-    {
-        ProfScope ps_("k_lvl_len", s);
-        k_lvl_len<<<nblk(n, 256), 256, 0, s>>>(t, n, w, lens, ntile);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_level_split(const PairTask* t, int64_t n, const int64_t* toff, int64_t ntiles,
-                              const int64_t* lens, const RowArgs& a, const LevelBufs& wb, int64_t* split,
-                              hipStream_t s) {
-    WsView w{wb.ws_col[0], wb.ws_col[1], wb.ws_val[0], wb.ws_val[1], wb.slots};
-    {
-        ProfScope ps_("k_lvl_split", s);
-        k_lvl_split<<<nblk(ntiles, 256), 256, 0, s>>>(t, n, toff, ntiles, lens, a, w, split);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_level_count(const PairTask* t, int64_t n, const int64_t* toff, int64_t ntiles,
-                              const int64_t* lens, const int64_t* split, const RowArgs& a, const LevelBufs& wb,
-                              int64_t* tcnt, hipStream_t s) {
-    WsView w{wb.ws_col[0], wb.ws_col[1], wb.ws_val[0], wb.ws_val[1], wb.slots};
-    {
-        ProfScope ps_("k_lvl_count", s);
-        k_lvl_count<<<(unsigned)ntiles, LT_NT, 0, s>>>(t, n, toff, lens, split, a, w, tcnt);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_level_write(const PairTask* t, int64_t n, const int64_t* toff, int64_t ntiles,
-                              const int64_t* lens, const int64_t* split, const int64_t* tout, const RowArgs& a,
-                              const LevelBufs& wb, const OutArgs& o, hipStream_t s) {
-    WsView w{wb.ws_col[0], wb.ws_col[1], wb.ws_val[0], wb.ws_val[1], wb.slots};
-    {
-        ProfScope ps_("k_lvl_write", s);
-        k_lvl_write<<<(unsigned)ntiles, LT_NT, 0, s>>>(t, n, toff, lens, split, tout, a, w, o);
+hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
+                        int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
+                        const OutArgs& o, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    WsView w{ws_col[0], ws_col[1], ws_val[0], ws_val[1]};
+    ProfScope ps_("k_group", s);
+    switch (params().group_cfg) {
+        case 1: k_group<128, 2048><<<(unsigned)n, 128, 0, s>>>(tasks, children, a, w, slots, counters, o); break;
+        case 2: k_group<64, 1024><<<(unsigned)n, 64, 0, s>>>(tasks, children, a, w, slots, counters, o); break;
+        case 3: k_group<256, 4096><<<(unsigned)n, 256, 0, s>>>(tasks, children, a, w, slots, counters, o); break;
+        default: k_group<256, 2048><<<(unsigned)n, 256, 0, s>>>(tasks, children, a, w, slots, counters, o); break;
     }
     return hipGetLastError();
 }

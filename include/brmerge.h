@@ -145,7 +145,8 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
                        int64_t *row_ip, int64_t *total, void *stream);
 
 /* Tuning knobs (names: "ws_budget_bytes", "med_cap", "leaf_cap",
- * "force_long", "profile"); returns BRM_E_INVALID for unknown names or values. */
+ * "force_long", "profile", "lds_cfg", "group_cfg"); returns BRM_E_INVALID for
+ * unknown names or values.  Knobs never change results, only speed. */
 int brm_set_param(const char *name, int64_t value);
 int64_t brm_get_param(const char *name);
 
@@ -162,8 +163,7 @@ int brm_memcpy_d2h(void *dst_host, const void *src_dev, uint64_t bytes);
 void brm_prof_reset(void);
 int brm_prof_report(char *buf, uint64_t cap);
 
-/* Work counters of the calling thread's last SpGEMM (filled always; the
- * per-level element counts only when "profile" is set).  Used to turn kernel
+/* Work counters of the calling thread's last SpGEMM.  Used to turn kernel
  * times into algorithmic bandwidth. */
 typedef struct brm_stats {
     int64_t rows_by_tier[4];   /* empty, short (<= 64 products), medium (LDS), long */
@@ -173,10 +173,11 @@ typedef struct brm_stats {
     int64_t leaf_tasks;        /* LDS-merged aligned subtrees of long rows */
     int64_t leaf_ip;           /* products inside those subtrees */
     int64_t leaf_out;          /* merged elements they produce */
-    int64_t level_tasks;       /* pair merges above the subtrees */
-    int64_t level_tiles;
-    int64_t level_in;          /* merge input elements (profile only) */
-    int64_t level_out;         /* merge output elements (profile only) */
+    int64_t passes;            /* group-merge passes (5 tree levels each) */
+    int64_t group_tasks;       /* tree nodes merged above the subtrees */
+    int64_t group_rounds;      /* LDS streaming rounds of those merges */
+    int64_t group_in;          /* their input elements */
+    int64_t group_out;         /* their output elements */
 } brm_stats;
 int brm_last_stats(brm_stats *out);
 
