@@ -15,6 +15,7 @@
 // counts, tile counts) between phases.
 #include <stdio.h>
 #include <stdarg.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -161,6 +162,26 @@ int check_dev_err(const int* derr, hipStream_t s, const char* stage) {
     return BRM_OK;
 }
 
+// Host-side trace of the orchestration (BRM_TRACE=1 in the environment):
+// elapsed milliseconds since the start of the call, after synchronising the
+// stream, so gaps between kernels become visible.
+struct HostTrace {
+    bool on;
+    double t0;
+    hipStream_t s;
+    explicit HostTrace(hipStream_t st) : on(getenv("BRM_TRACE") != nullptr), s(st) {
+        t0 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    }
+    void mark(const char* what, int64_t x = -1) {
+        if (!on) return;
+        const double t1 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+        (void)hipStreamSynchronize(s);
+        const double t2 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+        fprintf(stderr, "[brm] %9.3f ms (+%8.3f sync) %s %lld\n", (t1 - t0) * 1e3, (t2 - t1) * 1e3, what,
+                (long long)x);
+    }
+};
+
 int ceil_log2_host(int x) {
     int h = 0;
     while ((1LL << h) < (long long)x) ++h;
@@ -171,6 +192,8 @@ int ceil_log2_host(int x) {
 int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, const int64_t* final_off,
              hipStream_t s, Scratch& sc) {
     if (nlong <= 0) return BRM_OK;
+    HostTrace tr(s);
+    tr.mark("run_long start", nlong);
     int64_t* ip_d;
     int* len_d;
     BRMC(sc.alloc(&ip_d, nlong, "long-row ip"));
@@ -184,13 +207,17 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     sc.free_now(ip_d);
     sc.free_now(len_d);
 
-    // Wave partition by workspace budget.
+    // Wave partition by workspace budget.  The budget is quantised (4 GiB)
+    // and the workspace rounded up (2^27 elements) so that repeated calls of
+    // the same product size their buffers identically and reuse the cached
+    // blocks of the previous call.
     int64_t budget = params().ws_budget_bytes;
     if (budget <= 0) {
         size_t fr = 0, tot = 0;
         HIPC(hipMemGetInfo(&fr, &tot));
-        budget = (int64_t)(fr + dev_cached_bytes()) / 10 * 7;
-        if (budget < (int64_t)(256 << 20)) budget = 256 << 20;
+        budget = std::min((int64_t)(fr + dev_cached_bytes()) / 10 * 7, (int64_t)tot / 2);
+        budget = budget / (int64_t(4) << 30) * (int64_t(4) << 30);
+        if (budget < (int64_t)(1) << 30) budget = (int64_t)1 << 30;
     }
     auto row_bytes = [&](int k) {
         const int H = ceil_log2_host(len[k]);
@@ -198,6 +225,7 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
                (int64_t)(sizeof(GroupTask) + sizeof(LeafTask) + 2 * sizeof(Child)) * (len[k] + 16);
     };
 
+    tr.mark("long info read back");
     // ---- partition into waves and size every buffer for the largest wave
     struct Wave {
         int k0, k1;
@@ -246,6 +274,13 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         k0 = k1;
     }
 
+    auto round_up = [](int64_t n, int64_t g) { return (n + g - 1) / g * g; };
+    mx_ws = round_up(mx_ws, (int64_t)1 << 27);
+    mx_sl = round_up(mx_sl, (int64_t)1 << 24);
+    mx_leaf = round_up(mx_leaf, (int64_t)1 << 20);
+    mx_child = round_up(mx_child, (int64_t)1 << 20);
+    mx_task = round_up(mx_task, (int64_t)1 << 20);
+    mx_rows = (int)round_up(mx_rows, (int64_t)1 << 16);
     Scratch wsc;
     int *rows_w, *wcol0, *wcol1;
     double *wval0, *wval1;
@@ -266,6 +301,7 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     BRMC(wsc.alloc(&leaves, mx_leaf, "leaf tasks"));
     BRMC(wsc.alloc(&tasks, mx_task, "group tasks"));
     BRMC(wsc.alloc(&children, mx_child, "group children"));
+    tr.mark("workspace allocated", mx_ws);
     int* const wcol[2] = {wcol0, wcol1};
     double* const wval[2] = {wval0, wval1};
     int* derr = o.err;
@@ -301,17 +337,21 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         la.frontier_cap = std::min(params().leaf_cap, sh.cap);
         la.lmax = sh.lmax;
         la.log_lmax = sh.log_lmax;
+        tr.mark("wave uploaded", nw);
         HIPC(launch_plan(la, derr, s));
+        tr.mark("planned");
         std::vector<int64_t> cnt(CNT_N);
         BRMC(d2h(cnt.data(), counters, sizeof(int64_t) * CNT_N, s));
         BRMC(check_dev_err(derr, s, "long-row planner"));
 
         HIPC(launch_leaf(leaves, cnt[CNT_LEAF], a, wcol, wval, slots, counters, o, s));
+        tr.mark("leaves", cnt[CNT_LEAF]);
         const int passes = (int)cnt[CNT_MAXP];
         int64_t ntask = 0;
         for (int j = 1; j <= passes && j <= MAX_PASSES; ++j) {
             HIPC(launch_group(tasks + w.seg[j], cnt[CNT_TASK0 + j], children, a, wcol, wval, slots, counters, o, s));
             ntask += cnt[CNT_TASK0 + j];
+            tr.mark("group pass", cnt[CNT_TASK0 + j]);
         }
         std::vector<int64_t> c2(CNT_N);
         BRMC(d2h(c2.data(), counters, sizeof(int64_t) * CNT_N, s));
@@ -362,6 +402,8 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     Scratch sc;
     const int M = in.M;
     memset(&t_stats, 0, sizeof(t_stats));
+    HostTrace tr(s);
+    tr.mark("spgemm start", M);
 
     clk.begin(P_PRE);
     int* derr;
@@ -435,6 +477,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         BRMC(sc.alloc(&ccol, nnzC, "C col"));
         BRMC(sc.alloc(&cval, nnzC, "C val"));
 
+        tr.mark("C allocated", nnzC);
         clk.begin(P_COMPUTE);
         o.upper = 0;
         o.crpt = crpt;
@@ -442,7 +485,9 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         o.cval = cval;
         HIPC(launch_short_numeric(rows_short, bc[BIN_SHORT], a, o, s));
         HIPC(launch_medium_numeric(rows_med, bc[BIN_MEDIUM], a, o, s));
+        tr.mark("short+medium numeric");
         BRMC(run_long(a, rows_long, bc[BIN_LONG], o, crpt, s, sc));
+        tr.mark("long numeric");
         BRMC(check_dev_err(derr, s, "numeric"));
     } else {
         clk.begin(P_ALLOC);

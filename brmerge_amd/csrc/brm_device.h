@@ -61,6 +61,27 @@ __device__ __forceinline__ T block_excl_scan(T x, T* sm, T* total) {
     return res;
 }
 
+// Exclusive scan with a single barrier.  `sm` (NT/64 elements) must not be
+// written again before every thread of the block has passed another barrier.
+template <int NT, typename T>
+__device__ __forceinline__ T block_excl_scan_1b(T x, T* sm, T* total) {
+    constexpr int NW = NT / BRM_WAVE;
+    const int lane = lane_id();
+    const int w = threadIdx.x / BRM_WAVE;
+    const T inc = wave_incl_scan(x);
+    if (lane == BRM_WAVE - 1) sm[w] = inc;
+    __syncthreads();
+    T pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        const T v = sm[i];
+        pre += (i < w) ? v : T(0);
+        tot += v;
+    }
+    *total = tot;
+    return pre + inc - x;
+}
+
 template <int NT, typename T>
 __device__ __forceinline__ T block_sum(T x, T* sm) {
     T tot;

@@ -143,7 +143,7 @@ __device__ int merge_levels(MergeSmem<NT, CAP, LMAX>& sm, int n, int* cur_out) {
             }
         }
         int total;
-        const int obase = block_excl_scan<NT>(cnt, sm.scan, &total);
+        const int obase = block_excl_scan_1b<NT>(cnt, sm.scan, &total);
 
         // write pass
         {

@@ -220,6 +220,13 @@ __global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int* err) {
 }
 
 // ---------------------------------------------------------------- group merge
+struct WsView {
+    int* col0;
+    int* col1;
+    double* val0;
+    double* val1;
+};
+
 template <int GNT, int GCAP>
 struct GroupSmem {
     MergeSmem<GNT, GCAP, GROUP_KMAX> m;
@@ -238,17 +245,61 @@ struct GroupSmem {
     int done;
 };
 
-struct WsView {
-    int* col0;
-    int* col1;
-    double* val0;
-    double* val1;
-};
+// Window sizes of the next round (wavefront 0 computes, all threads read):
+// proportional to what each child has left, all of it when everything fits.
+template <int GNT, int GCAP>
+__device__ __forceinline__ void group_windows(GroupSmem<GNT, GCAP>& g, int k) {
+    const int t = threadIdx.x;
+    if (t < BRM_WAVE) {
+        const int64_t rem = t < k ? g.len[t] - g.cur[t] : 0;
+        const int64_t R = wave_sum(rem);
+        int64_t wv;
+        if (R <= GCAP) wv = rem;
+        else wv = rem ? max((int64_t)1, (int64_t)(GCAP - GROUP_KMAX) * rem / R) : 0;
+        if (wv > rem) wv = rem;
+        const int wi = (int)wv;
+        const int inc = wave_incl_scan(wi);
+        if (t < k) {
+            g.w[t] = wi;
+            g.woff[t] = inc - wi;
+        }
+        if (t == BRM_WAVE - 1) {
+            g.woff[k] = inc;
+            g.done = (R == 0);
+        }
+    }
+}
+
+// Issues the global loads of the current windows into registers.
+template <int GNT, int GCAP, int ITEMS>
+__device__ __forceinline__ void group_fetch(GroupSmem<GNT, GCAP>& g, int k, const RowArgs& a, const WsView& ws,
+                                            int (&rc)[ITEMS], double (&rv)[ITEMS], double (&rs)[ITEMS],
+                                            unsigned& vmask) {
+    const int W = g.woff[k];
+    int p = 0;
+    vmask = 0u;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int e = threadIdx.x + i * GNT;
+        if (e < W) {
+            p = gallop_idx(g.woff, p, k, e);
+            const int64_t x = g.off[p] + g.cur[p] + (e - g.woff[p]);
+            const bool virt = g.kind[p] == SRC_VIRTUAL;
+            const int* cbase = virt ? a.bcol : (g.buf[p] ? ws.col1 : ws.col0);
+            const double* vbase = virt ? a.bval : (g.buf[p] ? ws.val1 : ws.val0);
+            rc[i] = cbase[x];
+            rv[i] = vbase[x];
+            rs[i] = g.scale[p];
+            if (virt) vmask |= 1u << i;  // virtual: the product is rounded when stored
+        }
+    }
+}
 
 template <int GNT, int GCAP>
 __global__ __launch_bounds__(GNT) void k_group(const GroupTask* __restrict__ tasks,
-                                                    const Child* __restrict__ children, RowArgs a, WsView ws,
-                                                    int64_t* __restrict__ slots, int64_t* counters, OutArgs o) {
+                                               const Child* __restrict__ children, RowArgs a, WsView ws,
+                                               int64_t* __restrict__ slots, int64_t* counters, OutArgs o) {
+    constexpr int ITEMS = GCAP / GNT;
     __shared__ GroupSmem<GNT, GCAP> g;
     const int t = threadIdx.x;
     const int lane = lane_id();
@@ -286,47 +337,27 @@ __global__ __launch_bounds__(GNT) void k_group(const GroupTask* __restrict__ tas
     }
     int64_t out_n = 0;
     int rounds = 0;
-    for (;;) {
-        // ---- window sizes (wavefront 0): proportional to what each child has left
-        if (t < BRM_WAVE) {
-            const int64_t rem = t < k ? g.len[t] - g.cur[t] : 0;
-            const int64_t R = wave_sum(rem);
-            int64_t wv;
-            if (R <= GCAP) wv = rem;
-            else wv = rem ? max((int64_t)1, (int64_t)(GCAP - GROUP_KMAX) * rem / R) : 0;
-            if (wv > rem) wv = rem;
-            const int wi = (int)wv;
-            const int inc = wave_incl_scan(wi);
-            if (t < k) {
-                g.w[t] = wi;
-                g.woff[t] = inc - wi;
-            }
-            if (t == BRM_WAVE - 1) {
-                g.woff[k] = inc;
-                g.done = (R == 0);
-            }
-        }
-        __syncthreads();
-        if (g.done) break;
+    static_assert(ITEMS <= 32, "window items per thread");
+    int rc[ITEMS];
+    double rv[ITEMS], rs[ITEMS];
+    unsigned vmask = 0u;
+
+    group_windows<GNT, GCAP>(g, k);
+    __syncthreads();
+    bool done = g.done;
+    if (!done) group_fetch<GNT, GCAP, ITEMS>(g, k, a, ws, rc, rv, rs, vmask);
+    while (!done) {
         rounds++;
-        const int W = g.woff[k];
-        // ---- load the windows into buffer 1
+        // ---- the fetched windows into buffer 1 (products rounded here)
         {
-            int p = 0;
-            for (int e = t; e < W; e += GNT) {
-                p = gallop_idx(g.woff, p, k, e);
-                const int64_t x = g.cur[p] + (e - g.woff[p]);
-                int c;
-                double v;
-                if (g.kind[p] == SRC_VIRTUAL) {
-                    c = a.bcol[g.off[p] + x];
-                    v = fmul(g.scale[p], a.bval[g.off[p] + x]);
-                } else {
-                    c = (g.buf[p] ? ws.col1 : ws.col0)[g.off[p] + x];
-                    v = (g.buf[p] ? ws.val1 : ws.val0)[g.off[p] + x];
+            const int W = g.woff[k];
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i) {
+                const int e = t + i * GNT;
+                if (e < W) {
+                    g.m.col[1][e] = rc[i];
+                    g.m.val[1][e] = ((vmask >> i) & 1u) ? fmul(rs[i], rv[i]) : rv[i];
                 }
-                g.m.col[1][e] = c;
-                g.m.val[1][e] = v;
             }
         }
         __syncthreads();
@@ -350,16 +381,17 @@ __global__ __launch_bounds__(GNT) void k_group(const GroupTask* __restrict__ tas
             const uint64_t nz = __ballot(take > 0);
             const int idx = __popcll(nz & ((1ull << lane) - 1ull));
             const int ex = wave_incl_scan(take) - take;
-            if (t < k) g.take[t] = take;
             if (take > 0) {
                 g.m.lid[0][idx] = g.lid[t];
                 g.m.loff[0][idx] = ex;
                 g.orig[idx] = t;
+                g.take[idx] = g.woff[t];  // window start of the idx-th taken list
             }
             if (t == BRM_WAVE - 1) {
                 g.nlists = __popcll(nz);
                 g.m.loff[0][__popcll(nz)] = ex + take;
             }
+            if (t < k) g.cur[t] += take;  // consumed: the next windows start after it
         }
         __syncthreads();
         // ---- gather the taken prefixes into buffer 0, contiguous, in child order
@@ -369,12 +401,16 @@ __global__ __launch_bounds__(GNT) void k_group(const GroupTask* __restrict__ tas
             int q = 0;
             for (int e = t; e < S; e += GNT) {
                 q = gallop_idx(g.m.loff[0], q, nl, e);
-                const int src = g.woff[g.orig[q]] + (e - g.m.loff[0][q]);
+                const int src = g.take[q] + (e - g.m.loff[0][q]);
                 g.m.col[0][e] = g.m.col[1][src];
                 g.m.val[0][e] = g.m.val[1][src];
             }
         }
+        // ---- next round's windows; their loads overlap this round's merge
+        group_windows<GNT, GCAP>(g, k);
         __syncthreads();
+        done = g.done;
+        if (!done) group_fetch<GNT, GCAP, ITEMS>(g, k, a, ws, rc, rv, rs, vmask);
         int cb;
         const int len = merge_levels<GNT, GCAP, GROUP_KMAX>(g.m, nl, &cb);
         // ---- stream out
@@ -386,7 +422,6 @@ __global__ __launch_bounds__(GNT) void k_group(const GroupTask* __restrict__ tas
             }
         }
         out_n += len;
-        if (t < k) g.cur[t] += g.take[t];
         __syncthreads();
     }
 

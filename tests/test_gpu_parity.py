@@ -244,3 +244,22 @@ def test_allocation_failure_reports_nomem(gpu):
     L.brm_debug_fail_alloc_after(-1)
     R = oracle.brmerge_precise(as_tuple(A), as_tuple(A))
     assert_csr_equal(bm.brmerge_precise(A, A), R)
+
+
+@pytest.mark.parametrize("tier", list(TIER_SETTINGS), indirect=True)
+def test_signed_zero_products(gpu, tier):
+    """Explicit +0.0 / -0.0 A values: every product must still be rounded
+    and summed (sign of zero included), in every tier."""
+    rng = np.random.default_rng(21)
+    K, N = 800, 2000
+    B = rand_csr_tuple(rng, K, N, 120, empty_frac=0.1)
+    acols = [np.sort(rng.choice(K, size=s, replace=False)) for s in (700, 90, 40, 3, 500)]
+    arpt = np.concatenate([[0], np.cumsum([len(x) for x in acols])]).astype(np.int32)
+    aval = rng.uniform(-1, 1, arpt[-1])
+    aval[::3] = 0.0
+    aval[1::7] = -0.0
+    A = (arpt, np.concatenate(acols).astype(np.int32), aval, K)
+    R = oracle.brmerge_precise(A, B)
+    Cp, Cu = run_both(gpu, A, B)
+    assert_csr_equal(Cp, R)
+    assert_csr_equal(Cu, R)
