@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="do not bracket launches with hipEvents")
     ap.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
                     help="library tuning knob (brm_set_param), repeatable")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run the library's plain copy kernel over GIB GiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); multi-GPU sharding is "
@@ -123,6 +125,14 @@ def main():
     import brmerge_amd as bm
     if bm.lib().brm_init(0) != 0:
         raise RuntimeError(bm._lib.last_error())
+    if args.copy_calibration > 0:
+        nbytes = int(args.copy_calibration * (1 << 30)) // 16 * 16
+        src = bm.DeviceArray.alloc(nbytes // 8, np.float64)
+        dst = bm.DeviceArray.alloc(nbytes // 8, np.float64)
+        for _ in range(2):
+            bm._lib.check(bm.lib().brm_bench_copy(dst.ptr, src.ptr, nbytes, None))
+        print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 2}), flush=True)
+        return
     params = {}
     for kv in args.param:
         k, v = kv.split("=", 1)

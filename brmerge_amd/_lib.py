@@ -53,7 +53,7 @@ EXPORTED = [
     "brm_dev_malloc", "brm_dev_free", "brm_dev_trim",
     "brm_flop_count", "brm_flop_count_dev",
     "brm_set_param", "brm_get_param", "brm_debug_fail_alloc_after",
-    "brm_memcpy_h2d", "brm_memcpy_d2h", "brm_prof_reset", "brm_prof_report", "brm_last_stats",
+    "brm_memcpy_h2d", "brm_memcpy_d2h", "brm_prof_reset", "brm_prof_report", "brm_last_stats", "brm_bench_copy",
     "brm_gen_er", "brm_gen_er_var", "brm_gen_rmat", "brm_gen_stencil27", "brm_gen_aggregation",
     "brm_gen_banded", "brm_csr_transpose", "brm_read_mtx", "brm_write_mtx",
 ]
@@ -101,6 +101,7 @@ def lib():
         "brm_prof_reset": ([], None),
         "brm_prof_report": ([ctypes.c_char_p, U64], I),
         "brm_last_stats": ([P], I),
+        "brm_bench_copy": ([P, P, U64, P], I),
         "brm_gen_er": ([I, I, I, U64, PP, PP, PP], I),
         "brm_gen_er_var": ([I, I, I, I, U64, PP, PP, PP], I),
         "brm_gen_rmat": ([I, I, ctypes.c_double, ctypes.c_double, ctypes.c_double, U64, PP, PP, PP], I),

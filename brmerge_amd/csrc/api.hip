@@ -887,6 +887,14 @@ int64_t brm_get_param(const char* name) {
 
 void brm_debug_fail_alloc_after(int64_t n) { dev_set_fail_after(n); }
 
+int brm_bench_copy(void* dst, const void* src, uint64_t bytes, void* stream) {
+    if (!dst || !src || (bytes & 15)) return BRM_E_INVALID;
+    BRMC(use_device());
+    HIPC(launch_copy(dst, src, bytes, (hipStream_t)stream));
+    HIPC(hipStreamSynchronize((hipStream_t)stream));
+    return BRM_OK;
+}
+
 int brm_last_stats(brm_stats* out) {
     if (!out) return BRM_E_INVALID;
     *out = t_stats;

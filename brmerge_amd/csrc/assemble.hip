@@ -33,6 +33,11 @@ __global__ __launch_bounds__(CP_NT) void k_compact(const int64_t* __restrict__ c
     }
 }
 
+// Plain streaming copy (16-byte vectors), used to calibrate bandwidth counters.
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 __global__ void k_rpt_to_i32(const int64_t* __restrict__ in, int64_t n, int* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) out[i] = (int)in[i];
@@ -58,6 +63,16 @@ hipError_t launch_compact(const int64_t* crpt, int M, const RowArgs& a, const in
     {
         ProfScope ps_("k_compact", s);
         k_compact<<<nblk(nnzC, (int)CP_TILE), CP_NT, 0, s>>>(crpt, M, a, cbar_col, cbar_val, ccol, cval, nnzC);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s) {
+    const int64_t n16 = (int64_t)(bytes / 16);
+    if (n16 <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_copy", s);
+        k_copy<<<2048, 256, 0, s>>>((const uint4*)src, (uint4*)dst, n16);
     }
     return hipGetLastError();
 }

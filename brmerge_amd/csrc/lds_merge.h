@@ -82,7 +82,10 @@ __device__ int merge_levels(MergeSmem<NT, CAP, LMAX>& sm, int n, int* cur_out) {
         const int* __restrict__ cc = sm.col[cur];
         const double* __restrict__ cv = sm.val[cur];
         const int Sl = lo_[n];
-        const int c = (Sl + NT - 1) / NT;
+        // Elements per thread, odd: threads start their merges c elements
+        // apart, and an odd stride spreads a wavefront's LDS accesses over
+        // all banks (an even one, e.g. 8, lands 16 lanes on one bank).
+        const int c = ((Sl + NT - 1) / NT) | 1;
 
         // split points of threads t (and NT)
         for (int tt = t; tt <= NT; tt += NT) {

@@ -181,6 +181,10 @@ typedef struct brm_stats {
 } brm_stats;
 int brm_last_stats(brm_stats *out);
 
+/* A plain streaming device copy of `bytes` (a multiple of 16) bytes, the
+ * reference kernel against which measured HBM byte counters are scaled. */
+int brm_bench_copy(void *dst, const void *src, uint64_t bytes, void *stream);
+
 /* Test hook: make the n-th next device allocation fail (n < 0 disables). */
 void brm_debug_fail_alloc_after(int64_t n);
 
