@@ -126,7 +126,9 @@ __device__ void emit_group(const LongArgs& la, int row, int a0, const int64_t* i
         if (src.kind != SRC_EMPTY) {
             Child c;
             c.src = src;
-            c.lid = q - s;
+            // every child starts on a multiple of 2^hprev: dropping those
+            // low bits keeps the pairing and skips levels with no pairs
+            c.lid = (q - s) >> hprev;
             c.pad = 0;
             la.children[cb + k] = c;
             k++;
