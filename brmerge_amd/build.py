@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbrmerge.so")
+DRIVER = os.path.join(HERE, "brm_driver")
 
 HIP_SOURCES = ["alloc.hip", "scan.hip", "flop.hip", "short.hip", "lds.hip", "long.hip", "assemble.hip", "api.hip"]
 CXX_SOURCES = ["gen.cpp"]
@@ -66,6 +67,11 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-pthread"])
     os.replace(tmp, LIB)
+    # command-line driver over the C++ compatibility header
+    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(os.path.dirname(HERE), "include"),
+          os.path.join(CSRC, "driver.cpp"), "-o", DRIVER + ".tmp", "-L", HERE, "-lbrmerge",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + "/opt/rocm/lib"])
+    os.replace(DRIVER + ".tmp", DRIVER)
     return LIB
 
 
