@@ -44,6 +44,13 @@ class StatsC(ctypes.Structure):
         "group_rounds", "group_in", "group_out")]
 
 
+class PairC(ctypes.Structure):
+    _fields_ = [("arpt", ctypes.c_void_p), ("acol", ctypes.c_void_p), ("aval", ctypes.c_void_p),
+                ("nnzA", ctypes.c_int64), ("brpt", ctypes.c_void_p), ("bcol", ctypes.c_void_p),
+                ("bval", ctypes.c_void_p), ("nnzB", ctypes.c_int64), ("M", ctypes.c_int), ("K", ctypes.c_int),
+                ("N", ctypes.c_int)]
+
+
 # Every symbol include/brmerge.h and include/brmerge_gen.h declare.
 EXPORTED = [
     "brm_init", "brm_last_error", "brm_version",
@@ -54,6 +61,8 @@ EXPORTED = [
     "brm_flop_count", "brm_flop_count_dev",
     "brm_set_param", "brm_get_param", "brm_debug_fail_alloc_after",
     "brm_memcpy_h2d", "brm_memcpy_d2h", "brm_prof_reset", "brm_prof_report", "brm_last_stats", "brm_bench_copy",
+    "brm_spgemm_batched_dev", "brm_batch_plan_create", "brm_batch_plan_execute", "brm_batch_plan_result",
+    "brm_batch_plan_info", "brm_batch_plan_destroy",
     "brm_gen_er", "brm_gen_er_var", "brm_gen_rmat", "brm_gen_stencil27", "brm_gen_aggregation",
     "brm_gen_banded", "brm_csr_transpose", "brm_read_mtx", "brm_write_mtx",
 ]
@@ -102,6 +111,12 @@ def lib():
         "brm_prof_report": ([ctypes.c_char_p, U64], I),
         "brm_last_stats": ([P], I),
         "brm_bench_copy": ([P, P, U64, P], I),
+        "brm_spgemm_batched_dev": ([P, I, PP, PP, PP, ctypes.POINTER(I64), P, P, P], I),
+        "brm_batch_plan_create": ([P, I, I, PP, P], I),
+        "brm_batch_plan_execute": ([P, P], I),
+        "brm_batch_plan_result": ([P, PP, PP, PP, ctypes.POINTER(I64), PP], I),
+        "brm_batch_plan_info": ([P, ctypes.c_char_p, U64], I),
+        "brm_batch_plan_destroy": ([P], None),
         "brm_gen_er": ([I, I, I, U64, PP, PP, PP], I),
         "brm_gen_er_var": ([I, I, I, I, U64, PP, PP, PP], I),
         "brm_gen_rmat": ([I, I, ctypes.c_double, ctypes.c_double, ctypes.c_double, U64, PP, PP, PP], I),

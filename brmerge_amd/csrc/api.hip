@@ -26,7 +26,7 @@
 #include <vector>
 
 #include "brm_device.h"
-#include "kernels.h"
+#include "pipeline.h"
 
 namespace brm {
 
@@ -36,10 +36,12 @@ Params& params() {
 }
 
 namespace {
-
 thread_local std::string t_err;
 thread_local brm_stats t_stats;
 thread_local int t_device = 0;
+}  // namespace
+
+brm_stats& thread_stats() { return t_stats; }
 
 void set_err(const char* fmt, ...) {
     char buf[512];
@@ -60,87 +62,54 @@ int hip_status(hipError_t e, const char* what) {
     return BRM_E_DEVICE;
 }
 
-#define HIPC(expr)                                   \
-    do {                                             \
-        hipError_t e_ = (expr);                      \
-        if (e_ != hipSuccess) return hip_status(e_, #expr); \
-    } while (0)
+int use_device() {
+    HIPC(hipSetDevice(t_device));
+    return BRM_OK;
+}
 
-#define BRMC(expr)                  \
-    do {                            \
-        int st_ = (expr);           \
-        if (st_ != BRM_OK) return st_; \
-    } while (0)
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
-// Device allocations owned by one call; freed on scope exit unless released.
-struct Scratch {
-    std::vector<void*> bufs;
-    ~Scratch() {
-        for (void* p : bufs) dev_free(p);
+PhaseClock::~PhaseClock() {
+    for (auto& m : marks) {
+        (void)hipEventDestroy(m.second.first);
+        (void)hipEventDestroy(m.second.second);
     }
-    template <typename T>
-    int alloc(T** p, int64_t n, const char* what) {
-        *p = (T*)dev_alloc(sizeof(T) * (size_t)(n > 0 ? n : 1));
-        if (!*p) {
-            set_err("device allocation of %s (%lld elements) failed", what, (long long)n);
-            return BRM_E_NOMEM;
-        }
-        bufs.push_back(*p);
-        return BRM_OK;
-    }
-    void release(void* p) {
-        auto it = std::find(bufs.begin(), bufs.end(), p);
-        if (it != bufs.end()) bufs.erase(it);
-    }
-    void free_now(void* p) {
-        release(p);
-        dev_free(p);
-    }
-};
+    if (open_ev) (void)hipEventDestroy(open_ev);
+}
 
-enum Phase { P_PRE, P_FLOP, P_PFLOP, P_LB, P_SYM, P_PNNZ, P_ALLOC, P_COMPUTE, P_COPY, P_CLEAN, P_N };
-
-// hipEvent pairs around phases, summed per phase.
-struct PhaseClock {
-    hipStream_t s;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
-    int open_phase = -1;
-    hipEvent_t open_ev = nullptr;
-    explicit PhaseClock(hipStream_t st) : s(st) {}
-    ~PhaseClock() {
-        for (auto& m : marks) {
-            (void)hipEventDestroy(m.second.first);
-            (void)hipEventDestroy(m.second.second);
-        }
-        if (open_ev) (void)hipEventDestroy(open_ev);
-    }
-    void begin(int p) {
-        end();
-        if (hipEventCreate(&open_ev) != hipSuccess) { open_ev = nullptr; return; }
-        (void)hipEventRecord(open_ev, s);
-        open_phase = p;
-    }
-    void end() {
-        if (open_phase < 0 || !open_ev) return;
-        hipEvent_t e1;
-        if (hipEventCreate(&e1) == hipSuccess) {
-            (void)hipEventRecord(e1, s);
-            marks.push_back({open_phase, {open_ev, e1}});
-        } else {
-            (void)hipEventDestroy(open_ev);
-        }
+void PhaseClock::begin(int p) {
+    end();
+    if (hipEventCreate(&open_ev) != hipSuccess) {
         open_ev = nullptr;
-        open_phase = -1;
+        return;
     }
-    void collect(double* out) {
-        end();
-        for (int i = 0; i < P_N; ++i) out[i] = 0.0;
-        for (auto& m : marks) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, m.second.first, m.second.second) == hipSuccess) out[m.first] += ms * 1e-3;
-        }
+    (void)hipEventRecord(open_ev, s);
+    open_phase = p;
+}
+
+void PhaseClock::end() {
+    if (open_phase < 0 || !open_ev) return;
+    hipEvent_t e1;
+    if (hipEventCreate(&e1) == hipSuccess) {
+        (void)hipEventRecord(e1, s);
+        marks.push_back({open_phase, {open_ev, e1}});
+    } else {
+        (void)hipEventDestroy(open_ev);
     }
-};
+    open_ev = nullptr;
+    open_phase = -1;
+}
+
+void PhaseClock::collect(double* out) {
+    end();
+    for (int i = 0; i < P_N; ++i) out[i] = 0.0;
+    for (auto& m : marks) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, m.second.first, m.second.second) == hipSuccess) out[m.first] += ms * 1e-3;
+    }
+}
 
 int d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
     HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
@@ -161,6 +130,8 @@ int check_dev_err(const int* derr, hipStream_t s, const char* stage) {
     }
     return BRM_OK;
 }
+
+namespace {
 
 // Host-side trace of the orchestration (BRM_TRACE=1 in the environment):
 // elapsed milliseconds since the start of the call, after synchronising the
@@ -369,27 +340,10 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     return BRM_OK;
 }
 
+}  // namespace
+
 // ---------------------------------------------------------------- pipeline
-struct DevIn {
-    const int* arpt;
-    const int* acol;
-    const double* aval;
-    int64_t nnzA;
-    const int* brpt;
-    const int* bcol;
-    const double* bval;
-    int64_t nnzB;
-    int M, K, N;
-};
-
-struct DevOut {
-    int64_t* crpt;
-    int* ccol;
-    double* cval;
-    int64_t nnz;
-};
-
-int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph) {
+int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p) {
     if (in.M < 0 || in.K < 0 || in.N < 0 || in.nnzA < 0 || in.nnzB < 0) {
         set_err("negative dimension");
         return BRM_E_INVALID;
@@ -398,117 +352,145 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         set_err("nnz(A) and nnz(B) must fit int32 row pointers");
         return BRM_E_INVALID;
     }
-    PhaseClock clk(s);
-    Scratch sc;
     const int M = in.M;
-    memset(&t_stats, 0, sizeof(t_stats));
-    HostTrace tr(s);
-    tr.mark("spgemm start", M);
-
-    clk.begin(P_PRE);
-    int* derr;
-    int* bin_cnt;
-    int64_t *ipA, *iprow, *rowcnt, *crpt;
-    int* bins;
-    BRMC(sc.alloc(&derr, 1, "error flag"));
-    BRMC(sc.alloc(&bin_cnt, BIN_COUNT, "bin counters"));
-    BRMC(sc.alloc(&ipA, in.nnzA + 1, "ipA"));
-    BRMC(sc.alloc(&iprow, M, "row ip"));
-    BRMC(sc.alloc(&rowcnt, M, "row nnz"));
-    BRMC(sc.alloc(&crpt, (int64_t)M + 1, "crpt"));
-    BRMC(sc.alloc(&bins, 3 * (int64_t)M, "row bins"));
-    HIPC(launch_fill_i32(derr, 1, 0, s));
-    HIPC(launch_fill_i32(bin_cnt, BIN_COUNT, 0, s));
-
-    RowArgs a;
+    if (clk) clk->begin(P_PRE);
+    Scratch& sc = p.sc;
+    BRMC(sc.alloc(&p.derr, 1, "error flag"));
+    BRMC(sc.alloc(&p.bin_cnt, BIN_COUNT, "bin counters"));
+    BRMC(sc.alloc(&p.ipA, in.nnzA + 1, "ipA"));
+    BRMC(sc.alloc(&p.iprow, M, "row ip"));
+    BRMC(sc.alloc(&p.rowcnt, M, "row nnz"));
+    BRMC(sc.alloc(&p.crpt, (int64_t)M + 1, "crpt"));
+    BRMC(sc.alloc(&p.bins, 3 * (int64_t)M, "row bins"));
+    HIPC(launch_fill_i32(p.derr, 1, 0, s));
+    p.nnzA = in.nnzA;
+    p.nnzB = in.nnzB;
+    RowArgs& a = p.a;
     a.arpt = in.arpt;
     a.acol = in.acol;
     a.aval = in.aval;
     a.brpt = in.brpt;
     a.bcol = in.bcol;
     a.bval = in.bval;
-    a.ipA = ipA;
+    a.ipA = p.ipA;
     a.M = in.M;
     a.K = in.K;
     a.N = in.N;
-    HIPC(launch_validate(a, in.nnzA, in.nnzB, derr, s));
-    BRMC(check_dev_err(derr, s, "validation"));
-    HIPC(launch_validate_sorted(in.brpt, in.bcol, in.K, derr, s));
-    BRMC(check_dev_err(derr, s, "validation"));
+    HIPC(launch_validate(a, in.nnzA, in.nnzB, p.derr, s));
+    BRMC(check_dev_err(p.derr, s, "validation"));
+    HIPC(launch_validate_sorted(in.brpt, in.bcol, in.K, p.derr, s));
+    BRMC(check_dev_err(p.derr, s, "validation"));
 
-    clk.begin(P_FLOP);
-    HIPC(scan_alen(in.acol, in.brpt, in.nnzA, ipA, s));
+    if (clk) clk->begin(P_FLOP);
+    HIPC(scan_alen(in.acol, in.brpt, in.nnzA, p.ipA, s));
+    BRMC(d2h(&p.total_ip, p.ipA + in.nnzA, sizeof(int64_t), s));
 
-    clk.begin(P_LB);
+    if (clk) clk->begin(P_LB);
     const LdsShape shape = lds_shape(params().lds_cfg);
-    HIPC(launch_rowstats(a, std::min(params().med_cap, shape.cap), shape.lmax, params().force_long, iprow, rowcnt,
-                         bins, bin_cnt, s));
-    int bc[BIN_COUNT];
-    int64_t total_ip = 0;
-    BRMC(d2h(bc, bin_cnt, sizeof(bc), s));
-    BRMC(d2h(&total_ip, ipA + in.nnzA, sizeof(int64_t), s));
-    for (int b = 0; b < BIN_COUNT; ++b) t_stats.rows_by_tier[b] = bc[b];
-    t_stats.total_ip = total_ip;
-    const int* rows_short = bins;
-    const int* rows_med = bins + M;
-    const int* rows_long = bins + 2 * (int64_t)M;
+    return pattern_bin(p, params().lds_cfg, std::min(params().med_cap, shape.cap), params().force_long, s);
+}
 
+int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int force_long, hipStream_t s) {
+    const LdsShape shape = lds_shape(lds_cfg);
+    p.lds_cfg = lds_cfg;
+    p.med_cap = std::min(med_cap, shape.cap);
+    HIPC(launch_fill_i32(p.bin_cnt, BIN_COUNT, 0, s));
+    HIPC(launch_rowstats(p.a, p.med_cap, shape.lmax, force_long, p.iprow, p.rowcnt, p.bins, p.bin_cnt, s));
+    BRMC(d2h(p.bc, p.bin_cnt, sizeof(p.bc), s));
+    return BRM_OK;
+}
+
+int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk) {
+    if (clk) clk->begin(P_SYM);
+    HIPC(launch_short_count(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, p.rowcnt, s));
+    HIPC(launch_medium_count(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, p.rowcnt, s));
+    HIPC(launch_long_count(p.rows(BIN_LONG), p.bc[BIN_LONG], p.a, p.rowcnt, s));
+    if (clk) clk->begin(P_PNNZ);
+    HIPC(scan_i64(p.rowcnt, p.a.M, p.crpt, s));
+    BRMC(d2h(&p.nnzC, p.crpt + p.a.M, sizeof(int64_t), s));
+    return BRM_OK;
+}
+
+namespace {
+OutArgs precise_out(const Pattern& p, int* ccol, double* cval) {
     OutArgs o{};
-    o.err = derr;
+    o.err = p.derr;
+    o.upper = 0;
+    o.crpt = p.crpt;
+    o.ccol = ccol;
+    o.cval = cval;
+    return o;
+}
+}  // namespace
+
+int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t s) {
+    const OutArgs o = precise_out(p, ccol, cval);
+    HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
+    HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
+    return BRM_OK;
+}
+
+int pattern_numeric_long(const Pattern& p, int* ccol, double* cval, hipStream_t s) {
+    if (p.bc[BIN_LONG] == 0) return BRM_OK;
+    const OutArgs o = precise_out(p, ccol, cval);
+    Scratch sc;
+    return run_long(p.a, p.rows(BIN_LONG), p.bc[BIN_LONG], o, p.crpt, s, sc);
+}
+
+int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph) {
+    PhaseClock clk(s);
+    memset(&t_stats, 0, sizeof(t_stats));
+    HostTrace tr(s);
+    tr.mark("spgemm start", in.M);
+    Pattern p;
+    BRMC(pattern_prepare(in, s, &clk, p));
+    const int M = in.M;
+    for (int b = 0; b < BIN_COUNT; ++b) t_stats.rows_by_tier[b] = p.bc[b];
+    t_stats.total_ip = p.total_ip;
+    Scratch& sc = p.sc;
     int* ccol = nullptr;
     double* cval = nullptr;
     int64_t nnzC = 0;
 
     if (!upper) {
-        clk.begin(P_SYM);
-        HIPC(launch_short_count(rows_short, bc[BIN_SHORT], a, rowcnt, s));
-        HIPC(launch_medium_count(rows_med, bc[BIN_MEDIUM], a, rowcnt, s));
-        HIPC(launch_long_count(rows_long, bc[BIN_LONG], a, rowcnt, s));
-
-        clk.begin(P_PNNZ);
-        HIPC(scan_i64(rowcnt, M, crpt, s));
-        BRMC(d2h(&nnzC, crpt + M, sizeof(int64_t), s));
+        BRMC(pattern_symbolic(p, s, &clk));
+        nnzC = p.nnzC;
         if (nnzC > max_nnz) {
             set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
             return BRM_E_OVERFLOW;
         }
-
         clk.begin(P_ALLOC);
         BRMC(sc.alloc(&ccol, nnzC, "C col"));
         BRMC(sc.alloc(&cval, nnzC, "C val"));
-
         tr.mark("C allocated", nnzC);
         clk.begin(P_COMPUTE);
-        o.upper = 0;
-        o.crpt = crpt;
-        o.ccol = ccol;
-        o.cval = cval;
-        HIPC(launch_short_numeric(rows_short, bc[BIN_SHORT], a, o, s));
-        HIPC(launch_medium_numeric(rows_med, bc[BIN_MEDIUM], a, o, s));
+        BRMC(pattern_numeric_tiers(p, ccol, cval, s));
         tr.mark("short+medium numeric");
-        BRMC(run_long(a, rows_long, bc[BIN_LONG], o, crpt, s, sc));
+        BRMC(pattern_numeric_long(p, ccol, cval, s));
         tr.mark("long numeric");
-        BRMC(check_dev_err(derr, s, "numeric"));
+        BRMC(check_dev_err(p.derr, s, "numeric"));
     } else {
         clk.begin(P_ALLOC);
         int* bcol_bar;
         double* bval_bar;
-        BRMC(sc.alloc(&bcol_bar, total_ip, "C-bar col"));
-        BRMC(sc.alloc(&bval_bar, total_ip, "C-bar val"));
+        BRMC(sc.alloc(&bcol_bar, p.total_ip, "C-bar col"));
+        BRMC(sc.alloc(&bval_bar, p.total_ip, "C-bar val"));
 
         clk.begin(P_COMPUTE);
+        OutArgs o{};
+        o.err = p.derr;
         o.upper = 1;
         o.ccol = bcol_bar;
         o.cval = bval_bar;
-        o.rowlen = rowcnt;
-        HIPC(launch_short_numeric(rows_short, bc[BIN_SHORT], a, o, s));
-        HIPC(launch_medium_numeric(rows_med, bc[BIN_MEDIUM], a, o, s));
-        BRMC(run_long(a, rows_long, bc[BIN_LONG], o, nullptr, s, sc));
-        BRMC(check_dev_err(derr, s, "numeric"));
+        o.rowlen = p.rowcnt;
+        HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
+        HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
+        BRMC(run_long(p.a, p.rows(BIN_LONG), p.bc[BIN_LONG], o, nullptr, s, sc));
+        BRMC(check_dev_err(p.derr, s, "numeric"));
 
         clk.begin(P_PNNZ);
-        HIPC(scan_i64(rowcnt, M, crpt, s));
-        BRMC(d2h(&nnzC, crpt + M, sizeof(int64_t), s));
+        HIPC(scan_i64(p.rowcnt, M, p.crpt, s));
+        BRMC(d2h(&nnzC, p.crpt + M, sizeof(int64_t), s));
         if (nnzC > max_nnz) {
             set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
             return BRM_E_OVERFLOW;
@@ -517,7 +499,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         BRMC(sc.alloc(&ccol, nnzC, "C col"));
         BRMC(sc.alloc(&cval, nnzC, "C val"));
         clk.begin(P_COPY);
-        HIPC(launch_compact(crpt, M, a, bcol_bar, bval_bar, ccol, cval, nnzC, s));
+        HIPC(launch_compact(p.crpt, M, p.a, bcol_bar, bval_bar, ccol, cval, nnzC, s));
     }
     clk.begin(P_CLEAN);
     HIPC(hipStreamSynchronize(s));
@@ -526,10 +508,10 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     if (ph) clk.collect(ph);
 
     t_stats.nnz_c = nnzC;
-    sc.release(crpt);
+    sc.release(p.crpt);
     sc.release(ccol);
     sc.release(cval);
-    out->crpt = crpt;
+    out->crpt = p.crpt;
     out->ccol = ccol;
     out->cval = cval;
     out->nnz = nnzC;
@@ -537,9 +519,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     return BRM_OK;
 }
 
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
+namespace {
 
 void fill_precise(brm_precise_timing* t, const double* ph, double total, double h2d, double d2h_) {
     if (!t) return;
@@ -573,11 +553,6 @@ void fill_upper(brm_upper_timing* t, const double* ph, double total, double h2d,
     t->total = total;
     t->h2d = h2d;
     t->d2h = d2h_;
-}
-
-int use_device() {
-    HIPC(hipSetDevice(t_device));
-    return BRM_OK;
 }
 
 // Host-pointer entry shared by the four host variants.

@@ -1,0 +1,526 @@
+// Batched SpGEMM: many independent products C_i = A_i * B_i run as one
+// block-diagonal product
+//     diag(A_1 .. A_n) * diag(B_1 .. B_n) = diag(C_1 .. C_n).
+// A_i's columns are shifted by the first fused row of B_i so that each A row
+// selects rows of its own B; B_i's columns are kept, so every fused C row
+// carries exactly the columns (and, row by row, the same merge tree and the
+// same values) of the corresponding row of C_i.  One set of launches covers
+// the whole batch instead of one set per product.
+//
+// A plan (brm_batch_plan_*) keeps the fused structure, the symbolic result
+// and C resident, picks the tier shapes by timing candidates on the batch
+// itself, and captures the numeric phase -- value refresh from the caller's
+// arrays plus the short and medium tiers -- in one HIP graph that each
+// execution replays.
+#include <limits.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "brm_device.h"
+#include "pipeline.h"
+
+namespace brm {
+namespace {
+
+constexpr int CAT_NT = 256;
+constexpr int CAT_ITEMS = 4;
+
+// Device view of one product of the batch.
+struct PairDev {
+    const int* arpt;
+    const int* acol;
+    const double* aval;
+    const int* brpt;
+    const int* bcol;
+    const double* bval;
+    int64_t nnzA, nnzB;
+    int M, K, N;
+    int pad;
+};
+
+// Prefix tables of the fused layout, each [n+1].
+struct Fused {
+    const PairDev* pairs;
+    int n;
+    const int64_t* mbase;  // fused A rows
+    const int64_t* kbase;  // fused B rows
+    const int64_t* abase;  // fused A entries
+    const int64_t* bbase;  // fused B entries
+    int* arpt;
+    int* acol;
+    double* aval;
+    int* brpt;
+    int* bcol;
+    double* bval;
+    int64_t M, K, nnzA, nnzB;
+};
+
+// Largest i in [lo, hi] with base[i] <= x (base non-decreasing).
+__device__ __forceinline__ int seg_search(const int64_t* __restrict__ base, int lo, int hi, int64_t x) {
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Segment of x for a wavefront whose lanes hold x0 .. x0+63: one search for
+// each end of the wavefront's range, then a short walk per lane.
+__device__ __forceinline__ int wave_seg(const int64_t* __restrict__ base, int n, int64_t x0, int64_t x) {
+    const int plo = seg_search(base, 0, n - 1, x0);
+    const int phi = seg_search(base, plo, n - 1, x0 + BRM_WAVE - 1);
+    int p = plo;
+    while (p < phi && base[p + 1] <= x) ++p;
+    return p;
+}
+
+// Fused row pointers of A and B (per pair: shifted by the pair's first
+// fused entry) and the per-pair checks that the fused arrays cannot show:
+// rpt[0] == 0 and rpt[M] == nnz of each operand.
+__global__ __launch_bounds__(CAT_NT) void k_cat_rows(Fused f, int* err) {
+    const int64_t chunk = ((int64_t)blockIdx.x * CAT_NT + (threadIdx.x & ~(BRM_WAVE - 1))) * CAT_ITEMS;
+    const int lane = lane_id();
+    const int64_t total = f.M + f.K;
+    for (int it = 0; it < CAT_ITEMS; ++it) {
+        const int64_t x0 = chunk + (int64_t)it * BRM_WAVE;
+        if (x0 >= total) break;
+        const int64_t x = x0 + lane;
+        if (x < f.M) {
+            const int p = x0 + BRM_WAVE <= f.M ? wave_seg(f.mbase, f.n, x0, x) : seg_search(f.mbase, 0, f.n - 1, x);
+            const PairDev& P = f.pairs[p];
+            const int64_t r = x - f.mbase[p];
+            const int v = P.arpt[r];
+            f.arpt[x] = (int)(v + f.abase[p]);
+            if (r == 0 && v != 0) atomicOr(err, (int)DERR_INVALID);
+            if (r == P.M - 1 && (int64_t)P.arpt[P.M] != P.nnzA) atomicOr(err, (int)DERR_INVALID);
+            if (v < 0 || (int64_t)v > P.nnzA) atomicOr(err, (int)DERR_INVALID);
+        } else if (x < total) {
+            const int64_t y = x - f.M;
+            const int64_t y0 = x0 - f.M;
+            const int p = (y0 >= 0 && x0 + BRM_WAVE <= total) ? wave_seg(f.kbase, f.n, y0, y)
+                                                              : seg_search(f.kbase, 0, f.n - 1, y);
+            const PairDev& P = f.pairs[p];
+            const int64_t r = y - f.kbase[p];
+            const int v = P.brpt[r];
+            f.brpt[y] = (int)(v + f.bbase[p]);
+            if (r == 0 && v != 0) atomicOr(err, (int)DERR_INVALID);
+            if (r == P.K - 1 && (int64_t)P.brpt[P.K] != P.nnzB) atomicOr(err, (int)DERR_INVALID);
+            if (v < 0 || (int64_t)v > P.nnzB) atomicOr(err, (int)DERR_INVALID);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        f.arpt[f.M] = (int)f.nnzA;
+        f.brpt[f.K] = (int)f.nnzB;
+    }
+}
+
+// Fused entries: A columns shifted to the pair's B rows (checked against
+// the pair's K), B columns kept (checked against the pair's N); values
+// copied when `vals`, indices written when `cols`.
+__global__ __launch_bounds__(CAT_NT) void k_cat_entries(Fused f, int cols, int vals, int* err) {
+    const int64_t chunk = ((int64_t)blockIdx.x * CAT_NT + (threadIdx.x & ~(BRM_WAVE - 1))) * CAT_ITEMS;
+    const int lane = lane_id();
+    const int64_t total = f.nnzA + f.nnzB;
+    for (int it = 0; it < CAT_ITEMS; ++it) {
+        const int64_t x0 = chunk + (int64_t)it * BRM_WAVE;
+        if (x0 >= total) break;
+        const int64_t x = x0 + lane;
+        if (x < f.nnzA) {
+            const int p = x0 + BRM_WAVE <= f.nnzA ? wave_seg(f.abase, f.n, x0, x) : seg_search(f.abase, 0, f.n - 1, x);
+            const PairDev& P = f.pairs[p];
+            const int64_t j = x - f.abase[p];
+            if (cols) {
+                const int c = P.acol[j];
+                if (c < 0 || c >= P.K) atomicOr(err, (int)DERR_INVALID);
+                f.acol[x] = (int)(min(max(c, 0), max(P.K - 1, 0)) + f.kbase[p]);
+            }
+            if (vals) f.aval[x] = P.aval[j];
+        } else if (x < total) {
+            const int64_t y = x - f.nnzA;
+            const int64_t y0 = x0 - f.nnzA;
+            const int p = (y0 >= 0 && x0 + BRM_WAVE <= total) ? wave_seg(f.bbase, f.n, y0, y)
+                                                              : seg_search(f.bbase, 0, f.n - 1, y);
+            const PairDev& P = f.pairs[p];
+            const int64_t j = y - f.bbase[p];
+            if (cols) {
+                const int c = P.bcol[j];
+                if (c < 0 || c >= P.N) atomicOr(err, (int)DERR_INVALID);
+                f.bcol[y] = c;
+            }
+            if (vals) f.bval[y] = P.bval[j];
+        }
+    }
+}
+
+unsigned cat_blocks(int64_t n) {
+    const int64_t per = (int64_t)CAT_NT * CAT_ITEMS;
+    return (unsigned)((n + per - 1) / per);
+}
+
+// ---------------------------------------------------------------- host side
+// The fused operands of one batch (device), built from the caller's pairs.
+struct FusedBatch {
+    Scratch sc;
+    Fused f{};
+    int Nmax = 0;
+    std::vector<int64_t> mbase, kbase, abase, bbase;
+};
+
+int check_pairs(const brm_pair* pairs, int n) {
+    if (n < 0 || (n > 0 && !pairs)) {
+        set_err("invalid pair array");
+        return BRM_E_INVALID;
+    }
+    for (int i = 0; i < n; ++i) {
+        const brm_pair& q = pairs[i];
+        if (q.M < 0 || q.K < 0 || q.N < 0 || q.nnzA < 0 || q.nnzB < 0) {
+            set_err("pair %d: negative dimension", i);
+            return BRM_E_INVALID;
+        }
+        if ((q.M > 0 && !q.arpt) || (q.K > 0 && !q.brpt) || (q.nnzA > 0 && (!q.acol || !q.aval)) ||
+            (q.nnzB > 0 && (!q.bcol || !q.bval))) {
+            set_err("pair %d: null pointer", i);
+            return BRM_E_INVALID;
+        }
+        if ((q.M == 0 && q.nnzA != 0) || (q.K == 0 && q.nnzB != 0)) {
+            set_err("pair %d: entries without rows", i);
+            return BRM_E_INVALID;
+        }
+    }
+    return BRM_OK;
+}
+
+int build_fused(const brm_pair* pairs, int n, hipStream_t s, FusedBatch& fb, int* derr) {
+    BRMC(check_pairs(pairs, n));
+    fb.mbase.assign(n + 1, 0);
+    fb.kbase.assign(n + 1, 0);
+    fb.abase.assign(n + 1, 0);
+    fb.bbase.assign(n + 1, 0);
+    std::vector<PairDev> pd(n > 0 ? n : 1);
+    for (int i = 0; i < n; ++i) {
+        const brm_pair& q = pairs[i];
+        fb.mbase[i + 1] = fb.mbase[i] + q.M;
+        fb.kbase[i + 1] = fb.kbase[i] + q.K;
+        fb.abase[i + 1] = fb.abase[i] + q.nnzA;
+        fb.bbase[i + 1] = fb.bbase[i] + q.nnzB;
+        fb.Nmax = std::max(fb.Nmax, q.N);
+        pd[i] = PairDev{q.arpt, q.acol, q.aval, q.brpt, q.bcol, q.bval, q.nnzA, q.nnzB, q.M, q.K, q.N, 0};
+    }
+    Fused& f = fb.f;
+    f.n = n;
+    f.M = fb.mbase[n];
+    f.K = fb.kbase[n];
+    f.nnzA = fb.abase[n];
+    f.nnzB = fb.bbase[n];
+    if (f.M > INT32_MAX - 1 || f.K > INT32_MAX - 1 || f.nnzA > INT32_MAX || f.nnzB > INT32_MAX) {
+        set_err("fused batch exceeds int32 rows or entries (%lld rows, %lld + %lld entries)", (long long)f.M,
+                (long long)f.nnzA, (long long)f.nnzB);
+        return BRM_E_INVALID;
+    }
+    PairDev* pairs_d;
+    int64_t* tab_d;
+    Scratch& sc = fb.sc;
+    BRMC(sc.alloc(&pairs_d, n, "batch pairs"));
+    BRMC(sc.alloc(&tab_d, 4 * ((int64_t)n + 1), "batch tables"));
+    BRMC(sc.alloc(&f.arpt, f.M + 1, "fused A rpt"));
+    BRMC(sc.alloc(&f.acol, f.nnzA, "fused A col"));
+    BRMC(sc.alloc(&f.aval, f.nnzA, "fused A val"));
+    BRMC(sc.alloc(&f.brpt, f.K + 1, "fused B rpt"));
+    BRMC(sc.alloc(&f.bcol, f.nnzB, "fused B col"));
+    BRMC(sc.alloc(&f.bval, f.nnzB, "fused B val"));
+    std::vector<int64_t> tab;
+    tab.reserve(4 * ((size_t)n + 1));
+    for (auto* v : {&fb.mbase, &fb.kbase, &fb.abase, &fb.bbase}) tab.insert(tab.end(), v->begin(), v->end());
+    if (n > 0) HIPC(hipMemcpyAsync(pairs_d, pd.data(), sizeof(PairDev) * n, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(tab_d, tab.data(), sizeof(int64_t) * tab.size(), hipMemcpyHostToDevice, s));
+    f.pairs = pairs_d;
+    f.mbase = tab_d;
+    f.kbase = tab_d + (n + 1);
+    f.abase = tab_d + 2 * ((int64_t)n + 1);
+    f.bbase = tab_d + 3 * ((int64_t)n + 1);
+    HIPC(launch_fill_i32(derr, 1, 0, s));
+    if (n > 0) {
+        {
+            ProfScope ps_("k_cat_rows", s);
+            k_cat_rows<<<cat_blocks(f.M + f.K), CAT_NT, 0, s>>>(f, derr);
+        }
+        HIPC(hipGetLastError());
+        if (f.nnzA + f.nnzB > 0) {
+            ProfScope ps_("k_cat_entries", s);
+            k_cat_entries<<<cat_blocks(f.nnzA + f.nnzB), CAT_NT, 0, s>>>(f, 1, 1, derr);
+        }
+        HIPC(hipGetLastError());
+    } else {
+        HIPC(launch_fill_i32(f.arpt, 1, 0, s));
+        HIPC(launch_fill_i32(f.brpt, 1, 0, s));
+    }
+    return check_dev_err(derr, s, "batch assembly");
+}
+
+DevIn fused_in(const FusedBatch& fb) {
+    const Fused& f = fb.f;
+    return DevIn{f.arpt, f.acol, f.aval, f.nnzA, f.brpt, f.bcol, f.bval, f.nnzB, (int)f.M, (int)f.K, fb.Nmax};
+}
+
+hipError_t refresh_values(const Fused& f, int* derr, hipStream_t s) {
+    if (f.n == 0 || f.nnzA + f.nnzB == 0) return hipSuccess;
+    {
+        ProfScope ps_("k_cat_entries", s);
+        k_cat_entries<<<cat_blocks(f.nnzA + f.nnzB), CAT_NT, 0, s>>>(f, 0, 1, derr);
+    }
+    return hipGetLastError();
+}
+
+// Tier-shape candidates tried by the plan's tuner: (LDS shape, medium capacity).
+struct TuneCand {
+    int lds_cfg;
+    int med_cap;
+};
+const TuneCand kCands[] = {{0, 4096}, {1, 2048}, {2, 2048}, {2, 1024}, {2, 512}};
+
+}  // namespace
+}  // namespace brm
+
+using namespace brm;
+
+struct brm_batch_plan {
+    FusedBatch fb;
+    Pattern pat;
+    int* ccol = nullptr;
+    double* cval = nullptr;
+    std::vector<int64_t> row_off;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::pair<int, double>> tuned;  // candidate index, best ms
+    int chosen = -1;
+    double analyse_s = 0;
+    ~brm_batch_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
+
+namespace {
+
+int batched_run(const brm_pair* pairs, int n, hipStream_t s, DevOut* out, int64_t* row_off, double* ph) {
+    BRMC(check_pairs(pairs, n));
+    BRMC(use_device());
+    FusedBatch fb;
+    int* derr;
+    BRMC(fb.sc.alloc(&derr, 1, "error flag"));
+    BRMC(build_fused(pairs, n, s, fb, derr));
+    BRMC(run_spgemm(fused_in(fb), false, INT64_MAX, s, out, ph));
+    if (row_off)
+        for (int i = 0; i <= n; ++i) row_off[i] = fb.mbase[i];
+    return BRM_OK;
+}
+
+double time_numeric(brm_batch_plan* p, hipStream_t s, int reps) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return -1;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return -1;
+    }
+    double best = -1;
+    for (int r = 0; r < reps; ++r) {
+        (void)hipEventRecord(a, s);
+        if (pattern_numeric_tiers(p->pat, p->ccol, p->cval, s) != BRM_OK) break;
+        if (pattern_numeric_long(p->pat, p->ccol, p->cval, s) != BRM_OK) break;
+        (void)hipEventRecord(b, s);
+        if (hipEventSynchronize(b) != hipSuccess) break;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (best < 0 || ms < best) best = ms;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return best;
+}
+
+int plan_build(brm_batch_plan* p, const brm_pair* pairs, int n, int autotune, hipStream_t s) {
+    const double t0 = now_s();
+    BRMC(check_pairs(pairs, n));
+    BRMC(use_device());
+    int* derr;
+    BRMC(p->fb.sc.alloc(&derr, 1, "error flag"));
+    BRMC(build_fused(pairs, n, s, p->fb, derr));
+    p->row_off = p->fb.mbase;
+    Pattern& pat = p->pat;
+    BRMC(pattern_prepare(fused_in(p->fb), s, nullptr, pat));
+    BRMC(pattern_symbolic(pat, s, nullptr));
+    BRMC(pat.sc.alloc(&p->ccol, pat.nnzC, "C col"));
+    BRMC(pat.sc.alloc(&p->cval, pat.nnzC, "C val"));
+    if (!autotune) {
+        // one eager pass: validates the numeric phase outside the graph and
+        // initialises every kernel's attributes before capture
+        if (time_numeric(p, s, 1) < 0) {
+            set_err("batch plan: numeric pass failed");
+            return BRM_E_DEVICE;
+        }
+        BRMC(check_dev_err(pat.derr, s, "batch plan"));
+    } else {
+        double best = -1;
+        const int nc = (int)(sizeof(kCands) / sizeof(kCands[0]));
+        for (int c = 0; c < nc; ++c) {
+            BRMC(pattern_bin(pat, kCands[c].lds_cfg, kCands[c].med_cap, 0, s));
+            const double ms = time_numeric(p, s, 2);
+            BRMC(check_dev_err(pat.derr, s, "batch tuning"));
+            if (ms < 0) {
+                set_err("batch tuning: timing failed");
+                return BRM_E_DEVICE;
+            }
+            p->tuned.push_back({c, ms});
+            if (best < 0 || ms < best) {
+                best = ms;
+                p->chosen = c;
+            }
+        }
+        BRMC(pattern_bin(pat, kCands[p->chosen].lds_cfg, kCands[p->chosen].med_cap, 0, s));
+    }
+    // The graph: value refresh from the caller's arrays, then the short and
+    // medium tiers.  Kernels only (no memsets, no host round trips).
+    HIPC(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+    Params& prm = params();
+    const int prof = prm.profile;
+    prm.profile = 0;  // no event records inside the captured sequence
+    hipError_t e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal);
+    int st = hip_status(e, "hipStreamBeginCapture");
+    if (st == BRM_OK) {
+        hipError_t e1 = refresh_values(p->fb.f, pat.derr, p->cap);
+        int st1 = e1 == hipSuccess ? pattern_numeric_tiers(pat, p->ccol, p->cval, p->cap) : hip_status(e1, "refresh");
+        hipError_t e2 = hipStreamEndCapture(p->cap, &p->graph);
+        st = st1 != BRM_OK ? st1 : hip_status(e2, "hipStreamEndCapture");
+    }
+    prm.profile = prof;
+    BRMC(st);
+    HIPC(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    HIPC(hipStreamSynchronize(s));
+    p->analyse_s = now_s() - t0;
+    return BRM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int brm_spgemm_batched_dev(const brm_pair* pairs, int npairs, int64_t** crpt, int** ccol, double** cval,
+                           int64_t* cnnz, int64_t* row_off, brm_precise_timing* timing, void* stream) {
+    if (!crpt || !ccol || !cval || !cnnz) {
+        set_err("null output pointer");
+        return BRM_E_INVALID;
+    }
+    const double t0 = now_s();
+    double ph[P_N] = {0};
+    DevOut out{};
+    int st = batched_run(pairs, npairs, (hipStream_t)stream, &out, row_off, ph);
+    if (st == BRM_OK) {
+        *crpt = out.crpt;
+        *ccol = out.ccol;
+        *cval = out.cval;
+        *cnnz = out.nnz;
+        if (timing) {
+            memset(timing, 0, sizeof(*timing));
+            timing->pre_allocate = ph[P_PRE];
+            timing->compute_flop = ph[P_FLOP];
+            timing->load_balance = ph[P_LB];
+            timing->symbolic = ph[P_SYM];
+            timing->prefix_sum_nnz = ph[P_PNNZ];
+            timing->allocate_C = ph[P_ALLOC];
+            timing->compute = ph[P_COMPUTE];
+            timing->cleanup = ph[P_CLEAN];
+            timing->total = now_s() - t0;
+        }
+    }
+    dev_flush_deferred();
+    return st;
+}
+
+int brm_batch_plan_create(const brm_pair* pairs, int npairs, int autotune, brm_batch_plan** plan, void* stream) {
+    if (!plan) {
+        set_err("null plan pointer");
+        return BRM_E_INVALID;
+    }
+    *plan = nullptr;
+    brm_batch_plan* p = new (std::nothrow) brm_batch_plan();
+    if (!p) {
+        set_err("host allocation failed");
+        return BRM_E_NOMEM;
+    }
+    const int st = plan_build(p, pairs, npairs, autotune, (hipStream_t)stream);
+    if (st != BRM_OK) {
+        (void)hipStreamSynchronize((hipStream_t)stream);
+        delete p;
+        dev_flush_deferred();
+        return st;
+    }
+    *plan = p;
+    dev_flush_deferred();
+    return BRM_OK;
+}
+
+int brm_batch_plan_execute(brm_batch_plan* p, void* stream) {
+    if (!p) {
+        set_err("null plan");
+        return BRM_E_INVALID;
+    }
+    BRMC(use_device());
+    hipStream_t s = (hipStream_t)stream;
+    HIPC(hipGraphLaunch(p->exec, s));
+    BRMC(pattern_numeric_long(p->pat, p->ccol, p->cval, s));
+    HIPC(hipStreamSynchronize(s));
+    const int st = check_dev_err(p->pat.derr, s, "batch execute");
+    dev_flush_deferred();
+    return st;
+}
+
+int brm_batch_plan_result(const brm_batch_plan* p, const int64_t** crpt, const int** ccol, const double** cval,
+                          int64_t* cnnz, const int64_t** row_off) {
+    if (!p || !crpt || !ccol || !cval || !cnnz || !row_off) {
+        set_err("null argument");
+        return BRM_E_INVALID;
+    }
+    *crpt = p->pat.crpt;
+    *ccol = p->ccol;
+    *cval = p->cval;
+    *cnnz = p->pat.nnzC;
+    *row_off = p->row_off.data();
+    return BRM_OK;
+}
+
+int brm_batch_plan_info(const brm_batch_plan* p, char* buf, uint64_t cap) {
+    if (!p || !buf) return BRM_E_INVALID;
+    std::string s = "{";
+    char item[256];
+    snprintf(item, sizeof(item),
+             "\"pairs\": %d, \"rows\": %lld, \"total_ip\": %lld, \"nnz_c\": %lld, \"rows_by_tier\": [%d, %d, %d, %d], "
+             "\"analyse_s\": %.6f, \"lds_cfg\": %d, \"med_cap\": %d, \"tuning_ms\": [",
+             p->fb.f.n, (long long)p->fb.f.M, (long long)p->pat.total_ip, (long long)p->pat.nnzC, p->pat.bc[0],
+             p->pat.bc[1], p->pat.bc[2], p->pat.bc[3], p->analyse_s, p->pat.lds_cfg, p->pat.med_cap);
+    s += item;
+    for (size_t i = 0; i < p->tuned.size(); ++i) {
+        const TuneCand& c = kCands[p->tuned[i].first];
+        snprintf(item, sizeof(item), "%s[%d, %d, %.6f]", i ? ", " : "", c.lds_cfg, c.med_cap, p->tuned[i].second);
+        s += item;
+    }
+    s += "]}";
+    if (s.size() + 1 > cap) return BRM_E_INVALID;
+    memcpy(buf, s.c_str(), s.size() + 1);
+    return BRM_OK;
+}
+
+void brm_batch_plan_destroy(brm_batch_plan* p) {
+    if (!p) return;
+    (void)use_device();
+    (void)hipDeviceSynchronize();
+    delete p;
+    dev_flush_deferred();
+}
+
+}  // extern "C"

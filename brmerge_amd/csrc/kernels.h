@@ -39,7 +39,8 @@ hipError_t launch_short_numeric(const int* rows, int n, const RowArgs& a, const 
 
 // lds.hip
 hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
-hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s);
+hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
+                                 int lds_cfg);
 hipError_t launch_long_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
 hipError_t launch_leaf(const LeafTask* tasks, int64_t n, const RowArgs& a, int* const ws_col[2],
                        double* const ws_val[2], int64_t* slots, int64_t* counters, const OutArgs& o,

@@ -265,9 +265,10 @@ hipError_t leaf_t(const LeafTask* tasks, int64_t n, const RowArgs& a, const WsPt
 
 }  // namespace
 
-hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
+hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
+                                 int lds_cfg) {
     if (n <= 0) return hipSuccess;
-    switch (params().lds_cfg) {
+    switch (lds_cfg) {
         case 1: return medium_numeric_t<256, 2048, 256>(rows, n, a, o, s);
         case 2: return medium_numeric_t<128, 2048, 256>(rows, n, a, o, s);
         default: return medium_numeric_t<256, 4096, 512>(rows, n, a, o, s);

@@ -1,0 +1,137 @@
+// Host-side pipeline pieces shared by the single-product entries (api.hip),
+// the batched and planned entries (batch.hip) and the request server
+// (serve.hip).
+#pragma once
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace brm {
+
+void set_err(const char* fmt, ...);
+int hip_status(hipError_t e, const char* what);
+int use_device();
+int d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
+int check_dev_err(const int* derr, hipStream_t s, const char* stage);
+double now_s();
+brm_stats& thread_stats();
+
+#define HIPC(expr)                                          \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_status(e_, #expr); \
+    } while (0)
+
+#define BRMC(expr)                     \
+    do {                               \
+        int st_ = (expr);              \
+        if (st_ != BRM_OK) return st_; \
+    } while (0)
+
+// Device allocations owned by one call; freed on scope exit unless released.
+struct Scratch {
+    std::vector<void*> bufs;
+    Scratch() = default;
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+    ~Scratch() {
+        for (void* p : bufs) dev_free(p);
+    }
+    template <typename T>
+    int alloc(T** p, int64_t n, const char* what) {
+        *p = (T*)dev_alloc(sizeof(T) * (size_t)(n > 0 ? n : 1));
+        if (!*p) {
+            set_err("device allocation of %s (%lld elements) failed", what, (long long)n);
+            return BRM_E_NOMEM;
+        }
+        bufs.push_back(*p);
+        return BRM_OK;
+    }
+    void release(void* p) {
+        auto it = std::find(bufs.begin(), bufs.end(), p);
+        if (it != bufs.end()) bufs.erase(it);
+    }
+    void free_now(void* p) {
+        release(p);
+        dev_free(p);
+    }
+};
+
+enum Phase { P_PRE, P_FLOP, P_PFLOP, P_LB, P_SYM, P_PNNZ, P_ALLOC, P_COMPUTE, P_COPY, P_CLEAN, P_N };
+
+// hipEvent pairs around phases, summed per phase.
+struct PhaseClock {
+    hipStream_t s;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    int open_phase = -1;
+    hipEvent_t open_ev = nullptr;
+    explicit PhaseClock(hipStream_t st) : s(st) {}
+    PhaseClock(const PhaseClock&) = delete;
+    PhaseClock& operator=(const PhaseClock&) = delete;
+    ~PhaseClock();
+    void begin(int p);
+    void end();
+    void collect(double* out);
+};
+
+// A and B of one product, device pointers.
+struct DevIn {
+    const int* arpt;
+    const int* acol;
+    const double* aval;
+    int64_t nnzA;
+    const int* brpt;
+    const int* bcol;
+    const double* bval;
+    int64_t nnzB;
+    int M, K, N;
+};
+
+struct DevOut {
+    int64_t* crpt;
+    int* ccol;
+    double* cval;
+    int64_t nnz;
+};
+
+// The structure of one product: per-entry product offsets, row bins, and
+// (after pattern_symbolic) the exact row lengths and row pointer of C.
+// Values of A and B are read only by the numeric phase, so one Pattern
+// serves any number of numeric passes over the same sparsity patterns.
+struct Pattern {
+    Scratch sc;
+    RowArgs a{};
+    int64_t nnzA = 0, nnzB = 0;
+    int* derr = nullptr;
+    int* bin_cnt = nullptr;
+    int64_t* ipA = nullptr;
+    int64_t* iprow = nullptr;
+    int64_t* rowcnt = nullptr;
+    int64_t* crpt = nullptr;
+    int* bins = nullptr;  // [3][M]: short, medium, long row lists
+    int bc[BIN_COUNT] = {0, 0, 0, 0};
+    int64_t total_ip = 0;
+    int64_t nnzC = 0;
+    int lds_cfg = 0;
+    int med_cap = MED_CAP;
+    const int* rows(int bin) const { return bins + (int64_t)(bin - 1) * a.M; }
+};
+
+// Validation, product offsets (flop scan) and row bins under the current
+// tier parameters.
+int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p);
+// Re-bins rows for LDS shape `lds_cfg` and medium capacity `med_cap`.
+int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int force_long, hipStream_t s);
+// Exact nnz per row, C's row pointer and nnz(C) (BRMerge-Precise symbolic).
+int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk);
+// Numeric phase of BRMerge-Precise into C (ccol/cval of nnz(C) elements).
+// The short and medium tiers only enqueue kernels (capturable in a graph);
+// the long tier needs host round trips between its launches.
+int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t s);
+int pattern_numeric_long(const Pattern& p, int* ccol, double* cval, hipStream_t s);
+
+int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph);
+
+}  // namespace brm

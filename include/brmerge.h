@@ -181,6 +181,54 @@ typedef struct brm_stats {
 } brm_stats;
 int brm_last_stats(brm_stats *out);
 
+/*
+ * Batched products.  The reference multiplies one pair per call
+ * (brmerge_precise, spgemm.h:468-471); these entries run many independent
+ * products C_i = A_i * B_i in one pass, as the block-diagonal product
+ * diag(A_i) * diag(B_i).  Every C_i is bit-identical to a single call on
+ * (A_i, B_i).  All pointers of a brm_pair are device pointers.
+ */
+typedef struct brm_pair {
+    const int *arpt;
+    const int *acol;
+    const double *aval;
+    int64_t nnzA;
+    const int *brpt;
+    const int *bcol;
+    const double *bval;
+    int64_t nnzB;
+    int M, K, N;
+} brm_pair;
+
+/* One-shot batch (BRMerge-Precise).  C is returned as one device CSR holding
+ * the rows of C_0, C_1, ... in order, with an int64 row pointer and each
+ * product's own column indices; C_i is rows row_off[i] .. row_off[i+1]-1
+ * (row_off: caller-provided host int64[npairs+1]).  Release the three
+ * arrays with brm_dev_free.  Sums of rows, entries of A or entries of B over
+ * the batch must fit int32. */
+int brm_spgemm_batched_dev(const brm_pair *pairs, int npairs,
+                           int64_t **crpt, int **ccol, double **cval, int64_t *cnnz,
+                           int64_t *row_off, brm_precise_timing *timing, void *stream);
+
+/* Batch plan for repeated products over fixed sparsity patterns (values of
+ * A_i and B_i may change between executions; pointers may not).  Creation
+ * builds the fused operands, runs the symbolic phase, allocates C and, when
+ * `autotune` is non-zero, times the tier-shape candidates on this batch and
+ * keeps the fastest.  The numeric phase (value refresh + short and medium
+ * tiers) is captured in one HIP graph; execute replays it on `stream` (and
+ * runs long rows, if any, eagerly after it) and waits for completion.
+ * Result arrays stay owned by the plan until brm_batch_plan_destroy;
+ * *row_off is a host array of npairs+1 entries. */
+typedef struct brm_batch_plan brm_batch_plan;
+int brm_batch_plan_create(const brm_pair *pairs, int npairs, int autotune,
+                          brm_batch_plan **plan, void *stream);
+int brm_batch_plan_execute(brm_batch_plan *plan, void *stream);
+int brm_batch_plan_result(const brm_batch_plan *plan, const int64_t **crpt, const int **ccol,
+                          const double **cval, int64_t *cnnz, const int64_t **row_off);
+/* JSON summary: sizes, tier counts, chosen shape, tuner timings. */
+int brm_batch_plan_info(const brm_batch_plan *plan, char *buf, uint64_t cap);
+void brm_batch_plan_destroy(brm_batch_plan *plan);
+
 /* A plain streaming device copy of `bytes` (a multiple of 16) bytes, the
  * reference kernel against which measured HBM byte counters are scaled. */
 int brm_bench_copy(void *dst, const void *src, uint64_t bytes, void *stream);

@@ -76,3 +76,23 @@ def test_csr_equality_semantics():
     C = bm.gen.er(20, 30, 5, 0)
     with pytest.raises(RuntimeError):
         A == C  # noqa: B015  (reference operator== throws on nnz mismatch)
+
+
+def test_pair_struct_layout_and_batch_argument_checks():
+    assert ctypes.sizeof(_lib.PairC) == 80
+    L = bm.lib()
+    r, c, v = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    row_off = np.zeros(2, np.int64)
+    assert L.brm_spgemm_batched_dev(None, 1, ctypes.byref(r), ctypes.byref(c), ctypes.byref(v),
+                                    ctypes.byref(nnz), row_off.ctypes.data, None, None) == _lib.BRM_E_INVALID
+    bad = (_lib.PairC * 1)()
+    bad[0] = _lib.PairC(None, None, None, 0, None, None, None, 0, -1, 0, 0)
+    assert L.brm_spgemm_batched_dev(ctypes.cast(bad, ctypes.c_void_p), 1, ctypes.byref(r), ctypes.byref(c),
+                                    ctypes.byref(v), ctypes.byref(nnz), row_off.ctypes.data, None,
+                                    None) == _lib.BRM_E_INVALID
+    bad[0] = _lib.PairC(None, None, None, 5, None, None, None, 0, 3, 2, 2)
+    h = ctypes.c_void_p()
+    assert L.brm_batch_plan_create(ctypes.cast(bad, ctypes.c_void_p), 1, 0, ctypes.byref(h),
+                                   None) == _lib.BRM_E_INVALID
+    assert "null" in _lib.last_error()
