@@ -11,6 +11,7 @@ from .spgemm import (  # noqa: F401
     set_param, get_param, prof_reset, prof_report, last_stats,
 )
 from .batch import spgemm_batched, split_products, BatchPlan  # noqa: F401
+from .serve import Server  # noqa: F401
 from . import gen  # noqa: F401
 
 __version__ = "0.1.0"

@@ -51,6 +51,13 @@ class PairC(ctypes.Structure):
                 ("N", ctypes.c_int)]
 
 
+class ServerResultC(ctypes.Structure):
+    _fields_ = [("crpt", ctypes.c_void_p), ("ccol", ctypes.c_void_p), ("cval", ctypes.c_void_p),
+                ("nnz", ctypes.c_int64), ("ip", ctypes.c_int64), ("M", ctypes.c_int), ("N", ctypes.c_int),
+                ("status", ctypes.c_int), ("batch_size", ctypes.c_int), ("t_submit", ctypes.c_double),
+                ("t_start", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
 # Every symbol include/brmerge.h and include/brmerge_gen.h declare.
 EXPORTED = [
     "brm_init", "brm_last_error", "brm_version",
@@ -63,6 +70,8 @@ EXPORTED = [
     "brm_memcpy_h2d", "brm_memcpy_d2h", "brm_prof_reset", "brm_prof_report", "brm_last_stats", "brm_bench_copy",
     "brm_spgemm_batched_dev", "brm_batch_plan_create", "brm_batch_plan_execute", "brm_batch_plan_result",
     "brm_batch_plan_info", "brm_batch_plan_destroy",
+    "brm_server_create", "brm_server_add_b", "brm_server_submit", "brm_server_poll", "brm_server_wait",
+    "brm_server_release", "brm_server_stats", "brm_server_destroy",
     "brm_gen_er", "brm_gen_er_var", "brm_gen_rmat", "brm_gen_stencil27", "brm_gen_aggregation",
     "brm_gen_banded", "brm_csr_transpose", "brm_read_mtx", "brm_write_mtx",
 ]
@@ -117,6 +126,14 @@ def lib():
         "brm_batch_plan_result": ([P, PP, PP, PP, ctypes.POINTER(I64), PP], I),
         "brm_batch_plan_info": ([P, ctypes.c_char_p, U64], I),
         "brm_batch_plan_destroy": ([P], None),
+        "brm_server_create": ([I64, I, I, PP], I),
+        "brm_server_add_b": ([P, P, P, P, I64, I, I, ctypes.POINTER(I)], I),
+        "brm_server_submit": ([P, I, P, P, P, I64, I, ctypes.POINTER(I64)], I),
+        "brm_server_poll": ([P, I64, ctypes.POINTER(I)], I),
+        "brm_server_wait": ([P, I64, P], I),
+        "brm_server_release": ([P, I64], I),
+        "brm_server_stats": ([P, ctypes.c_char_p, U64], I),
+        "brm_server_destroy": ([P], None),
         "brm_gen_er": ([I, I, I, U64, PP, PP, PP], I),
         "brm_gen_er_var": ([I, I, I, I, U64, PP, PP, PP], I),
         "brm_gen_rmat": ([I, I, ctypes.c_double, ctypes.c_double, ctypes.c_double, U64, PP, PP, PP], I),

@@ -42,6 +42,8 @@ thread_local int t_device = 0;
 }  // namespace
 
 brm_stats& thread_stats() { return t_stats; }
+int thread_device() { return t_device; }
+void set_thread_device(int device) { t_device = device; }
 
 void set_err(const char* fmt, ...) {
     char buf[512];
@@ -376,9 +378,8 @@ int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p)
     a.M = in.M;
     a.K = in.K;
     a.N = in.N;
-    HIPC(launch_validate(a, in.nnzA, in.nnzB, p.derr, s));
-    BRMC(check_dev_err(p.derr, s, "validation"));
-    HIPC(launch_validate_sorted(in.brpt, in.bcol, in.K, p.derr, s));
+    HIPC(launch_validate(a, in.nnzA, in.b_checked ? -1 : in.nnzB, p.derr, s));
+    if (!in.b_checked) HIPC(launch_validate_sorted(in.brpt, in.bcol, in.K, p.derr, s));
     BRMC(check_dev_err(p.derr, s, "validation"));
 
     if (clk) clk->begin(P_FLOP);

@@ -156,6 +156,23 @@ __global__ __launch_bounds__(CAT_NT) void k_cat_entries(Fused f, int cols, int v
     }
 }
 
+// Per-product row pointers of a fused result: product p owns fused rows
+// mbase[p] .. mbase[p+1]-1 and entries ebase[p] .. ebase[p] + M_p of
+// rpt_out, which restart at 0; base_out[p] is the product's first entry in
+// the fused column and value arrays.
+__global__ __launch_bounds__(CAT_NT) void k_split_rpt(const int64_t* __restrict__ crpt,
+                                                      const int64_t* __restrict__ mbase,
+                                                      const int64_t* __restrict__ ebase, int n, int64_t total,
+                                                      int64_t* __restrict__ rpt_out, int64_t* __restrict__ base_out) {
+    const int64_t y = (int64_t)blockIdx.x * CAT_NT + threadIdx.x;
+    if (y >= total) return;
+    const int p = seg_search(ebase, 0, n - 1, y);
+    const int64_t i = y - ebase[p];
+    const int64_t b = crpt[mbase[p]];
+    rpt_out[y] = crpt[mbase[p] + i] - b;
+    if (i == 0) base_out[p] = b;
+}
+
 unsigned cat_blocks(int64_t n) {
     const int64_t per = (int64_t)CAT_NT * CAT_ITEMS;
     return (unsigned)((n + per - 1) / per);
@@ -194,7 +211,7 @@ int check_pairs(const brm_pair* pairs, int n) {
     return BRM_OK;
 }
 
-int build_fused(const brm_pair* pairs, int n, hipStream_t s, FusedBatch& fb, int* derr) {
+int build_fused(const brm_pair* pairs, int n, hipStream_t s, FusedBatch& fb, int* derr, const int64_t* a_col_off) {
     BRMC(check_pairs(pairs, n));
     fb.mbase.assign(n + 1, 0);
     fb.kbase.assign(n + 1, 0);
@@ -216,6 +233,13 @@ int build_fused(const brm_pair* pairs, int n, hipStream_t s, FusedBatch& fb, int
     f.K = fb.kbase[n];
     f.nnzA = fb.abase[n];
     f.nnzB = fb.bbase[n];
+    if (a_col_off) {
+        // B operands are fused elsewhere: only A is assembled, with the
+        // caller's per-pair column offsets into that fused B
+        for (int i = 0; i < n; ++i) fb.kbase[i] = a_col_off[i];
+        f.K = 0;
+        f.nnzB = 0;
+    }
     if (f.M > INT32_MAX - 1 || f.K > INT32_MAX - 1 || f.nnzA > INT32_MAX || f.nnzB > INT32_MAX) {
         set_err("fused batch exceeds int32 rows or entries (%lld rows, %lld + %lld entries)", (long long)f.M,
                 (long long)f.nnzA, (long long)f.nnzB);
@@ -283,6 +307,29 @@ struct TuneCand {
 const TuneCand kCands[] = {{0, 4096}, {1, 2048}, {2, 2048}, {2, 1024}, {2, 512}};
 
 }  // namespace
+
+int fuse_pairs(const brm_pair* pairs, int n, const int64_t* a_col_off, hipStream_t s, FusedOut* out) {
+    FusedBatch fb;
+    int* derr;
+    BRMC(fb.sc.alloc(&derr, 1, "error flag"));
+    BRMC(build_fused(pairs, n, s, fb, derr, a_col_off));
+    out->in = fused_in(fb);
+    out->mbase = fb.mbase;
+    out->sc.bufs.swap(fb.sc.bufs);
+    return BRM_OK;
+}
+
+hipError_t launch_split_rpt(const int64_t* crpt, const int64_t* mbase_d, const int64_t* ebase_d, int n,
+                            int64_t total, int64_t* rpt_out, int64_t* base_out, hipStream_t s) {
+    if (n <= 0 || total <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_split_rpt", s);
+        k_split_rpt<<<(unsigned)((total + CAT_NT - 1) / CAT_NT), CAT_NT, 0, s>>>(crpt, mbase_d, ebase_d, n, total,
+                                                                                  rpt_out, base_out);
+    }
+    return hipGetLastError();
+}
+
 }  // namespace brm
 
 using namespace brm;
@@ -314,7 +361,7 @@ int batched_run(const brm_pair* pairs, int n, hipStream_t s, DevOut* out, int64_
     FusedBatch fb;
     int* derr;
     BRMC(fb.sc.alloc(&derr, 1, "error flag"));
-    BRMC(build_fused(pairs, n, s, fb, derr));
+    BRMC(build_fused(pairs, n, s, fb, derr, nullptr));
     BRMC(run_spgemm(fused_in(fb), false, INT64_MAX, s, out, ph));
     if (row_off)
         for (int i = 0; i <= n; ++i) row_off[i] = fb.mbase[i];
@@ -350,7 +397,7 @@ int plan_build(brm_batch_plan* p, const brm_pair* pairs, int n, int autotune, hi
     BRMC(use_device());
     int* derr;
     BRMC(p->fb.sc.alloc(&derr, 1, "error flag"));
-    BRMC(build_fused(pairs, n, s, p->fb, derr));
+    BRMC(build_fused(pairs, n, s, p->fb, derr, nullptr));
     p->row_off = p->fb.mbase;
     Pattern& pat = p->pat;
     BRMC(pattern_prepare(fused_in(p->fb), s, nullptr, pat));

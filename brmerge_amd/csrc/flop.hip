@@ -96,11 +96,12 @@ inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per
 
 hipError_t launch_validate(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* err, hipStream_t s) {
     // Row pointers first: the sortedness check indexes columns through them.
+    // nnzB < 0: B was validated before and is skipped.
     {
         ProfScope ps_("k_validate_rpt", s);
         k_validate_rpt<<<nblk((int64_t)a.M + 1, NT), NT, 0, s>>>(a.arpt, a.M, nnzA, err);
     }
-    {
+    if (nnzB >= 0) {
         ProfScope ps_("k_validate_rpt", s);
         k_validate_rpt<<<nblk((int64_t)a.K + 1, NT), NT, 0, s>>>(a.brpt, a.K, nnzB, err);
     }

@@ -87,6 +87,7 @@ struct DevIn {
     const double* bval;
     int64_t nnzB;
     int M, K, N;
+    int b_checked = 0;  // B already validated (row pointers, column range, sorted rows)
 };
 
 struct DevOut {
@@ -133,5 +134,23 @@ int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t
 int pattern_numeric_long(const Pattern& p, int* ccol, double* cval, hipStream_t s);
 
 int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph);
+
+// The calling thread's device for library calls (set by brm_init).
+int thread_device();
+void set_thread_device(int device);
+
+// Fused operands of a batch: A_i stacked by rows with columns shifted to
+// their B_i's rows; B_i stacked by rows, or, when a_col_off is given, not
+// assembled at all (the A columns are then shifted by a_col_off[i]).
+struct FusedOut {
+    Scratch sc;
+    DevIn in{};
+    std::vector<int64_t> mbase;  // first fused row of each A_i, [n+1]
+};
+int fuse_pairs(const brm_pair* pairs, int n, const int64_t* a_col_off, hipStream_t s, FusedOut* out);
+// Row pointers restarting at 0 for each product of a fused result, and each
+// product's first entry in the fused column/value arrays.
+hipError_t launch_split_rpt(const int64_t* crpt, const int64_t* mbase_d, const int64_t* ebase_d, int n,
+                            int64_t total, int64_t* rpt_out, int64_t* base_out, hipStream_t s);
 
 }  // namespace brm

@@ -229,6 +229,47 @@ int brm_batch_plan_result(const brm_batch_plan *plan, const int64_t **crpt, cons
 int brm_batch_plan_info(const brm_batch_plan *plan, char *buf, uint64_t cap);
 void brm_batch_plan_destroy(brm_batch_plan *plan);
 
+/*
+ * Request server (no counterpart in the reference, which has no serving
+ * path): B operands resident in HBM, C = A * B requests admitted against an
+ * HBM budget by their BRMerge-Upper bound (12 bytes per intermediate product
+ * plus 8 per row pointer entry) and fused into batches as they arrive;
+ * `streams` worker threads run batches concurrently, one HIP stream each.
+ * Each result is bit-identical to a single call on (A, B).
+ *
+ * brm_server_add_b copies B into the server (the caller's arrays may be
+ * freed afterwards); it waits until no request is queued or running.  A of
+ * a submitted request is read when its batch runs: keep it alive until the
+ * request is done.  Results live in HBM until brm_server_release; their
+ * bytes count against the budget until then.  Times are seconds on the
+ * host's steady clock.
+ */
+typedef struct brm_server brm_server;
+typedef struct brm_server_result {
+    const int64_t *crpt;   /* device, M+1 entries, crpt[0] = 0 */
+    const int *ccol;       /* device, nnz entries */
+    const double *cval;    /* device, nnz entries */
+    int64_t nnz;
+    int64_t ip;            /* intermediate products of the request */
+    int M, N;
+    int status;            /* BRM_OK or the failure of the request's batch */
+    int batch_size;        /* requests fused into the request's batch */
+    double t_submit, t_start, t_done;
+} brm_server_result;
+
+int brm_server_create(int64_t budget_bytes, int max_batch, int streams, brm_server **srv);
+int brm_server_add_b(brm_server *srv, const int *brpt, const int *bcol, const double *bval, int64_t nnzB,
+                     int K, int N, int *b_id);
+int brm_server_submit(brm_server *srv, int b_id, const int *arpt, const int *acol, const double *aval,
+                      int64_t nnzA, int M, int64_t *req_id);
+int brm_server_poll(brm_server *srv, int64_t req_id, int *done);
+int brm_server_wait(brm_server *srv, int64_t req_id, brm_server_result *out);
+int brm_server_release(brm_server *srv, int64_t req_id);
+/* JSON: completed, batches, mean_batch, p50_s, p99_s, max_s, span_s, total_ip, in_use_bytes, queued. */
+int brm_server_stats(brm_server *srv, char *buf, uint64_t cap);
+/* Finishes queued requests, stops the workers and frees everything. */
+void brm_server_destroy(brm_server *srv);
+
 /* A plain streaming device copy of `bytes` (a multiple of 16) bytes, the
  * reference kernel against which measured HBM byte counters are scaled. */
 int brm_bench_copy(void *dst, const void *src, uint64_t bytes, void *stream);
