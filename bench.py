@@ -38,12 +38,15 @@ def row_stats(A, B):
     return L, ip
 
 
-def tiers(L, ip, med_cap, force_long=False):
+def tiers(L, ip, med_cap, wmed_cap, force_long=False):
+    """Row tiers as the library bins them: 0 empty, 1 short, 2 medium
+    (workgroup LDS), 3 long, 4 medium (wavefront LDS)."""
     t = np.full(len(ip), 3, np.int8)
     t[ip == 0] = 0
     if not force_long:
-        t[(ip > 0) & (ip <= 64)] = 1
         t[(ip > 64) & (ip <= med_cap) & (L <= 512)] = 2
+        t[(ip > 64) & (ip <= wmed_cap) & (L <= 64)] = 4
+        t[(ip > 0) & (ip <= 64)] = 1
     return t
 
 
@@ -55,11 +58,12 @@ def kernel_bytes(name, A, L, ip, nnz_row, tier, st):
     only, 4 B), C written once (12 B per nnz).  Upper-level merges read and
     write 12 B per element (k_group: each input element once, each output
     once).  Ping-pong traffic inside LDS is not counted."""
-    sel = {"k_short_numeric": 1, "k_short_count": 1, "k_medium": 2, "k_sym_hash": 2, "k_sym_bitmap": 3}
+    sel = {"k_short_numeric": 1, "k_short_count": 1, "k_medium": 2, "k_sym_hash": 2, "k_sym_bitmap": 3,
+           "k_medium_w": 4, "k_sym_hash_w": 4}
     if name in sel:
         m = tier == sel[name]
         nA, nip, nc = int(L[m].sum()), int(ip[m].sum()), int(nnz_row[m].sum())
-        if name in ("k_short_numeric", "k_medium"):
+        if name in ("k_short_numeric", "k_medium", "k_medium_w"):
             return 20 * nA + 12 * nip + 12 * nc + 16 * int(m.sum())
         return 12 * nA + 4 * nip + 8 * int(m.sum())
     if name == "k_leaf":
@@ -176,7 +180,7 @@ def main():
     last.free()
     nnz_row = np.diff(crpt)
 
-    tier = tiers(L, ip, bm.get_param("med_cap"))
+    tier = tiers(L, ip, bm.get_param("med_cap"), bm.get_param("wmed_cap"))
     ms_per_step = 1000.0 * elapsed / args.steps
     gflops = 2.0 * total_ip * args.steps / elapsed / 1e9
     bytes_alg = 4 * (A.M + 1) + 20 * A.nnz + 12 * total_ip + 8 * (A.M + 1) + 12 * nnz_c

@@ -31,7 +31,7 @@
 namespace brm {
 
 Params& params() {
-    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, 0};
+    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, 0, 512};
     return p;
 }
 
@@ -363,7 +363,7 @@ int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p)
     BRMC(sc.alloc(&p.iprow, M, "row ip"));
     BRMC(sc.alloc(&p.rowcnt, M, "row nnz"));
     BRMC(sc.alloc(&p.crpt, (int64_t)M + 1, "crpt"));
-    BRMC(sc.alloc(&p.bins, 3 * (int64_t)M, "row bins"));
+    BRMC(sc.alloc(&p.bins, (BIN_COUNT - 1) * (int64_t)M, "row bins"));
     HIPC(launch_fill_i32(p.derr, 1, 0, s));
     p.nnzA = in.nnzA;
     p.nnzB = in.nnzB;
@@ -388,15 +388,18 @@ int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p)
 
     if (clk) clk->begin(P_LB);
     const LdsShape shape = lds_shape(params().lds_cfg);
-    return pattern_bin(p, params().lds_cfg, std::min(params().med_cap, shape.cap), params().force_long, s);
+    return pattern_bin(p, params().lds_cfg, std::min(params().med_cap, shape.cap), params().wmed_cap,
+                       params().force_long, s);
 }
 
-int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int force_long, hipStream_t s) {
+int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int force_long, hipStream_t s) {
     const LdsShape shape = lds_shape(lds_cfg);
     p.lds_cfg = lds_cfg;
     p.med_cap = std::min(med_cap, shape.cap);
+    p.wmed_cap = std::min(wmed_cap, WMED_CAP);
     HIPC(launch_fill_i32(p.bin_cnt, BIN_COUNT, 0, s));
-    HIPC(launch_rowstats(p.a, p.med_cap, shape.lmax, force_long, p.iprow, p.rowcnt, p.bins, p.bin_cnt, s));
+    HIPC(launch_rowstats(p.a, p.med_cap, shape.lmax, p.wmed_cap, force_long, p.iprow, p.rowcnt, p.bins, p.bin_cnt,
+                         s));
     BRMC(d2h(p.bc, p.bin_cnt, sizeof(p.bc), s));
     return BRM_OK;
 }
@@ -404,6 +407,7 @@ int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int force_long, hipStream_
 int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk) {
     if (clk) clk->begin(P_SYM);
     HIPC(launch_short_count(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, p.rowcnt, s));
+    HIPC(launch_wmed_count(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, p.rowcnt, s));
     HIPC(launch_medium_count(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, p.rowcnt, s));
     HIPC(launch_long_count(p.rows(BIN_LONG), p.bc[BIN_LONG], p.a, p.rowcnt, s));
     if (clk) clk->begin(P_PNNZ);
@@ -427,6 +431,7 @@ OutArgs precise_out(const Pattern& p, int* ccol, double* cval) {
 int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t s) {
     const OutArgs o = precise_out(p, ccol, cval);
     HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
+    HIPC(launch_wmed_numeric(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, o, s, p.wmed_cap));
     HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
     return BRM_OK;
 }
@@ -446,7 +451,8 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     Pattern p;
     BRMC(pattern_prepare(in, s, &clk, p));
     const int M = in.M;
-    for (int b = 0; b < BIN_COUNT; ++b) t_stats.rows_by_tier[b] = p.bc[b];
+    for (int b = 0; b < 4; ++b) t_stats.rows_by_tier[b] = p.bc[b];
+    t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_WMED];
     t_stats.total_ip = p.total_ip;
     Scratch& sc = p.sc;
     int* ccol = nullptr;
@@ -485,6 +491,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         o.cval = bval_bar;
         o.rowlen = p.rowcnt;
         HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
+        HIPC(launch_wmed_numeric(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, o, s, p.wmed_cap));
         HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
         BRMC(run_long(p.a, p.rows(BIN_LONG), p.bc[BIN_LONG], o, nullptr, s, sc));
         BRMC(check_dev_err(p.derr, s, "numeric"));
@@ -844,6 +851,7 @@ int brm_set_param(const char* name, int64_t value) {
     else if (!strcmp(name, "profile")) { p.profile = value ? 1 : 0; }
     else if (!strcmp(name, "lds_cfg")) { if (value < 0 || value >= N_LDS_CFG) return BRM_E_INVALID; p.lds_cfg = (int)value; }
     else if (!strcmp(name, "group_cfg")) { if (value < 0 || value >= N_GROUP_CFG) return BRM_E_INVALID; p.group_cfg = (int)value; }
+    else if (!strcmp(name, "wmed_cap")) { if (value < 0 || value > WMED_CAP) return BRM_E_INVALID; p.wmed_cap = (int)value; }
     else return BRM_E_INVALID;
     return BRM_OK;
 }
@@ -858,6 +866,7 @@ int64_t brm_get_param(const char* name) {
     if (!strcmp(name, "profile")) return p.profile;
     if (!strcmp(name, "lds_cfg")) return p.lds_cfg;
     if (!strcmp(name, "group_cfg")) return p.group_cfg;
+    if (!strcmp(name, "wmed_cap")) return p.wmed_cap;
     return -1;
 }
 

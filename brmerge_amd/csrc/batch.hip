@@ -299,12 +299,15 @@ hipError_t refresh_values(const Fused& f, int* derr, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Tier-shape candidates tried by the plan's tuner: (LDS shape, medium capacity).
+// Tier-shape candidates tried by the plan's tuner: (LDS shape, medium
+// capacity, wavefront-tier capacity).
 struct TuneCand {
     int lds_cfg;
     int med_cap;
+    int wmed_cap;
 };
-const TuneCand kCands[] = {{0, 4096}, {1, 2048}, {2, 2048}, {2, 1024}, {2, 512}};
+const TuneCand kCands[] = {{0, 4096, 0},   {2, 2048, 0},   {0, 4096, 256}, {0, 4096, 512},
+                           {0, 4096, 1024}, {2, 2048, 512}, {2, 2048, 1024}};
 
 }  // namespace
 
@@ -416,7 +419,7 @@ int plan_build(brm_batch_plan* p, const brm_pair* pairs, int n, int autotune, hi
         double best = -1;
         const int nc = (int)(sizeof(kCands) / sizeof(kCands[0]));
         for (int c = 0; c < nc; ++c) {
-            BRMC(pattern_bin(pat, kCands[c].lds_cfg, kCands[c].med_cap, 0, s));
+            BRMC(pattern_bin(pat, kCands[c].lds_cfg, kCands[c].med_cap, kCands[c].wmed_cap, 0, s));
             const double ms = time_numeric(p, s, 2);
             BRMC(check_dev_err(pat.derr, s, "batch tuning"));
             if (ms < 0) {
@@ -429,7 +432,8 @@ int plan_build(brm_batch_plan* p, const brm_pair* pairs, int n, int autotune, hi
                 p->chosen = c;
             }
         }
-        BRMC(pattern_bin(pat, kCands[p->chosen].lds_cfg, kCands[p->chosen].med_cap, 0, s));
+        const TuneCand& best_c = kCands[p->chosen];
+        BRMC(pattern_bin(pat, best_c.lds_cfg, best_c.med_cap, best_c.wmed_cap, 0, s));
     }
     // The graph: value refresh from the caller's arrays, then the short and
     // medium tiers.  Kernels only (no memsets, no host round trips).
@@ -547,13 +551,15 @@ int brm_batch_plan_info(const brm_batch_plan* p, char* buf, uint64_t cap) {
     char item[256];
     snprintf(item, sizeof(item),
              "\"pairs\": %d, \"rows\": %lld, \"total_ip\": %lld, \"nnz_c\": %lld, \"rows_by_tier\": [%d, %d, %d, %d], "
-             "\"analyse_s\": %.6f, \"lds_cfg\": %d, \"med_cap\": %d, \"tuning_ms\": [",
+             "\"analyse_s\": %.6f, \"lds_cfg\": %d, \"med_cap\": %d, \"wmed_cap\": %d, \"tuning_ms\": [",
              p->fb.f.n, (long long)p->fb.f.M, (long long)p->pat.total_ip, (long long)p->pat.nnzC, p->pat.bc[0],
-             p->pat.bc[1], p->pat.bc[2], p->pat.bc[3], p->analyse_s, p->pat.lds_cfg, p->pat.med_cap);
+             p->pat.bc[1], p->pat.bc[2] + p->pat.bc[BIN_WMED], p->pat.bc[3], p->analyse_s, p->pat.lds_cfg,
+             p->pat.med_cap, p->pat.wmed_cap);
     s += item;
     for (size_t i = 0; i < p->tuned.size(); ++i) {
         const TuneCand& c = kCands[p->tuned[i].first];
-        snprintf(item, sizeof(item), "%s[%d, %d, %.6f]", i ? ", " : "", c.lds_cfg, c.med_cap, p->tuned[i].second);
+        snprintf(item, sizeof(item), "%s[%d, %d, %d, %.6f]", i ? ", " : "", c.lds_cfg, c.med_cap, c.wmed_cap,
+                 p->tuned[i].second);
         s += item;
     }
     s += "]}";

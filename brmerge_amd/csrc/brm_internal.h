@@ -15,10 +15,16 @@ enum RowBin : int {
     BIN_SHORT = 1,   // ip <= 64: one wavefront per row, merge in registers
     BIN_MEDIUM = 2,  // ip <= MED_CAP and L <= MED_LMAX: one workgroup per row, merge in LDS
     BIN_LONG = 3,    // otherwise: LDS leaf subtrees + HBM-streamed upper tree levels
-    BIN_COUNT = 4
+    BIN_WMED = 4,    // ip <= wmed_cap and L <= WMED_LMAX: one wavefront per row, merge in LDS
+    BIN_COUNT = 5
 };
 
 constexpr int SHORT_MAX = 64;
+
+// Wavefront-sized LDS merges for small medium rows (several rows resident
+// per CU instead of one 256-thread workgroup per row).
+constexpr int WMED_LMAX = 64;
+constexpr int WMED_CAP = 1024;
 
 // LDS tree-merge configurations (threads, element capacity, list capacity).
 constexpr int MED_NT = 256;
@@ -143,6 +149,7 @@ struct Params {
     int profile;               // bracket every kernel launch with hipEvents
     int lds_cfg;               // LDS tier shape (threads, elements, lists): see lds_shape()
     int group_cfg;             // group-merge shape (threads, window elements): see group_shape()
+    int wmed_cap;              // wavefront LDS tier: products per row (0 = off, <= WMED_CAP)
 };
 
 // Shapes of the LDS tree-merge kernels, selectable at run time.

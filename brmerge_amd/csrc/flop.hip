@@ -43,7 +43,7 @@ __global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __rest
     if (__ballot(bad) != 0ull && lane_id() == 0) atomicOr(err, (int)DERR_INVALID);
 }
 
-__global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int med_lmax, int force_long,
+__global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int med_lmax, int wmed_cap, int force_long,
                                                  int64_t* __restrict__ iprow,
                                                  int64_t* __restrict__ rowcnt, int* __restrict__ bins,
                                                  int* __restrict__ bin_cnt) {
@@ -61,6 +61,8 @@ __global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int med
             bin = BIN_LONG;
         } else if (ip <= SHORT_MAX) {
             bin = BIN_SHORT;
+        } else if (ip <= wmed_cap && L <= WMED_LMAX) {
+            bin = BIN_WMED;
         } else if (ip <= med_cap && L <= med_lmax) {
             bin = BIN_MEDIUM;
         } else {
@@ -125,12 +127,13 @@ hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* er
     return hipGetLastError();
 }
 
-hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int force_long, int64_t* iprow, int64_t* rowcnt,
-                           int* bins, int* bin_cnt, hipStream_t s) {
+hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int wmed_cap, int force_long, int64_t* iprow,
+                           int64_t* rowcnt, int* bins, int* bin_cnt, hipStream_t s) {
     if (a.M <= 0) return hipSuccess;
     {
         ProfScope ps_("k_rowstats", s);
-        k_rowstats<<<nblk(a.M, NT), NT, 0, s>>>(a, med_cap, med_lmax, force_long, iprow, rowcnt, bins, bin_cnt);
+        k_rowstats<<<nblk(a.M, NT), NT, 0, s>>>(a, med_cap, med_lmax, wmed_cap, force_long, iprow, rowcnt, bins,
+                                                bin_cnt);
     }
     return hipGetLastError();
 }

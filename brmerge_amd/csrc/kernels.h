@@ -30,8 +30,9 @@ struct OutArgs {
 // flop.hip
 hipError_t launch_validate(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* err, hipStream_t s);
 hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* err, hipStream_t s);
-hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int force_long, int64_t* iprow, int64_t* nnzrow,
-                           int* bins /*[3][M]*/, int* bin_cnt /*[4]*/, hipStream_t s);
+// bins: [BIN_COUNT-1][M] row lists (bin b at (b-1)*M); bin_cnt: [BIN_COUNT]
+hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int wmed_cap, int force_long, int64_t* iprow,
+                           int64_t* nnzrow, int* bins, int* bin_cnt, hipStream_t s);
 
 // short.hip
 hipError_t launch_short_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
@@ -42,6 +43,9 @@ hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t
 hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
                                  int lds_cfg);
 hipError_t launch_long_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
+hipError_t launch_wmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
+hipError_t launch_wmed_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
+                               int wmed_cap);
 hipError_t launch_leaf(const LeafTask* tasks, int64_t n, const RowArgs& a, int* const ws_col[2],
                        double* const ws_val[2], int64_t* slots, int64_t* counters, const OutArgs& o,
                        hipStream_t s);

@@ -141,13 +141,11 @@ __global__ __launch_bounds__(NT) void k_leaf(const LeafTask* __restrict__ tasks,
 // Medium rows: distinct columns counted with an LDS open-addressing table
 // (the reference uses a per-thread linear-probing table of upper_log(ip)
 // slots, spgemm.h:528-556; any table gives the same count).
-constexpr int HS_NT = 256;
-constexpr int HS_CAP = 2 * MED_CAP;
-
+template <int HS_NT, int HS_CAP, int HS_LMAX>
 __global__ __launch_bounds__(HS_NT) void k_sym_hash(const int* __restrict__ rows, int nrows, RowArgs a,
                                                     int64_t* __restrict__ nnzrow) {
     __shared__ int table[HS_CAP];
-    __shared__ int loff[MED_LMAX + 1];
+    __shared__ int loff[HS_LMAX + 1];
     __shared__ int sm[HS_NT / BRM_WAVE + 1];
     const int t = threadIdx.x;
     const int row = rows[blockIdx.x];
@@ -229,7 +227,8 @@ constexpr size_t bitmap_smem() {
 }
 
 template <int NT, int CAP, int LMAX>
-hipError_t medium_numeric_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
+hipError_t medium_numeric_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
+                            const char* name = "k_medium") {
     using SM = MergeSmem<NT, CAP, LMAX>;
     auto kern = k_medium<NT, CAP, LMAX>;
     static bool attr = false;
@@ -239,7 +238,7 @@ hipError_t medium_numeric_t(const int* rows, int n, const RowArgs& a, const OutA
         attr = true;
     }
     {
-        ProfScope ps_("k_medium", s);
+        ProfScope ps_(name, s);
         kern<<<n, NT, sizeof(SM), s>>>(rows, n, a, o);
     }
     return hipGetLastError();
@@ -291,9 +290,28 @@ hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t
     if (n <= 0) return hipSuccess;
     {
         ProfScope ps_("k_sym_hash", s);
-        k_sym_hash<<<n, HS_NT, 0, s>>>(rows, n, a, nnzrow);
+        k_sym_hash<256, 2 * MED_CAP, MED_LMAX><<<n, 256, 0, s>>>(rows, n, a, nnzrow);
     }
     return hipGetLastError();
+}
+
+// Wavefront tier: one 64-thread workgroup per row, so several rows share a
+// CU and no row waits on another's barriers.
+hipError_t launch_wmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_sym_hash_w", s);
+        k_sym_hash<BRM_WAVE, 2 * WMED_CAP, WMED_LMAX><<<n, BRM_WAVE, 0, s>>>(rows, n, a, nnzrow);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_wmed_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
+                               int wmed_cap) {
+    if (n <= 0) return hipSuccess;
+    if (wmed_cap <= 256) return medium_numeric_t<BRM_WAVE, 256, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");
+    if (wmed_cap <= 512) return medium_numeric_t<BRM_WAVE, 512, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");
+    return medium_numeric_t<BRM_WAVE, WMED_CAP, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");
 }
 
 hipError_t launch_long_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {

@@ -111,20 +111,22 @@ struct Pattern {
     int64_t* iprow = nullptr;
     int64_t* rowcnt = nullptr;
     int64_t* crpt = nullptr;
-    int* bins = nullptr;  // [3][M]: short, medium, long row lists
-    int bc[BIN_COUNT] = {0, 0, 0, 0};
+    int* bins = nullptr;  // [BIN_COUNT-1][M]: row lists of the non-empty bins
+    int bc[BIN_COUNT] = {};
     int64_t total_ip = 0;
     int64_t nnzC = 0;
     int lds_cfg = 0;
     int med_cap = MED_CAP;
+    int wmed_cap = 0;
     const int* rows(int bin) const { return bins + (int64_t)(bin - 1) * a.M; }
 };
 
 // Validation, product offsets (flop scan) and row bins under the current
 // tier parameters.
 int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p);
-// Re-bins rows for LDS shape `lds_cfg` and medium capacity `med_cap`.
-int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int force_long, hipStream_t s);
+// Re-bins rows for LDS shape `lds_cfg`, medium capacity `med_cap` and
+// wavefront-tier capacity `wmed_cap`.
+int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int force_long, hipStream_t s);
 // Exact nnz per row, C's row pointer and nnz(C) (BRMerge-Precise symbolic).
 int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk);
 // Numeric phase of BRMerge-Precise into C (ccol/cval of nnz(C) elements).

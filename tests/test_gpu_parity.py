@@ -18,14 +18,16 @@ TIER_SETTINGS = {
     "default": {},
     "all_long": {"force_long": 1},
     "tiny_leaves": {"force_long": 1, "leaf_cap": 8},
-    "small_medium": {"med_cap": 64, "leaf_cap": 100},
+    "small_medium": {"med_cap": 64, "leaf_cap": 100, "wmed_cap": 0},
+    "wave_medium": {"wmed_cap": 1024},
+    "no_wave_medium": {"wmed_cap": 0},
 }
 
 
 @pytest.fixture
 def tier(request, gpu):
     name = request.param
-    saved = {k: gpu.get_param(k) for k in ("force_long", "leaf_cap", "med_cap")}
+    saved = {k: gpu.get_param(k) for k in ("force_long", "leaf_cap", "med_cap", "wmed_cap")}
     for k, v in TIER_SETTINGS[name].items():
         gpu.set_param(k, v)
     yield name
