@@ -199,6 +199,9 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     };
 
     tr.mark("long info read back");
+    const int group_cfg = params().group_cfg;
+    const bool bitmap = group_cfg == GROUP_CFG_BITMAP;
+    const int pass_bits = bitmap ? BPASS_BITS : PASS_BITS;
     // ---- partition into waves and size every buffer for the largest wave
     struct Wave {
         int k0, k1;
@@ -222,7 +225,7 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         for (int k = 0; k < k1 - k0; ++k) {
             const int Lr = len[k0 + k];
             const int H = ceil_log2_host(Lr);
-            const int J = H == 0 ? 1 : (H + PASS_BITS - 1) / PASS_BITS;
+            const int J = H == 0 ? 1 : (H + pass_bits - 1) / pass_bits;
             if (J > MAX_PASSES) {
                 set_err("long row with 2^%d lists exceeds the supported tree height", H);
                 return BRM_E_INVALID;
@@ -233,7 +236,7 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
             w.sl_tot += (int64_t)2 << H;
             w.leaf_cap += Lr;
             w.child_cap += 2 * (int64_t)Lr + 2;
-            for (int j = 1; j <= J; ++j) capP[j] += (int64_t)(Lr >> std::min(j * PASS_BITS, H)) + 2;
+            for (int j = 1; j <= J; ++j) capP[j] += (int64_t)(Lr >> std::min(j * pass_bits, H)) + 2;
         }
         w.seg.assign(MAX_PASSES + 2, 0);
         for (int j = 0; j <= MAX_PASSES; ++j) w.seg[j + 1] = w.seg[j] + capP[j];
@@ -307,9 +310,10 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         la.task_seg = seg_d;
         la.children = children;
         la.child_cap = w.child_cap;
-        la.frontier_cap = std::min(params().leaf_cap, sh.cap);
+        la.frontier_cap = bitmap ? 0 : std::min(params().leaf_cap, sh.cap);
         la.lmax = sh.lmax;
         la.log_lmax = sh.log_lmax;
+        la.pass_bits = pass_bits;
         tr.mark("wave uploaded", nw);
         HIPC(launch_plan(la, derr, s));
         tr.mark("planned");
@@ -322,7 +326,8 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         const int passes = (int)cnt[CNT_MAXP];
         int64_t ntask = 0;
         for (int j = 1; j <= passes && j <= MAX_PASSES; ++j) {
-            HIPC(launch_group(tasks + w.seg[j], cnt[CNT_TASK0 + j], children, a, wcol, wval, slots, counters, o, s));
+            HIPC(launch_group(tasks + w.seg[j], cnt[CNT_TASK0 + j], children, a, wcol, wval, slots, counters, o, s,
+                              group_cfg));
             ntask += cnt[CNT_TASK0 + j];
             tr.mark("group pass", cnt[CNT_TASK0 + j]);
         }

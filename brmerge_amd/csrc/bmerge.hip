@@ -1,0 +1,514 @@
+// Bitmap-rank group merge for long rows (group_cfg 4).
+//
+// One workgroup materialises one tree node above the lists of a long row
+// from up to 256 child lists (B rows read in place, or nodes of the previous
+// pass), in rounds over a streaming window, like the merge-path engine, but
+// without merge levels:
+//   1. every child loads a window proportional to what it has left; then,
+//      twice, the children whose windows stop short of a target column (the
+//      lower edge of the median log2-extent among partially loaded
+//      children) share the free window space as top-ups; all loaded
+//      elements with a column <= bound (the smallest last loaded column of
+//      a partially loaded child, and at most c_lo + 2^17 - 1) are taken --
+//      every contribution to those columns is in the round;
+//   2. the taken columns set bits of an LDS bitmap over [c_lo, c_lo + 2^17);
+//      a popcount prefix over the bitmap words ranks every distinct column,
+//      which is its output position in the round;
+//   3. contributions are bucketed by output position (counting sort), each
+//      bucket is ordered by child, i.e. by list position, and summed as the
+//      aligned binary tree over list positions sums them
+//      (BRMerge-SpGEMM/inc/spgemm.h:639-707: siblings added left + right,
+//      a node with one present child passes it through).
+// The work per element does not depend on the number of children, so one
+// pass merges 2^8 tree levels' worth of lists.
+#include <limits.h>
+
+#include "brm_device.h"
+#include "kernels.h"
+
+namespace brm {
+namespace {
+
+constexpr int BG_NT = 256;
+constexpr int BG_W = 2048;                    // window elements per round
+constexpr int BG_IPT = BG_W / BG_NT;          // window elements per thread
+constexpr int BG_SPAN_LOG2 = 17;
+constexpr int64_t BG_SPAN = 1 << BG_SPAN_LOG2;  // columns per round
+constexpr int BG_WORDS = (int)(BG_SPAN / 64);
+constexpr int BG_WPT = BG_WORDS / BG_NT;      // bitmap words per thread
+constexpr int BG_KMAX = BGROUP_KMAX;
+constexpr int BG_NW = BG_NT / BRM_WAVE;
+constexpr int BG_NSLOT = 8;    // rotating reduction scratch slots
+constexpr int BG_TOPUPS = 2;   // top-up iterations per round
+static_assert(BG_KMAX <= BG_NT, "one child per thread in the window logic");
+static_assert(BG_W <= 65535, "u16 ranks and contribution indices");
+
+struct BgSmem {
+    double val[BG_W];
+    int col[BG_W];
+    unsigned long long bm[BG_WORDS];
+    int cnt[BG_W];
+    unsigned short wrank[BG_WORDS];
+    unsigned short contrib[BG_W];
+    unsigned char cidx[BG_W];
+    int64_t off[BG_KMAX];
+    int64_t len[BG_KMAX];
+    int64_t cur[BG_KMAX];
+    double scale[BG_KMAX];
+    int w[BG_KMAX];         // elements of the child loaded this round
+    int woff[BG_KMAX + 1];  // proportional windows: offsets in the buffer
+    int lastc[BG_KMAX];     // last loaded column of the child
+    int xoff[BG_KMAX + 1];  // top-up offsets (relative to the fill level)
+    int xamt[BG_KMAX];
+    int take[BG_KMAX];
+    int hist[32];
+    unsigned char kind[BG_KMAX];
+    unsigned char buf[BG_KMAX];
+    unsigned char lid[BG_KMAX];
+    int64_t red64[BG_NSLOT][BG_NW];
+    int red[BG_NSLOT][BG_NW];
+};
+
+struct WsCols {
+    int* col0;
+    int* col1;
+    double* val0;
+    double* val1;
+};
+
+__device__ __forceinline__ int wave_min(int x) {
+#pragma unroll
+    for (int d = BRM_WAVE / 2; d > 0; d >>= 1) x = min(x, __shfl_xor(x, d, BRM_WAVE));
+    return x;
+}
+
+// Block reductions and scans with one barrier each.  Every call takes the
+// next of BG_NSLOT scratch slots (the slot counter is uniform), so a slot is
+// written again only after several more barriers.
+struct Slots {
+    int i = 0;
+    __device__ int next() {
+        const int r = i;
+        i = (i + 1) % BG_NSLOT;
+        return r;
+    }
+};
+
+__device__ __forceinline__ int block_sum_i(BgSmem& g, Slots& sl, int x) {
+    const int k = sl.next();
+    const int s = wave_sum(x);
+    if (lane_id() == 0) g.red[k][threadIdx.x / BRM_WAVE] = s;
+    __syncthreads();
+    int tot = 0;
+#pragma unroll
+    for (int i = 0; i < BG_NW; ++i) tot += g.red[k][i];
+    return tot;
+}
+
+__device__ __forceinline__ int64_t block_sum_l(BgSmem& g, Slots& sl, int64_t x) {
+    const int k = sl.next();
+    const int64_t s = wave_sum(x);
+    if (lane_id() == 0) g.red64[k][threadIdx.x / BRM_WAVE] = s;
+    __syncthreads();
+    int64_t tot = 0;
+#pragma unroll
+    for (int i = 0; i < BG_NW; ++i) tot += g.red64[k][i];
+    return tot;
+}
+
+__device__ __forceinline__ int block_min_i(BgSmem& g, Slots& sl, int x) {
+    const int k = sl.next();
+    const int m = wave_min(x);
+    if (lane_id() == 0) g.red[k][threadIdx.x / BRM_WAVE] = m;
+    __syncthreads();
+    int r = INT_MAX;
+#pragma unroll
+    for (int i = 0; i < BG_NW; ++i) r = min(r, g.red[k][i]);
+    return r;
+}
+
+__device__ __forceinline__ int block_scan_i(BgSmem& g, Slots& sl, int x, int* total) {
+    return block_excl_scan_1b<BG_NT>(x, g.red[sl.next()], total);
+}
+
+// Proportional windows for the next round: everything left if it fits,
+// else a share of BG_W - BG_KMAX proportional to what each child has left
+// (at least one element).  Returns true when nothing is left.  The windows
+// are published by the caller's next barrier.
+__device__ __forceinline__ bool bg_windows(BgSmem& g, Slots& sl, int k) {
+    const int t = threadIdx.x;
+    const int64_t rem = t < k ? g.len[t] - g.cur[t] : 0;
+    const int64_t R = block_sum_l(g, sl, rem);
+    int wi = 0;
+    if (rem > 0) {
+        if (R <= BG_W) {
+            wi = (int)rem;
+        } else {
+            int64_t x = (int64_t)((double)(BG_W - BG_KMAX) * (double)rem / (double)R);
+            if (x < 1) x = 1;
+            if (x > rem) x = rem;
+            wi = (int)x;
+        }
+    }
+    int tot;
+    const int ex = block_scan_i(g, sl, wi, &tot);
+    if (t < k) {
+        g.w[t] = wi;
+        g.woff[t] = ex;
+    }
+    if (t == BG_NT - 1) g.woff[k] = tot;
+    return R == 0;
+}
+
+// Global source of child j's elements.
+struct SrcPtr {
+    const int* c;
+    const double* v;
+    bool virt;
+};
+__device__ __forceinline__ SrcPtr src_of(const BgSmem& g, int j, const RowArgs& a, const WsCols& ws) {
+    const bool virt = g.kind[j] == SRC_VIRTUAL;
+    return SrcPtr{virt ? a.bcol : (g.buf[j] ? ws.col1 : ws.col0), virt ? a.bval : (g.buf[j] ? ws.val1 : ws.val0),
+                  virt};
+}
+
+// Loads this thread's elements of the proportional windows (buffer range
+// t*IPT ..) into registers; cj = child, ci = index in the child's window.
+__device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const RowArgs& a, const WsCols& ws,
+                                         int (&rc)[BG_IPT], double (&rv)[BG_IPT], double (&rs)[BG_IPT],
+                                         int (&cj)[BG_IPT], int (&ci)[BG_IPT], unsigned& vmask) {
+    const int e0 = threadIdx.x * BG_IPT;
+    vmask = 0u;
+    int j = 0;
+    if (e0 < Wn) j = upper_idx(g.woff, 0, k, e0);
+#pragma unroll
+    for (int i = 0; i < BG_IPT; ++i) {
+        const int e = e0 + i;
+        cj[i] = -1;
+        if (e < Wn) {
+            while (j + 1 < k && g.woff[j + 1] <= e) ++j;
+            const int q = e - g.woff[j];
+            const SrcPtr p = src_of(g, j, a, ws);
+            const int64_t x = g.off[j] + g.cur[j] + q;
+            rc[i] = p.c[x];
+            rv[i] = p.v[x];
+            rs[i] = g.scale[j];
+            if (p.virt) vmask |= 1u << i;
+            cj[i] = j;
+            ci[i] = q;
+        }
+    }
+}
+
+// Sum of the m >= 2 contributions contrib[s .. s+m) of one column in the
+// aligned tree over list positions.  Sorting by window index orders them by
+// child, i.e. by list position; values are combined in place in g.val.
+__device__ double bg_tree_sum(BgSmem& g, int s, int m) {
+    unsigned short* c = g.contrib + s;
+    if (m == 2) {
+        int e0 = c[0], e1 = c[1];
+        if (e0 > e1) {
+            const int tmp = e0;
+            e0 = e1;
+            e1 = tmp;
+        }
+        return fadd(g.val[e0], g.val[e1]);
+    }
+    for (int i = 1; i < m; ++i) {  // insertion sort (m is small except for hub columns)
+        const unsigned short x = c[i];
+        int j = i - 1;
+        while (j >= 0 && c[j] > x) {
+            c[j + 1] = c[j];
+            --j;
+        }
+        c[j + 1] = x;
+    }
+    if (m == 3) {
+        const int e0 = c[0], e1 = c[1], e2 = c[2];
+        const int l0 = g.lid[g.cidx[e0]], l1 = g.lid[g.cidx[e1]], l2 = g.lid[g.cidx[e2]];
+        const int d = 31 - __clz(l0 ^ l2);  // the node where l0 and l2 meet splits at bit d
+        if (((l1 >> d) & 1) == 0) return fadd(fadd(g.val[e0], g.val[e1]), g.val[e2]);
+        return fadd(g.val[e0], fadd(g.val[e1], g.val[e2]));
+    }
+    // level by level: entries whose node ids share a parent are added
+    int n = m;
+    for (int lev = 0; n > 1; ++lev) {
+        int o = 0;
+        for (int i = 0; i < n;) {
+            const int e0 = c[i];
+            const int id0 = g.lid[g.cidx[e0]] >> lev;
+            if (i + 1 < n) {
+                const int e1 = c[i + 1];
+                const int id1 = g.lid[g.cidx[e1]] >> lev;
+                if ((id0 >> 1) == (id1 >> 1)) {
+                    g.val[e0] = fadd(g.val[e0], g.val[e1]);
+                    c[o++] = (unsigned short)e0;
+                    i += 2;
+                    continue;
+                }
+            }
+            c[o++] = (unsigned short)e0;
+            ++i;
+        }
+        n = o;
+    }
+    return g.val[c[0]];
+}
+
+__global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ tasks,
+                                                  const Child* __restrict__ children, RowArgs a, WsCols ws,
+                                                  int64_t* __restrict__ slots, int64_t* counters, OutArgs o) {
+    extern __shared__ __align__(16) unsigned char smraw[];
+    BgSmem& g = *reinterpret_cast<BgSmem*>(smraw);
+    const int t = threadIdx.x;
+    const GroupTask T = tasks[blockIdx.x];
+    const int k = T.nchild;
+    Slots sl;
+    if (t < k) {
+        const Child c = children[T.cbase + t];
+        g.kind[t] = (unsigned char)c.src.kind;
+        g.buf[t] = (unsigned char)c.src.buf;
+        g.lid[t] = (unsigned char)c.lid;
+        g.off[t] = c.src.off;
+        g.scale[t] = c.src.scale;
+        g.len[t] = c.src.kind == SRC_VIRTUAL ? c.src.len : slots[c.src.slot];
+        g.cur[t] = 0;
+        g.take[t] = 0;
+    }
+    if (t < 32) g.hist[t] = 0;
+#pragma unroll
+    for (int i = 0; i < BG_WPT; ++i) g.bm[t * BG_WPT + i] = 0ull;
+    __syncthreads();
+
+    int* dc;
+    double* dv;
+    int64_t dcap;
+    if (T.final_dst) {
+        dc = o.ccol + T.dst_off;
+        dv = o.cval + T.dst_off;
+        dcap = o.upper ? INT64_MAX : o.crpt[T.row + 1] - o.crpt[T.row];
+    } else {
+        dc = (T.dst_buf ? ws.col1 : ws.col0) + T.dst_off;
+        dv = (T.dst_buf ? ws.val1 : ws.val0) + T.dst_off;
+        dcap = INT64_MAX;
+    }
+    const int64_t in_total = block_sum_l(g, sl, t < k ? g.len[t] : 0);
+
+    int rc[BG_IPT], cj[BG_IPT], ci[BG_IPT];
+    double rv[BG_IPT], rs[BG_IPT];
+    unsigned vmask = 0u;
+    bool done = bg_windows(g, sl, k);
+    __syncthreads();
+    int Wn = g.woff[k];
+    if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, ci, vmask);
+    int64_t out_n = 0;
+    int rounds = 0;
+    while (!done) {
+        rounds++;
+        // ---- the proportional windows into the buffer (products rounded here)
+#pragma unroll
+        for (int i = 0; i < BG_IPT; ++i) {
+            const int e = t * BG_IPT + i;
+            if (e < Wn) {
+                g.col[e] = rc[i];
+                g.val[e] = ((vmask >> i) & 1u) ? fmul(rs[i], rv[i]) : rv[i];
+                g.cidx[e] = (unsigned char)cj[i];
+                if (ci[i] == g.w[cj[i]] - 1) g.lastc[cj[i]] = rc[i];
+            }
+        }
+        __syncthreads();
+        const int64_t rem = t < k ? g.len[t] - g.cur[t] : 0;
+        const int c_lo = block_min_i(g, sl, (t < k && g.w[t] > 0) ? g.col[g.woff[t]] : INT_MAX);
+        int filled = Wn;
+        // ---- top-ups: children stopping short of the target share the free space
+        for (int it = 0; it < BG_TOPUPS; ++it) {
+            const bool partial = t < k && g.w[t] > 0 && (int64_t)g.w[t] < rem;
+            if (partial) atomicAdd(&g.hist[31 - __clz(g.lastc[t] - c_lo + 1)], 1);
+            const int np = block_sum_i(g, sl, partial ? 1 : 0);
+            if (np == 0 || filled >= BG_W) {
+                if (t < 32) g.hist[t] = 0;  // read by nobody this round
+                break;
+            }
+            // median bucket of the partial children's extents
+            int mb;
+            {
+                const int lane = lane_id();
+                const int hv = lane < 32 ? g.hist[lane] : 0;
+                const int inc = wave_incl_scan(hv);
+                const uint64_t m = __ballot(inc >= (np + 1) / 2);
+                mb = __ffsll((long long)m) - 1;
+            }
+            const int64_t tgt = (int64_t)c_lo + ((int64_t)1 << mb) - 1;
+            const bool shorty = partial && (int64_t)g.lastc[t] < tgt;
+            const int ns = block_sum_i(g, sl, shorty ? 1 : 0);
+            if (t < 32) g.hist[t] = 0;  // every thread has read it before the barrier above
+            const int add = ns > 0 ? (BG_W - filled) / ns : 0;
+            if (add == 0) break;
+            const int amt = shorty ? (int)min((int64_t)add, rem - g.w[t]) : 0;
+            int tot;
+            const int ex = block_scan_i(g, sl, amt, &tot);
+            if (t < k) {
+                g.xoff[t] = ex;
+                g.xamt[t] = amt;
+            }
+            if (t == BG_NT - 1) g.xoff[k] = tot;
+            __syncthreads();
+            for (int e = t; e < tot; e += BG_NT) {
+                const int j = upper_idx(g.xoff, 0, k, e);
+                const int q = e - g.xoff[j];
+                const SrcPtr p = src_of(g, j, a, ws);
+                const int64_t x = g.off[j] + g.cur[j] + g.w[j] + q;
+                const int c = p.c[x];
+                const double v = p.v[x];
+                const int pos = filled + e;
+                g.col[pos] = c;
+                g.val[pos] = p.virt ? fmul(g.scale[j], v) : v;
+                g.cidx[pos] = (unsigned char)j;
+                if (q == g.xamt[j] - 1) g.lastc[j] = c;
+            }
+            __syncthreads();
+            if (t < k) g.w[t] += amt;
+            filled += tot;
+        }
+        // ---- round bound
+        const bool partial = t < k && g.w[t] > 0 && (int64_t)g.w[t] < rem;
+        const int lm = block_min_i(g, sl, partial ? g.lastc[t] : INT_MAX);
+        const int bound = (int)min((int64_t)lm, (int64_t)c_lo + BG_SPAN - 1);
+        // ---- taken elements: bitmap bits, per-child counts; counters cleared
+        unsigned taken = 0u;
+#pragma unroll
+        for (int i = 0; i < BG_IPT; ++i) {
+            const int e = t * BG_IPT + i;
+            if (e < filled) {
+                const int c = g.col[e];
+                if (c <= bound) {
+                    taken |= 1u << i;
+                    const int bit = c - c_lo;
+                    atomicOr(&g.bm[bit >> 6], 1ull << (bit & 63));
+                    atomicAdd(&g.take[g.cidx[e]], 1);
+                }
+            }
+            g.cnt[e] = 0;
+        }
+        __syncthreads();
+        if (t < k) {
+            g.cur[t] += g.take[t];
+            g.take[t] = 0;
+        }
+        // ---- next windows (published by the scan barrier below)
+        done = bg_windows(g, sl, k);
+        // ---- ranks of the bitmap words
+        int pc[BG_WPT];
+        int sum = 0;
+#pragma unroll
+        for (int i = 0; i < BG_WPT; ++i) {
+            pc[i] = __popcll(g.bm[t * BG_WPT + i]);
+            sum += pc[i];
+        }
+        int nout;
+        int wex = block_scan_i(g, sl, sum, &nout);
+        Wn = g.woff[k];
+        if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, ci, vmask);
+#pragma unroll
+        for (int i = 0; i < BG_WPT; ++i) {
+            g.wrank[t * BG_WPT + i] = (unsigned short)wex;
+            wex += pc[i];
+        }
+        __syncthreads();
+        // ---- output position of every taken element; contributions per position
+        int pos[BG_IPT];
+#pragma unroll
+        for (int i = 0; i < BG_IPT; ++i) {
+            pos[i] = 0;
+            if ((taken >> i) & 1u) {
+                const int e = t * BG_IPT + i;
+                const int bit = g.col[e] - c_lo;
+                const int wd = bit >> 6;
+                pos[i] = g.wrank[wd] + __popcll(g.bm[wd] & ((1ull << (bit & 63)) - 1ull));
+                atomicAdd(&g.cnt[pos[i]], 1);
+            }
+        }
+        __syncthreads();
+        // ---- bucket offsets (exclusive scan of the counts)
+        {
+            int loc[BG_IPT];
+            int s = 0;
+#pragma unroll
+            for (int i = 0; i < BG_IPT; ++i) {
+                const int oi = t * BG_IPT + i;
+                loc[i] = oi < nout ? g.cnt[oi] : 0;
+                s += loc[i];
+            }
+            int tot;
+            int ex = block_scan_i(g, sl, s, &tot);
+#pragma unroll
+            for (int i = 0; i < BG_IPT; ++i) {
+                const int oi = t * BG_IPT + i;
+                if (oi < nout) g.cnt[oi] = ex;
+                ex += loc[i];
+            }
+        }
+        __syncthreads();
+        // ---- bucket the contributions (cnt[p] ends as the bucket's end)
+#pragma unroll
+        for (int i = 0; i < BG_IPT; ++i) {
+            if ((taken >> i) & 1u) {
+                const int slot = atomicAdd(&g.cnt[pos[i]], 1);
+                g.contrib[slot] = (unsigned short)(t * BG_IPT + i);
+            }
+        }
+        __syncthreads();
+        // ---- one output per thread (strided: coalesced stores); bitmap cleared
+#pragma unroll
+        for (int i = 0; i < BG_WPT; ++i) g.bm[t * BG_WPT + i] = 0ull;
+        for (int oi = t; oi < nout; oi += BG_NT) {
+            const int s = oi ? g.cnt[oi - 1] : 0;
+            const int m = g.cnt[oi] - s;
+            const int e0 = g.contrib[s];
+            const int c = g.col[e0];
+            const double v = m == 1 ? g.val[e0] : bg_tree_sum(g, s, m);
+            const int64_t q = out_n + oi;
+            if (q < dcap) {
+                dc[q] = c;
+                dv[q] = v;
+            }
+        }
+        out_n += nout;
+        __syncthreads();
+    }
+
+    if (t == 0) {
+        if (T.final_dst) {
+            if (o.upper) o.rowlen[T.row] = out_n;
+            else if (out_n != dcap) atomicOr(o.err, (int)DERR_COUNT_MISMATCH);
+        } else {
+            slots[T.slot] = out_n;
+        }
+        atomicAdd((unsigned long long*)&counters[CNT_GRP_IN], (unsigned long long)in_total);
+        atomicAdd((unsigned long long*)&counters[CNT_GRP_OUT], (unsigned long long)out_n);
+        atomicAdd((unsigned long long*)&counters[CNT_ROUNDS], (unsigned long long)rounds);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
+                         int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
+                         const OutArgs& o, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_bgroup, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)sizeof(BgSmem));
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    WsCols w{ws_col[0], ws_col[1], ws_val[0], ws_val[1]};
+    {
+        ProfScope ps_("k_bgroup", s);
+        k_bgroup<<<(unsigned)n, BG_NT, sizeof(BgSmem), s>>>(tasks, children, a, w, slots, counters, o);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace brm

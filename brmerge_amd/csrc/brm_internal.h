@@ -38,6 +38,10 @@ constexpr int MED_LOG_LMAX = 9;
 constexpr int PASS_BITS = 5;
 constexpr int GROUP_KMAX = 1 << PASS_BITS;
 constexpr int MAX_PASSES = 8;
+// The bitmap-rank group merge (bmerge.hip) takes up to 2^BPASS_BITS children.
+constexpr int BPASS_BITS = 8;
+constexpr int BGROUP_KMAX = 1 << BPASS_BITS;
+constexpr int GROUP_CFG_BITMAP = 4;
 
 constexpr int MAX_HEIGHT = 32;
 
@@ -119,6 +123,7 @@ struct LongArgs {
     int frontier_cap; // frontier: products per LDS subtree
     int lmax;         // frontier: lists per LDS subtree (a power of two)
     int log_lmax;
+    int pass_bits;    // tree levels merged per group pass
 };
 
 // ------------------------------------------------------------------ error flags (device)
@@ -164,7 +169,7 @@ inline LdsShape lds_shape(int cfg) {
         default: return {256, 4096, 512, 9};
     }
 }
-constexpr int N_GROUP_CFG = 4;
+constexpr int N_GROUP_CFG = 5;
 Params& params();
 
 // ------------------------------------------------------------------ profiling

@@ -52,9 +52,15 @@ hipError_t launch_leaf(const LeafTask* tasks, int64_t n, const RowArgs& a, int* 
 
 // long.hip
 hipError_t launch_plan(const LongArgs& la, int* err, hipStream_t s);
+// group_cfg: the engine the tasks were planned for (GROUP_CFG_BITMAP: bmerge.hip)
 hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
                         int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
-                        const OutArgs& o, hipStream_t s);
+                        const OutArgs& o, hipStream_t s, int group_cfg);
+
+// bmerge.hip
+hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
+                         int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
+                         const OutArgs& o, hipStream_t s);
 
 // assemble.hip
 hipError_t launch_compact(const int64_t* crpt, int M, const RowArgs& a, const int* cbar_col,

@@ -7,10 +7,12 @@
 // into a workspace buffer; one with a single non-empty list stays in place
 // as a "virtual" list (that B row, scaled on the fly).
 //
-// Above the frontier the tree is merged PASS_BITS levels per pass: pass j
-// materialises every node at height h_j = min(j * PASS_BITS, H) from its
+// Above the frontier the tree is merged pass_bits levels per pass: pass j
+// materialises every node at height h_j = min(j * pass_bits, H) from its
 // children -- the highest materialised nodes below it (frontier nodes or
-// pass j-1 nodes), at most 2^PASS_BITS of them.  One workgroup streams a
+// pass j-1 nodes), at most 2^pass_bits of them.  (pass_bits is 5 for this
+// file's merge-path engine, 8 for the bitmap engine of bmerge.hip, which
+// also runs without a frontier: every list is a child of its pass-1 node.)  One workgroup streams a
 // node's children through LDS in rounds: every child contributes a window
 // proportional to what it has left, and all elements up to the smallest
 // window-end column are merged (tree order, in LDS) and written out.  Every
@@ -37,7 +39,7 @@ __device__ __forceinline__ int node_end(int L, int s, int h) {
 __device__ __forceinline__ int64_t heap_idx(int H, int s, int h) {
     return ((int64_t)1 << (H - h)) - 1 + (s >> h);
 }
-__device__ __forceinline__ int pass_height(int j, int H) { return min(j * PASS_BITS, H); }
+__device__ __forceinline__ int pass_height(int j, int H, int pb) { return min(j * pb, H); }
 
 struct NodeK {
     int kind;
@@ -114,7 +116,7 @@ __device__ void emit_group(const LongArgs& la, int row, int a0, const int64_t* i
         q += 1 << ch;
     }
     const unsigned long long cb = atomicAdd((unsigned long long*)&la.counters[CNT_CHILD], (unsigned long long)nc);
-    if ((int64_t)cb + nc > la.child_cap || nc > GROUP_KMAX) {
+    if ((int64_t)cb + nc > la.child_cap || nc > (1 << la.pass_bits)) {
         atomicOr(err, (int)DERR_CAPACITY);
         return;
     }
@@ -166,7 +168,8 @@ __global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int* err) {
     const int L = la.arpt[row + 1] - a0;
     const int64_t* ipr = la.ipA + a0;
     const int H = ceil_log2i(L);
-    const int J = H == 0 ? 1 : (H + PASS_BITS - 1) / PASS_BITS;
+    const int pb = la.pass_bits;
+    const int J = H == 0 ? 1 : (H + pb - 1) / pb;
     const int64_t wsb = la.ws_base[r], slb = la.slot_base[r];
     const int64_t final_off = la.final_off ? la.final_off[row] : ipr[0];
 
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int* err) {
             }
             continue;
         }
-        const int jf = hf / PASS_BITS + 1;  // pass that consumes this frontier node
+        const int jf = hf / pb + 1;  // pass that consumes this frontier node
         if (fk.kind == SRC_STORED) {
             const unsigned long long li = atomicAdd((unsigned long long*)&la.counters[CNT_LEAF], 1ull);
             atomicAdd((unsigned long long*)&la.counters[CNT_LEAF_IP], (unsigned long long)fk.ip);
@@ -213,9 +216,9 @@ __global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int* err) {
         }
         // pass nodes whose leftmost frontier node starts at s
         for (int j = jf; j <= J; ++j) {
-            const int hj = pass_height(j, H);
+            const int hj = pass_height(j, H, pb);
             if (s & ((1 << hj) - 1)) break;
-            emit_group(la, row, a0, ipr, L, H, j, s, hj, pass_height(j - 1, H), cap, wsb, slb, final_off, err);
+            emit_group(la, row, a0, ipr, L, H, j, s, hj, pass_height(j - 1, H, pb), cap, wsb, slb, final_off, err);
         }
     }
     if (threadIdx.x == 0) atomicMax((unsigned long long*)&la.counters[CNT_MAXP], (unsigned long long)J);
@@ -453,11 +456,13 @@ hipError_t launch_plan(const LongArgs& la, int* err, hipStream_t s) {
 
 hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
                         int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
-                        const OutArgs& o, hipStream_t s) {
+                        const OutArgs& o, hipStream_t s, int group_cfg) {
     if (n <= 0) return hipSuccess;
+    if (group_cfg == GROUP_CFG_BITMAP)
+        return launch_bgroup(tasks, n, children, a, ws_col, ws_val, slots, counters, o, s);
     WsView w{ws_col[0], ws_col[1], ws_val[0], ws_val[1]};
     ProfScope ps_("k_group", s);
-    switch (params().group_cfg) {
+    switch (group_cfg) {
         case 1: k_group<128, 2048><<<(unsigned)n, 128, 0, s>>>(tasks, children, a, w, slots, counters, o); break;
         case 2: k_group<64, 1024><<<(unsigned)n, 64, 0, s>>>(tasks, children, a, w, slots, counters, o); break;
         case 3: k_group<256, 4096><<<(unsigned)n, 256, 0, s>>>(tasks, children, a, w, slots, counters, o); break;

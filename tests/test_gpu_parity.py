@@ -21,18 +21,30 @@ TIER_SETTINGS = {
     "small_medium": {"med_cap": 64, "leaf_cap": 100, "wmed_cap": 0},
     "wave_medium": {"wmed_cap": 1024},
     "no_wave_medium": {"wmed_cap": 0},
+    "bitmap_all_long": {"force_long": 1, "group_cfg": 4},
+    "bitmap": {"group_cfg": 4},
 }
+KNOBS = ("force_long", "leaf_cap", "med_cap", "wmed_cap", "group_cfg")
 
 
 @pytest.fixture
 def tier(request, gpu):
     name = request.param
-    saved = {k: gpu.get_param(k) for k in ("force_long", "leaf_cap", "med_cap", "wmed_cap")}
+    saved = {k: gpu.get_param(k) for k in KNOBS}
     for k, v in TIER_SETTINGS[name].items():
         gpu.set_param(k, v)
     yield name
     for k, v in saved.items():
         gpu.set_param(k, v)
+
+
+@pytest.fixture(params=[0, 4], ids=["mergepath", "bitmap"])
+def engine(request, gpu):
+    """Long-row group-merge engine: merge-path levels (0) or bitmap ranks (4)."""
+    old = gpu.get_param("group_cfg")
+    gpu.set_param("group_cfg", request.param)
+    yield request.param
+    gpu.set_param("group_cfg", old)
 
 
 def run_both(bm, A, B, i64=False):
@@ -78,7 +90,7 @@ def test_er16384_config(gpu):
 
 
 @pytest.mark.parametrize("scale", [10, 13, 15])
-def test_rmat(gpu, scale):
+def test_rmat(gpu, engine, scale):
     A = gpu.gen.rmat(scale)
     R = oracle.brmerge_precise(as_tuple(A), as_tuple(A))
     assert_csr_equal(gpu.brmerge_precise(A, A), R)
@@ -124,7 +136,7 @@ def test_edge_shapes(gpu):
     assert C.nnz == 0
 
 
-def test_row_with_many_empty_lists(gpu):
+def test_row_with_many_empty_lists(gpu, engine):
     """Rows with thousands of A entries, almost all pointing at empty B rows."""
     rng = np.random.default_rng(5)
     K, N = 20000, 3000
@@ -141,7 +153,7 @@ def test_row_with_many_empty_lists(gpu):
     assert_csr_equal(Cu, R)
 
 
-def test_single_long_list(gpu):
+def test_single_long_list(gpu, engine):
     """A row selecting one B row longer than any LDS tile (a copied list)."""
     rng = np.random.default_rng(6)
     N = 100000
@@ -154,7 +166,7 @@ def test_single_long_list(gpu):
     assert_csr_equal(Cu, R)
 
 
-def test_long_rows_many_levels(gpu):
+def test_long_rows_many_levels(gpu, engine):
     """Rows far beyond one tile with deep trees, dense column overlap."""
     rng = np.random.default_rng(7)
     K, N = 3000, 6000
@@ -265,3 +277,34 @@ def test_signed_zero_products(gpu, tier):
     Cp, Cu = run_both(gpu, A, B)
     assert_csr_equal(Cp, R)
     assert_csr_equal(Cu, R)
+
+
+@pytest.mark.parametrize("force_long", [0, 1])
+def test_wide_columns_and_hub_columns(gpu, engine, force_long):
+    """Columns spread over a million (several bitmap spans per node), a few
+    columns shared by most B rows (many contributions per output, deep
+    trees of pass-throughs), and rows with more than 256 lists (two passes
+    of the bitmap engine)."""
+    old = gpu.get_param("force_long")
+    gpu.set_param("force_long", force_long)
+    try:
+        rng = np.random.default_rng(11)
+        K, N = 4000, 1_000_000
+        hubs = np.array([0, 17, 500_000, 999_999])
+        rows = []
+        for r in range(K):
+            d = int(rng.integers(0, 40))
+            c = rng.choice(N, size=d, replace=False)
+            c = np.union1d(c, hubs[rng.random(len(hubs)) < 0.8])
+            rows.append(np.sort(c))
+        brpt = np.concatenate([[0], np.cumsum([len(b) for b in rows])]).astype(np.int32)
+        B = (brpt, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, brpt[-1]), N)
+        acols = [np.sort(rng.choice(K, size=sz, replace=False)) for sz in (3000, 600, 257, 256, 255, 40, 3)]
+        arpt = np.concatenate([[0], np.cumsum([len(a) for a in acols])]).astype(np.int32)
+        A = (arpt, np.concatenate(acols).astype(np.int32), rng.uniform(-1, 1, arpt[-1]), K)
+        R = oracle.brmerge_precise(A, B)
+        Cp, Cu = run_both(gpu, A, B)
+        assert_csr_equal(Cp, R)
+        assert_csr_equal(Cu, R)
+    finally:
+        gpu.set_param("force_long", old)
