@@ -342,6 +342,11 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         t_stats.group_in += c2[CNT_GRP_IN];
         t_stats.group_out += c2[CNT_GRP_OUT];
         t_stats.passes = std::max<int64_t>(t_stats.passes, passes);
+        if (tr.on && bitmap) {
+            fprintf(stderr, "[brm] bitmap engine cycles by phase (sum over workgroups):");
+            for (int p = 0; p < N_PHASES; ++p) fprintf(stderr, " %.3g", (double)c2[CNT_PH0 + p]);
+            fprintf(stderr, "\n");
+        }
         dev_flush_deferred();
     }
     return BRM_OK;

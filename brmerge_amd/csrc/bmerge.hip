@@ -50,6 +50,7 @@ struct BgSmem {
     int cnt[BG_W];
     unsigned short wrank[BG_WORDS];
     unsigned short contrib[BG_W];
+    unsigned short contrib2[BG_W];  // contributions ordered by child, per output
     unsigned char cidx[BG_W];
     int64_t off[BG_KMAX];
     int64_t len[BG_KMAX];
@@ -201,58 +202,74 @@ __device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const R
 }
 
 // Sum of the m >= 2 contributions contrib[s .. s+m) of one column in the
-// aligned tree over list positions.  Sorting by window index orders them by
-// child, i.e. by list position; values are combined in place in g.val.
+// aligned tree over list positions (node ids lid < 256, 8 levels).
+//
+// Each child contributes at most one element per column, so ranking the
+// contributions by child index -- through a 256-bit presence mask -- orders
+// them by list position (window and top-up elements of a child are not
+// adjacent in the buffer); the ordered list goes to contrib2.  Then one
+// pass folds them: when the next leaf l follows leaf p and they first differ
+// at bit d, every pending left subtree below height d on p's path is
+// complete and is added (left + right) to p's side, which becomes the
+// pending left sibling at height d; at the end all pending left subtrees are
+// added in increasing height.  This is the reference's bracketing: siblings
+// added left + right, a node with one present child passes it through.
 __device__ double bg_tree_sum(BgSmem& g, int s, int m) {
-    unsigned short* c = g.contrib + s;
+    const unsigned short* c = g.contrib + s;
     if (m == 2) {
         int e0 = c[0], e1 = c[1];
-        if (e0 > e1) {
+        if (g.cidx[e0] > g.cidx[e1]) {
             const int tmp = e0;
             e0 = e1;
             e1 = tmp;
         }
         return fadd(g.val[e0], g.val[e1]);
     }
-    for (int i = 1; i < m; ++i) {  // insertion sort (m is small except for hub columns)
-        const unsigned short x = c[i];
-        int j = i - 1;
-        while (j >= 0 && c[j] > x) {
-            c[j + 1] = c[j];
-            --j;
-        }
-        c[j + 1] = x;
+    unsigned long long mk0 = 0ull, mk1 = 0ull, mk2 = 0ull, mk3 = 0ull;
+    for (int i = 0; i < m; ++i) {
+        const int cid = g.cidx[c[i]];
+        const unsigned long long bit = 1ull << (cid & 63);
+        const int wd = cid >> 6;
+        mk0 |= wd == 0 ? bit : 0ull;
+        mk1 |= wd == 1 ? bit : 0ull;
+        mk2 |= wd == 2 ? bit : 0ull;
+        mk3 |= wd == 3 ? bit : 0ull;
     }
-    if (m == 3) {
-        const int e0 = c[0], e1 = c[1], e2 = c[2];
-        const int l0 = g.lid[g.cidx[e0]], l1 = g.lid[g.cidx[e1]], l2 = g.lid[g.cidx[e2]];
-        const int d = 31 - __clz(l0 ^ l2);  // the node where l0 and l2 meet splits at bit d
-        if (((l1 >> d) & 1) == 0) return fadd(fadd(g.val[e0], g.val[e1]), g.val[e2]);
-        return fadd(g.val[e0], fadd(g.val[e1], g.val[e2]));
+    unsigned short* c2 = g.contrib2 + s;
+    for (int i = 0; i < m; ++i) {
+        const int e = c[i];
+        const int cid = g.cidx[e];
+        const int wd = cid >> 6;
+        const unsigned long long below = (1ull << (cid & 63)) - 1ull;
+        int r = (wd > 0 ? __popcll(mk0) : 0) + (wd > 1 ? __popcll(mk1) : 0) + (wd > 2 ? __popcll(mk2) : 0);
+        const unsigned long long mw = wd == 0 ? mk0 : wd == 1 ? mk1 : wd == 2 ? mk2 : mk3;
+        r += __popcll(mw & below);
+        c2[r] = (unsigned short)e;
     }
-    // level by level: entries whose node ids share a parent are added
-    int n = m;
-    for (int lev = 0; n > 1; ++lev) {
-        int o = 0;
-        for (int i = 0; i < n;) {
-            const int e0 = c[i];
-            const int id0 = g.lid[g.cidx[e0]] >> lev;
-            if (i + 1 < n) {
-                const int e1 = c[i + 1];
-                const int id1 = g.lid[g.cidx[e1]] >> lev;
-                if ((id0 >> 1) == (id1 >> 1)) {
-                    g.val[e0] = fadd(g.val[e0], g.val[e1]);
-                    c[o++] = (unsigned short)e0;
-                    i += 2;
-                    continue;
-                }
-            }
-            c[o++] = (unsigned short)e0;
-            ++i;
-        }
-        n = o;
+    double lv[BPASS_BITS];
+    unsigned pend = 0u;
+    int prev = g.lid[g.cidx[c2[0]]];
+    double cur = g.val[c2[0]];
+    for (int i = 1; i < m; ++i) {
+        const int e = c2[i];
+        const int l = g.lid[g.cidx[e]];
+        const int d = 31 - __clz(prev ^ l);
+        double v = cur;
+#pragma unroll
+        for (int h = 0; h < BPASS_BITS; ++h)
+            if (h < d && ((pend >> h) & 1u)) v = fadd(lv[h], v);
+#pragma unroll
+        for (int h = 0; h < BPASS_BITS; ++h)
+            if (h == d) lv[h] = v;
+        pend = (pend & ~((1u << d) - 1u)) | (1u << d);
+        cur = g.val[e];
+        prev = l;
     }
-    return g.val[c[0]];
+    double v = cur;
+#pragma unroll
+    for (int h = 0; h < BPASS_BITS; ++h)
+        if ((pend >> h) & 1u) v = fadd(lv[h], v);
+    return v;
 }
 
 __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ tasks,
@@ -303,6 +320,16 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
     if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, ci, vmask);
     int64_t out_n = 0;
     int rounds = 0;
+    // shader cycles per round phase, seen by thread 0 (barrier waits included)
+    long long tph[N_PHASES] = {};
+    long long tprev = clock64();
+    auto mark = [&](int p) {
+        if (t == 0) {
+            const long long now = clock64();
+            tph[p] += now - tprev;
+            tprev = now;
+        }
+    };
     while (!done) {
         rounds++;
         // ---- the proportional windows into the buffer (products rounded here)
@@ -320,6 +347,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         const int64_t rem = t < k ? g.len[t] - g.cur[t] : 0;
         const int c_lo = block_min_i(g, sl, (t < k && g.w[t] > 0) ? g.col[g.woff[t]] : INT_MAX);
         int filled = Wn;
+        mark(0);
         // ---- top-ups: children stopping short of the target share the free space
         for (int it = 0; it < BG_TOPUPS; ++it) {
             const bool partial = t < k && g.w[t] > 0 && (int64_t)g.w[t] < rem;
@@ -370,6 +398,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             if (t < k) g.w[t] += amt;
             filled += tot;
         }
+        mark(1);
         // ---- round bound
         const bool partial = t < k && g.w[t] > 0 && (int64_t)g.w[t] < rem;
         const int lm = block_min_i(g, sl, partial ? g.lastc[t] : INT_MAX);
@@ -395,6 +424,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             g.cur[t] += g.take[t];
             g.take[t] = 0;
         }
+        mark(2);
         // ---- next windows (published by the scan barrier below)
         done = bg_windows(g, sl, k);
         // ---- ranks of the bitmap words
@@ -415,6 +445,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             wex += pc[i];
         }
         __syncthreads();
+        mark(3);
         // ---- output position of every taken element; contributions per position
         int pos[BG_IPT];
 #pragma unroll
@@ -429,6 +460,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             }
         }
         __syncthreads();
+        mark(4);
         // ---- bucket offsets (exclusive scan of the counts)
         {
             int loc[BG_IPT];
@@ -449,6 +481,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             }
         }
         __syncthreads();
+        mark(5);
         // ---- bucket the contributions (cnt[p] ends as the bucket's end)
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i) {
@@ -458,6 +491,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             }
         }
         __syncthreads();
+        mark(6);
         // ---- one output per thread (strided: coalesced stores); bitmap cleared
 #pragma unroll
         for (int i = 0; i < BG_WPT; ++i) g.bm[t * BG_WPT + i] = 0ull;
@@ -475,6 +509,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         }
         out_n += nout;
         __syncthreads();
+        mark(7);
     }
 
     if (t == 0) {
@@ -487,6 +522,9 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         atomicAdd((unsigned long long*)&counters[CNT_GRP_IN], (unsigned long long)in_total);
         atomicAdd((unsigned long long*)&counters[CNT_GRP_OUT], (unsigned long long)out_n);
         atomicAdd((unsigned long long*)&counters[CNT_ROUNDS], (unsigned long long)rounds);
+#pragma unroll
+        for (int p = 0; p < N_PHASES; ++p)
+            atomicAdd((unsigned long long*)&counters[CNT_PH0 + p], (unsigned long long)tph[p]);
     }
 }
 

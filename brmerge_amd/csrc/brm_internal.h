@@ -100,7 +100,9 @@ constexpr int CNT_GRP_OUT = 5;   // group merge output elements
 constexpr int CNT_CHILD = 6;     // child descriptors used
 constexpr int CNT_ROUNDS = 7;    // streaming rounds of the group merges
 constexpr int CNT_TASK0 = 8;     // CNT_TASK0 + j: group tasks of pass j (1-based)
-constexpr int CNT_N = CNT_TASK0 + MAX_PASSES + 1;
+constexpr int CNT_PH0 = CNT_TASK0 + MAX_PASSES + 1;  // bitmap engine: shader cycles per round phase
+constexpr int N_PHASES = 8;
+constexpr int CNT_N = CNT_PH0 + N_PHASES;
 
 struct LongArgs {
     const int* rows;        // long rows of this wave
