@@ -73,6 +73,20 @@ def kernel_bytes(name, A, L, ip, nnz_row, tier, st):
     return None
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f).get(f"{kernel}|{workload}")
+    except (OSError, ValueError):
+        return None
+    if not t:
+        return None
+    return int(t["read_bytes_per_launch"] + t["write_bytes_per_launch"])
+
+
 def cpu_baseline(A, target_s=20.0):
     """Time the oracle's BRMerge-Precise on a row panel of A*A."""
     import oracle
@@ -107,12 +121,225 @@ def cpu_baseline(A, target_s=20.0):
     }
 
 
+def _stats_ip(A, B):
+    return int(row_stats(A, B)[1].sum())
+
+
+def run_batched(args, bm):
+    """Config: 4,096 independent products C_i = A_i * B_i (square, rows
+    log-uniform in [256, 65536], 4-32 nnz per row, even i Erdos-Renyi, odd i
+    banded with half-width 64; values U[-1,1]; A seed 2i, B seed 2i+1),
+    one fused launch sequence per step, against one call per product."""
+    n = args.pairs
+    t0 = time.perf_counter()
+    host = [bm.gen.batched_pair(i) for i in range(n)]
+    tgen = time.perf_counter() - t0
+    ip = sum(_stats_ip(A, B) for A, B in host)
+    rows = sum(A.M for A, _ in host)
+    dev = [(bm.DeviceCSR.upload(A), bm.DeviceCSR.upload(B)) for A, B in host]
+    nnz_in = sum(A.nnz + B.nnz for A, B in host)
+    del host
+    print(f"[bench] batched: {n} pairs, {rows} rows, {nnz_in} input nnz, ip={ip} (generated in {tgen:.1f}s)",
+          file=sys.stderr, flush=True)
+    # one call per product on a sample (the same distribution)
+    ns = min(n, args.loop_sample)
+    C = bm.spgemm_dev(*dev[0])
+    C.free()
+    t0 = time.perf_counter()
+    for i in range(ns):
+        C = bm.spgemm_dev(*dev[i])
+        C.free()
+    loop_rate = ns / (time.perf_counter() - t0)
+    print(f"[bench] one call per product: {loop_rate:.1f} products/s over {ns}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        C, ro = bm.spgemm_batched(dev)
+        C.free()
+    bm.set_param("profile", 0 if args.no_profile else 1)
+    bm.prof_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        C, ro = bm.spgemm_batched(dev)
+        nnz_c = C.nnz
+        C.free()
+    el = (time.perf_counter() - t0) / args.steps
+    kernels = {k: {"launches": n_, "ms": round(ms / args.steps, 3)} for k, (n_, ms) in bm.prof_report().items()}
+    bm.set_param("profile", 0)
+    st = bm.last_stats()
+    print(f"[bench] fused batch: {el * 1e3:.1f} ms per batch, nnz(C)={nnz_c}", file=sys.stderr, flush=True)
+    plan_out = None
+    if not args.no_plan:
+        t0 = time.perf_counter()
+        plan = bm.BatchPlan(dev, autotune=True)
+        tcreate = time.perf_counter() - t0
+        plan.execute()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.execute()
+        pel = (time.perf_counter() - t0) / args.steps
+        info = plan.info()
+        plan.close()
+        plan_out = {"products_per_s": round(n / pel, 2), "ms_per_execute": round(pel * 1e3, 3),
+                    "create_s": round(tcreate, 3), "gflops": round(2.0 * ip / pel / 1e9, 3),
+                    "tuned": {"lds_cfg": info["lds_cfg"], "med_cap": info["med_cap"], "wmed_cap": info["wmed_cap"],
+                              "tuning_ms": info["tuning_ms"]}}
+    for a, b in dev:
+        a.free()
+        b.free()
+    rate = n / el
+    return {
+        "metric": f"Batched SpGEMM products/s ({n} independent pairs, BRMerge-Precise)",
+        "value": round(rate, 2), "unit": "products/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(el * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched{n}", "pairs": n, "rows": rows, "input_nnz": nnz_in,
+                   "intermediate_products": ip, "nnz_c": int(nnz_c), "inputs": "resident in HBM"},
+        "gflops": round(2.0 * ip / el / 1e9, 3),
+        "one_call_per_product": {"products_per_s": round(loop_rate, 2), "sample": ns},
+        "speedup_vs_loop": round(rate / loop_rate, 2),
+        "plan": plan_out, "kernels_per_step": kernels, "stats": st,
+    }
+
+
+def run_serve(args, bm):
+    """Config: 8 resident B operands (R-MAT scale 18, seeds 0-7); requests
+    A with round(lognormal(ln 4096, 1.5)) rows, 8-24 uniform columns per row
+    over B's rows; Poisson arrivals; admission by the Upper bound against a
+    64 GiB budget.  Reports p50/p99 latency at the offered rate, saturated
+    throughput, and one call per request for comparison."""
+    scale = 18
+    Bs = [bm.gen.rmat(scale, seed=s) for s in range(8)]
+    K = 1 << scale
+    dBs = [bm.DeviceCSR.upload(B) for B in Bs]
+    rng = np.random.default_rng(0)
+    n = args.requests
+    Ms = np.clip(np.round(np.exp(rng.normal(np.log(4096), 1.5, n))), 1, 1 << 20).astype(int)
+    bsel = rng.integers(0, len(Bs), n)
+    As = [bm.gen.er_var(int(M), K, 8, 24, 7919 + i) for i, M in enumerate(Ms)]
+    ips = [_stats_ip(A, Bs[b]) for A, b in zip(As, bsel)]
+    dAs = [bm.DeviceCSR.upload(A) for A in As]
+    del As
+    ip_tot = int(sum(ips))
+    print(f"[bench] serve: {n} requests, median rows {int(np.median(Ms))}, mean ip {ip_tot / n:.3g}",
+          file=sys.stderr, flush=True)
+    # one call per request, sequential
+    C = bm.spgemm_dev(dAs[0], dBs[bsel[0]])
+    C.free()
+    t0 = time.perf_counter()
+    for i in range(n):
+        C = bm.spgemm_dev(dAs[i], dBs[bsel[i]])
+        C.free()
+    loop_s = time.perf_counter() - t0
+    loop_rate = n / loop_s
+    out = {}
+    with bm.Server(budget_bytes=64 << 30, max_batch=args.max_batch, streams=2) as srv:
+        ids = [srv.add_b(d) for d in dBs]
+
+        def drive(rate):
+            """Submits all requests (Poisson at `rate`/s, or all at once if
+            rate <= 0), waits for all; returns (span_s, latencies, batch sizes)."""
+            gaps = rng.exponential(1.0 / rate, n) if rate > 0 else np.zeros(n)
+            t_start = time.perf_counter()
+            t_next = t_start
+            rids = []
+            for i in range(n):
+                t_next += gaps[i]
+                while True:
+                    d = t_next - time.perf_counter()
+                    if d <= 0:
+                        break
+                    time.sleep(min(d, 0.002))
+                rids.append(srv.submit(ids[bsel[i]], dAs[i]))
+            lat, bsz = [], []
+            for r in rids:
+                res = srv.wait(r)
+                lat.append(res.t_done - res.t_submit)
+                bsz.append(res.batch_size)
+                srv.release(r)
+            return time.perf_counter() - t_start, np.array(lat), np.array(bsz)
+
+        drive(0)  # warm-up
+        span, lat, bsz = drive(0)
+        sat_rate = n / span
+        offered = args.rate if args.rate > 0 else 0.5 * sat_rate
+        span2, lat2, bsz2 = drive(offered)
+    for d in dAs + dBs:
+        d.free()
+    return {
+        "metric": "Served SpGEMM requests/s (8 resident R-MAT-18 B operands, continuous batching)",
+        "value": round(sat_rate, 2), "unit": "requests/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(span * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "serve", "requests": n, "resident_b": len(Bs), "b": f"rmat{scale}",
+                   "median_rows": int(np.median(Ms)), "intermediate_products": ip_tot,
+                   "budget_bytes": 64 << 30, "max_batch": args.max_batch, "streams": 2},
+        "gflops_saturated": round(2.0 * ip_tot / span / 1e9, 3),
+        "saturated": {"requests_per_s": round(sat_rate, 2), "p50_ms": round(1e3 * float(np.median(lat)), 3),
+                      "p99_ms": round(1e3 * float(np.quantile(lat, 0.99)), 3),
+                      "mean_batch": round(float(bsz.mean()), 2)},
+        "poisson": {"offered_per_s": round(offered, 2), "achieved_per_s": round(n / span2, 2),
+                    "p50_ms": round(1e3 * float(np.median(lat2)), 3),
+                    "p99_ms": round(1e3 * float(np.quantile(lat2, 0.99)), 3),
+                    "mean_batch": round(float(bsz2.mean()), 2)},
+        "one_call_per_request": {"requests_per_s": round(loop_rate, 2)},
+        "speedup_vs_loop": round(sat_rate / loop_rate, 2),
+    }
+
+
+def run_amg(args, bm, n):
+    """Config: A = 27-point stencil on an n^3 grid (26 diagonal, -1 off),
+    P = 3x3x3 aggregation (one 1.0 per row); one step computes A*P and then
+    P^T * (A*P) on the device (P^T formed once, outside the timing)."""
+    S, P = bm.gen.stencil27(n), bm.gen.aggregation(n)
+    PT = P.transpose()
+    dS, dP, dPT = bm.DeviceCSR.upload(S), bm.DeviceCSR.upload(P), bm.DeviceCSR.upload(PT)
+    ip1 = _stats_ip(S, P)
+
+    def step():
+        AP = bm.spgemm_dev(dS, dP)
+        APo = AP.as_operand()
+        C = bm.spgemm_dev(dPT, APo)
+        return AP, APo, C
+
+    AP, APo, C = step()
+    APh = AP.download()
+    ip2 = _stats_ip(PT, bm.CSR(np.asarray(APh.rpt, np.int32), APh.col, APh.val, APh.M, APh.N))
+    nnz = (AP.nnz, C.nnz)
+    for x in (AP, C):
+        x.free()
+    APo.rpt.free()
+    for _ in range(args.warmup):
+        AP, APo, C = step()
+        AP.free(); C.free(); APo.rpt.free()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        AP, APo, C = step()
+        AP.free(); C.free(); APo.rpt.free()
+    el = (time.perf_counter() - t0) / args.steps
+    for d in (dS, dP, dPT):
+        d.free()
+    return {
+        "metric": "SpGEMM GFLOP/s (2*intermediate products/s), AMG Galerkin A*P then P^T*(A*P), BRMerge-Precise",
+        "value": round(2.0 * (ip1 + ip2) / el / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(el * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"amg{n}", "grid": n, "rows": S.M, "coarse": P.N, "nnz_a": S.nnz,
+                   "intermediate_products": [ip1, ip2], "nnz_c": list(nnz)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat20", help="rmatS | er16k | amgN")
+    ap.add_argument("--workload", default="rmat20",
+                    help="rmatS | er16k | amgN (stencil*aggregation) | galerkinN (A*P then P^T*(AP)) | batched | serve")
+    ap.add_argument("--pairs", type=int, default=4096, help="batched: number of products")
+    ap.add_argument("--loop-sample", type=int, default=256, help="batched: products timed one call each")
+    ap.add_argument("--no-plan", action="store_true", help="batched: skip the graph-captured plan")
+    ap.add_argument("--requests", type=int, default=256, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=0.0, help="serve: offered requests/s (0: half of saturation)")
+    ap.add_argument("--max-batch", type=int, default=64, help="serve: requests fused per batch")
     ap.add_argument("--upper", action="store_true", help="also time BRMerge-Upper")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="do not bracket launches with hipEvents")
@@ -142,6 +369,16 @@ def main():
         k, v = kv.split("=", 1)
         bm.set_param(k, int(v))
         params[k] = int(v)
+
+    if args.workload == "batched":
+        print(json.dumps(run_batched(args, bm)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(run_serve(args, bm)), flush=True)
+        return
+    if args.workload.startswith("galerkin"):
+        print(json.dumps(run_amg(args, bm, int(args.workload[8:]))), flush=True)
+        return
 
     t_gen = time.perf_counter()
     A, B = bm.gen.workload(args.workload)
@@ -197,7 +434,8 @@ def main():
             avg_s = ms / n / 1e3
             ach = by_per_launch / avg_s / 1e9
             roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(dom, args.workload),
                     "avg_launch_ms": round(ms / n, 4), "alg_bytes_per_launch": int(by_per_launch)}
 
     out = {

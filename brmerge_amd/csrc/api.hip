@@ -793,6 +793,25 @@ void* brm_dev_malloc(uint64_t bytes) {
     return dev_alloc((size_t)bytes);
 }
 void brm_dev_free(void* p) { dev_free(p); }
+
+int brm_rpt_to_i32(const int64_t* in, int64_t n, int* out, void* stream) {
+    if (n < 0 || (n > 0 && (!in || !out))) {
+        set_err("invalid argument");
+        return BRM_E_INVALID;
+    }
+    if (n == 0) return BRM_OK;
+    BRMC(use_device());
+    hipStream_t s = (hipStream_t)stream;
+    int64_t last = 0;
+    BRMC(d2h(&last, in + n - 1, sizeof(int64_t), s));
+    if (last > INT32_MAX) {
+        set_err("row pointer value %lld exceeds int32", (long long)last);
+        return BRM_E_OVERFLOW;
+    }
+    HIPC(launch_rpt_to_i32(in, n, out, s));
+    HIPC(hipStreamSynchronize(s));
+    return BRM_OK;
+}
 void brm_dev_trim(void) { dev_trim(); }
 
 int brm_flop_count_dev(const int* arpt, const int* acol, int64_t nnzA, const int* brpt, int M, int K,

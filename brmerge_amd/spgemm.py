@@ -143,6 +143,15 @@ class DeviceCSR:
         for a in (self.rpt, self.col, self.val):
             a.free()
 
+    def as_operand(self, stream=None):
+        """This CSR with an int32 row pointer (a new device array; the
+        column and value arrays are shared), usable as an operand."""
+        if self.rpt.dtype == np.int32:
+            return self
+        r32 = DeviceArray.alloc(self.M + 1, np.int32)
+        _lib.check(_lib.lib().brm_rpt_to_i32(self.rpt.ptr, self.M + 1, r32.ptr, stream))
+        return DeviceCSR(r32, self.col, self.val, self.M, self.N, self.nnz)
+
 
 def spgemm_dev(dA, dB, upper=False, stream=None, timing=None):
     """C = A*B for operands already in HBM; returns a DeviceCSR (int64 rpt)."""

@@ -130,6 +130,10 @@ int brm_spgemm_upper_dev(const int *arpt, const int *acol, const double *aval, i
                          brm_upper_timing *timing, void *stream);
 
 void *brm_dev_malloc(uint64_t bytes);  /* caching device allocator */
+/* Device copy of an int64 row pointer (n entries) into int32, so that a
+ * product from a _dev entry can be the operand of the next product (e.g.
+ * P^T * (A * P)); returns BRM_E_OVERFLOW if a value exceeds INT32_MAX. */
+int brm_rpt_to_i32(const int64_t *in, int64_t n, int *out, void *stream);
 void brm_dev_free(void *p);
 void brm_dev_trim(void);               /* return cached device memory to the driver */
 
@@ -145,8 +149,10 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
                        int64_t *row_ip, int64_t *total, void *stream);
 
 /* Tuning knobs (names: "ws_budget_bytes", "med_cap", "leaf_cap",
- * "force_long", "profile", "lds_cfg", "group_cfg"); returns BRM_E_INVALID for
- * unknown names or values.  Knobs never change results, only speed. */
+ * "force_long", "profile", "lds_cfg", "group_cfg", "wmed_cap"); returns
+ * BRM_E_INVALID for unknown names or values.  Knobs never change results,
+ * only speed.  group_cfg 0-3: merge-path engine shapes for long rows; 4: the
+ * bitmap-rank engine. */
 int brm_set_param(const char *name, int64_t value);
 int64_t brm_get_param(const char *name);
 
