@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libbrmerge.so")
 DRIVER = os.path.join(HERE, "brm_driver")
 
 HIP_SOURCES = ["alloc.hip", "scan.hip", "flop.hip", "short.hip", "lds.hip", "long.hip", "assemble.hip", "api.hip",
-               "batch.hip", "serve.hip", "bmerge.hip"]
+               "batch.hip", "serve.hip", "bmerge.hip", "rmed.hip"]
 CXX_SOURCES = ["gen.cpp"]
 
 ARCH = os.environ.get("BRM_OFFLOAD_ARCH", "gfx950")

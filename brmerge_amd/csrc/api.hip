@@ -31,7 +31,7 @@
 namespace brm {
 
 Params& params() {
-    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, 0, 512};
+    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, 0, 512, 0, 1};
     return p;
 }
 
@@ -417,7 +417,7 @@ int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int force_lo
 int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk) {
     if (clk) clk->begin(P_SYM);
     HIPC(launch_short_count(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, p.rowcnt, s));
-    HIPC(launch_wmed_count(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, p.rowcnt, s));
+    HIPC(launch_wmed_count(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, p.rowcnt, s, p.wmed_cap));
     HIPC(launch_medium_count(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, p.rowcnt, s));
     HIPC(launch_long_count(p.rows(BIN_LONG), p.bc[BIN_LONG], p.a, p.rowcnt, s));
     if (clk) clk->begin(P_PNNZ);
@@ -881,6 +881,8 @@ int brm_set_param(const char* name, int64_t value) {
     else if (!strcmp(name, "lds_cfg")) { if (value < 0 || value >= N_LDS_CFG) return BRM_E_INVALID; p.lds_cfg = (int)value; }
     else if (!strcmp(name, "group_cfg")) { if (value < 0 || value >= N_GROUP_CFG) return BRM_E_INVALID; p.group_cfg = (int)value; }
     else if (!strcmp(name, "wmed_cap")) { if (value < 0 || value > WMED_CAP) return BRM_E_INVALID; p.wmed_cap = (int)value; }
+    else if (!strcmp(name, "launch_chunk")) { if (value < 0) return BRM_E_INVALID; p.launch_chunk = value; }
+    else if (!strcmp(name, "wmed_engine")) { if (value < 0 || value > 1) return BRM_E_INVALID; p.wmed_engine = (int)value; }
     else return BRM_E_INVALID;
     return BRM_OK;
 }
@@ -896,6 +898,8 @@ int64_t brm_get_param(const char* name) {
     if (!strcmp(name, "lds_cfg")) return p.lds_cfg;
     if (!strcmp(name, "group_cfg")) return p.group_cfg;
     if (!strcmp(name, "wmed_cap")) return p.wmed_cap;
+    if (!strcmp(name, "launch_chunk")) return p.launch_chunk;
+    if (!strcmp(name, "wmed_engine")) return p.wmed_engine;
     return -1;
 }
 

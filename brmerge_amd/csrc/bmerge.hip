@@ -23,6 +23,8 @@
 // pass merges 2^8 tree levels' worth of lists.
 #include <limits.h>
 
+#include <algorithm>
+
 #include "brm_device.h"
 #include "kernels.h"
 
@@ -544,7 +546,10 @@ hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* childre
     WsCols w{ws_col[0], ws_col[1], ws_val[0], ws_val[1]};
     {
         ProfScope ps_("k_bgroup", s);
-        k_bgroup<<<(unsigned)n, BG_NT, sizeof(BgSmem), s>>>(tasks, children, a, w, slots, counters, o);
+        const int64_t ch = launch_chunk(BG_NT);
+        for (int64_t b = 0; b < n; b += ch)
+            k_bgroup<<<(unsigned)std::min<int64_t>(ch, n - b), BG_NT, sizeof(BgSmem), s>>>(tasks + b, children, a, w,
+                                                                                            slots, counters, o);
     }
     return hipGetLastError();
 }

@@ -157,6 +157,8 @@ struct Params {
     int lds_cfg;               // LDS tier shape (threads, elements, lists): see lds_shape()
     int group_cfg;             // group-merge shape (threads, window elements): see group_shape()
     int wmed_cap;              // wavefront LDS tier: products per row (0 = off, <= WMED_CAP)
+    int64_t launch_chunk;      // test hook: rows/tasks per launch (0 = the grid limit)
+    int wmed_engine;           // wavefront tier numeric: 0 LDS merge levels, 1 register bitonic sort
 };
 
 // Shapes of the LDS tree-merge kernels, selectable at run time.
@@ -173,6 +175,16 @@ inline LdsShape lds_shape(int cfg) {
 }
 constexpr int N_GROUP_CFG = 5;
 Params& params();
+
+// ------------------------------------------------------------------ launch limits
+// HSA dispatch grids count work-items in 32 bits: launches with one
+// workgroup (or wavefront) per row or task are split into chunks of at most
+// 2^31 work-items (fewer under the launch_chunk test hook).
+inline int64_t launch_chunk(int threads_per_item) {
+    const int64_t lim = ((int64_t)1 << 31) / threads_per_item;
+    const int64_t o = params().launch_chunk;
+    return (o > 0 && o < lim) ? o : lim;
+}
 
 // ------------------------------------------------------------------ profiling
 void prof_begin(const char* name, hipStream_t s);

@@ -10,6 +10,8 @@
 //
 // Validation replaces the checks the reference's reader performs on load
 // (src/CSR.cpp:454-477) and guards the kernels against out-of-range indices.
+#include <algorithm>
+
 #include "brm_device.h"
 #include "kernels.h"
 
@@ -122,7 +124,11 @@ hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* er
     if (n <= 0) return hipSuccess;
     {
         ProfScope ps_("k_validate_sorted", s);
-        k_validate_sorted<<<nblk(n, NT / BRM_WAVE), NT, 0, s>>>(rpt, col, n, err);
+        const int64_t ch = launch_chunk(BRM_WAVE);
+        for (int64_t b = 0; b < n; b += ch) {
+            const int c = (int)std::min<int64_t>(ch, n - b);
+            k_validate_sorted<<<nblk(c, NT / BRM_WAVE), NT, 0, s>>>(rpt + b, col, c, err);
+        }
     }
     return hipGetLastError();
 }

@@ -43,7 +43,7 @@ hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t
 hipError_t launch_medium_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
                                  int lds_cfg);
 hipError_t launch_long_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
-hipError_t launch_wmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
+hipError_t launch_wmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s, int wmed_cap);
 hipError_t launch_wmed_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
                                int wmed_cap);
 hipError_t launch_leaf(const LeafTask* tasks, int64_t n, const RowArgs& a, int* const ws_col[2],
@@ -56,6 +56,9 @@ hipError_t launch_plan(const LongArgs& la, int* err, hipStream_t s);
 hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
                         int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
                         const OutArgs& o, hipStream_t s, int group_cfg);
+
+// rmed.hip (register tier; cap = products per row, <= WMED_CAP)
+hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s, int cap);
 
 // bmerge.hip
 hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,

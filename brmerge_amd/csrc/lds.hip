@@ -17,6 +17,8 @@
 // is that of the reference's aligned tree.
 #include <limits.h>
 
+#include <algorithm>
+
 #include "brm_device.h"
 #include "kernels.h"
 #include "lds_merge.h"
@@ -34,7 +36,12 @@ __device__ int64_t lds_tree_merge(MergeSmem<NT, CAP, LMAX>& sm, const RowArgs& a
     const int64_t ibase = a.ipA[ab];
     const int S = (int)(a.ipA[ae] - ibase);
 
-    // ---- list descriptors of the non-empty lists
+    // ---- list descriptors of the non-empty lists: node id, offset, and (in
+    //      the second ping-pong buffers, unused until the first merge level)
+    //      the B row start and the A value, so that the expansion below issues
+    //      independent loads only
+    int* bst = sm.lid[1];
+    double* sc = sm.val[1];
     int n = 0;
     for (int cs = ab; cs < ae; cs += NT) {
         const int ai = cs + t;
@@ -46,6 +53,8 @@ __device__ int64_t lds_tree_merge(MergeSmem<NT, CAP, LMAX>& sm, const RowArgs& a
         if (flag && n + ex < LMAX) {
             sm.lid[0][n + ex] = ai - ab;
             sm.loff[0][n + ex] = (int)(a.ipA[ai] - ibase);
+            bst[n + ex] = a.brpt[a.acol[ai]];
+            sc[n + ex] = a.aval[ai];
         }
         n += tot;
     }
@@ -56,15 +65,31 @@ __device__ int64_t lds_tree_merge(MergeSmem<NT, CAP, LMAX>& sm, const RowArgs& a
     if (t == 0) sm.loff[0][n] = S;
     __syncthreads();
 
-    // ---- EXPAND (spgemm.h:618-624): products rounded before any add
+    // ---- EXPAND (spgemm.h:618-624): products rounded before any add; all of
+    //      a thread's loads are issued before its first LDS store
     {
+        constexpr int ITEMS = CAP / NT;
+        int rc[ITEMS];
+        double rv[ITEMS], rs[ITEMS];
         int k = 0;
-        for (int e = t; e < S; e += NT) {
-            k = gallop_idx(sm.loff[0], k, n, e);
-            const int ai = ab + sm.lid[0][k];
-            const int bi = a.brpt[a.acol[ai]] + (e - sm.loff[0][k]);
-            sm.col[0][e] = a.bcol[bi];
-            sm.val[0][e] = fmul(a.aval[ai], a.bval[bi]);
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = t + i * NT;
+            if (e < S) {
+                k = gallop_idx(sm.loff[0], k, n, e);
+                const int bi = bst[k] + (e - sm.loff[0][k]);
+                rc[i] = a.bcol[bi];
+                rv[i] = a.bval[bi];
+                rs[i] = sc[k];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = t + i * NT;
+            if (e < S) {
+                sm.col[0][e] = rc[i];
+                sm.val[0][e] = fmul(rs[i], rv[i]);
+            }
         }
     }
     __syncthreads();
@@ -144,8 +169,10 @@ __global__ __launch_bounds__(NT) void k_leaf(const LeafTask* __restrict__ tasks,
 template <int HS_NT, int HS_CAP, int HS_LMAX>
 __global__ __launch_bounds__(HS_NT) void k_sym_hash(const int* __restrict__ rows, int nrows, RowArgs a,
                                                     int64_t* __restrict__ nnzrow) {
+    constexpr int ITEMS = 8;  // loads in flight per thread
     __shared__ int table[HS_CAP];
     __shared__ int loff[HS_LMAX + 1];
+    __shared__ int bst[HS_LMAX];
     __shared__ int sm[HS_NT / BRM_WAVE + 1];
     const int t = threadIdx.x;
     const int row = rows[blockIdx.x];
@@ -156,20 +183,34 @@ __global__ __launch_bounds__(HS_NT) void k_sym_hash(const int* __restrict__ rows
     int hs = 64;
     while (hs < 2 * S) hs <<= 1;
     for (int k = t; k < hs; k += HS_NT) table[k] = -1;
-    for (int k = t; k <= L; k += HS_NT) loff[k] = (int)(a.ipA[ab + k] - ib);
+    for (int k = t; k <= L; k += HS_NT) {
+        loff[k] = (int)(a.ipA[ab + k] - ib);
+        if (k < L) bst[k] = a.brpt[a.acol[ab + k]];
+    }
     __syncthreads();
     int cnt = 0;
     int hint = 0;
-    for (int e = t; e < S; e += HS_NT) {
-        hint = gallop_idx(loff, hint, L, e);
-        const int ai = ab + hint;
-        const int key = a.bcol[a.brpt[a.acol[ai]] + (e - loff[hint])];
-        unsigned h = ((unsigned)key * 2654435761u) & (unsigned)(hs - 1);
-        for (;;) {
-            const int old = atomicCAS(&table[h], -1, key);
-            if (old == -1) { cnt++; break; }
-            if (old == key) break;
-            h = (h + 1) & (unsigned)(hs - 1);
+    for (int e0 = 0; e0 < S; e0 += ITEMS * HS_NT) {
+        int key[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = e0 + t + i * HS_NT;
+            key[i] = -1;
+            if (e < S) {
+                hint = gallop_idx(loff, hint, L, e);
+                key[i] = a.bcol[bst[hint] + (e - loff[hint])];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (key[i] < 0) continue;
+            unsigned h = ((unsigned)key[i] * 2654435761u) & (unsigned)(hs - 1);
+            for (;;) {
+                const int old = atomicCAS(&table[h], -1, key[i]);
+                if (old == -1) { cnt++; break; }
+                if (old == key[i]) break;
+                h = (h + 1) & (unsigned)(hs - 1);
+            }
         }
     }
     const int tot = block_sum<HS_NT>(cnt, sm);
@@ -239,7 +280,11 @@ hipError_t medium_numeric_t(const int* rows, int n, const RowArgs& a, const OutA
     }
     {
         ProfScope ps_(name, s);
-        kern<<<n, NT, sizeof(SM), s>>>(rows, n, a, o);
+        const int64_t ch = launch_chunk(NT);
+        for (int64_t b = 0; b < n; b += ch) {
+            const int c = (int)std::min<int64_t>(ch, n - b);
+            kern<<<c, NT, sizeof(SM), s>>>(rows + b, c, a, o);
+        }
     }
     return hipGetLastError();
 }
@@ -257,7 +302,9 @@ hipError_t leaf_t(const LeafTask* tasks, int64_t n, const RowArgs& a, const WsPt
     }
     {
         ProfScope ps_("k_leaf", s);
-        kern<<<(unsigned)n, NT, sizeof(SM), s>>>(tasks, a, ws, slots, counters, o);
+        const int64_t ch = launch_chunk(NT);
+        for (int64_t b = 0; b < n; b += ch)
+            kern<<<(unsigned)std::min<int64_t>(ch, n - b), NT, sizeof(SM), s>>>(tasks + b, a, ws, slots, counters, o);
     }
     return hipGetLastError();
 }
@@ -290,18 +337,27 @@ hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t
     if (n <= 0) return hipSuccess;
     {
         ProfScope ps_("k_sym_hash", s);
-        k_sym_hash<256, 2 * MED_CAP, MED_LMAX><<<n, 256, 0, s>>>(rows, n, a, nnzrow);
+        const int64_t ch = launch_chunk(256);
+        for (int64_t b = 0; b < n; b += ch) {
+            const int c = (int)std::min<int64_t>(ch, n - b);
+            k_sym_hash<256, 2 * MED_CAP, MED_LMAX><<<c, 256, 0, s>>>(rows + b, c, a, nnzrow);
+        }
     }
     return hipGetLastError();
 }
 
 // Wavefront tier: one 64-thread workgroup per row, so several rows share a
 // CU and no row waits on another's barriers.
-hipError_t launch_wmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {
+hipError_t launch_wmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s,
+                             int wmed_cap) {
     if (n <= 0) return hipSuccess;
     {
         ProfScope ps_("k_sym_hash_w", s);
-        k_sym_hash<BRM_WAVE, 2 * WMED_CAP, WMED_LMAX><<<n, BRM_WAVE, 0, s>>>(rows, n, a, nnzrow);
+        const int64_t ch = launch_chunk(BRM_WAVE);
+        for (int64_t b = 0; b < n; b += ch) {
+            const int c = (int)std::min<int64_t>(ch, n - b);
+            k_sym_hash<BRM_WAVE, 2 * WMED_CAP, WMED_LMAX><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+        }
     }
     return hipGetLastError();
 }
@@ -309,6 +365,7 @@ hipError_t launch_wmed_count(const int* rows, int n, const RowArgs& a, int64_t* 
 hipError_t launch_wmed_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
                                int wmed_cap) {
     if (n <= 0) return hipSuccess;
+    if (params().wmed_engine == 1) return launch_rmed_numeric(rows, n, a, o, s, wmed_cap);
     if (wmed_cap <= 256) return medium_numeric_t<BRM_WAVE, 256, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");
     if (wmed_cap <= 512) return medium_numeric_t<BRM_WAVE, 512, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");
     return medium_numeric_t<BRM_WAVE, WMED_CAP, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");
@@ -325,7 +382,11 @@ hipError_t launch_long_count(const int* rows, int n, const RowArgs& a, int64_t* 
     }
     {
         ProfScope ps_("k_sym_bitmap", s);
-        k_sym_bitmap<<<n, BM_NT, bitmap_smem(), s>>>(rows, n, a, nnzrow);
+        const int64_t ch = launch_chunk(BM_NT);
+        for (int64_t b = 0; b < n; b += ch) {
+            const int c = (int)std::min<int64_t>(ch, n - b);
+            k_sym_bitmap<<<c, BM_NT, bitmap_smem(), s>>>(rows + b, c, a, nnzrow);
+        }
     }
     return hipGetLastError();
 }

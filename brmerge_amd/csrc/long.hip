@@ -23,6 +23,8 @@
 // product offset in the row and reads buffer (j-1)&1.
 #include <limits.h>
 
+#include <algorithm>
+
 #include "brm_device.h"
 #include "kernels.h"
 #include "lds_merge.h"
@@ -160,9 +162,9 @@ __device__ void emit_group(const LongArgs& la, int row, int a0, const int64_t* i
     la.tasks[la.task_seg[j] + (int64_t)idx] = T;
 }
 
-__global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int* err) {
+__global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int r0, int* err) {
     const Fit cap{la.frontier_cap, la.lmax, la.log_lmax};
-    const int r = blockIdx.x;
+    const int r = r0 + blockIdx.x;
     const int row = la.rows[r];
     const int a0 = la.arpt[row];
     const int L = la.arpt[row + 1] - a0;
@@ -449,7 +451,9 @@ hipError_t launch_plan(const LongArgs& la, int* err, hipStream_t s) {
     if (la.nrows <= 0) return hipSuccess;
     {
         ProfScope ps_("k_plan", s);
-        k_plan<<<la.nrows, PL_NT, 0, s>>>(la, err);
+        const int64_t ch = launch_chunk(PL_NT);
+        for (int64_t b = 0; b < la.nrows; b += ch)
+            k_plan<<<(unsigned)std::min<int64_t>(ch, la.nrows - b), PL_NT, 0, s>>>(la, (int)b, err);
     }
     return hipGetLastError();
 }
@@ -462,11 +466,16 @@ hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children
         return launch_bgroup(tasks, n, children, a, ws_col, ws_val, slots, counters, o, s);
     WsView w{ws_col[0], ws_col[1], ws_val[0], ws_val[1]};
     ProfScope ps_("k_group", s);
-    switch (group_cfg) {
-        case 1: k_group<128, 2048><<<(unsigned)n, 128, 0, s>>>(tasks, children, a, w, slots, counters, o); break;
-        case 2: k_group<64, 1024><<<(unsigned)n, 64, 0, s>>>(tasks, children, a, w, slots, counters, o); break;
-        case 3: k_group<256, 4096><<<(unsigned)n, 256, 0, s>>>(tasks, children, a, w, slots, counters, o); break;
-        default: k_group<256, 2048><<<(unsigned)n, 256, 0, s>>>(tasks, children, a, w, slots, counters, o); break;
+    const int64_t ch = launch_chunk(256);
+    for (int64_t b = 0; b < n; b += ch) {
+        const unsigned c = (unsigned)std::min<int64_t>(ch, n - b);
+        const GroupTask* tb = tasks + b;
+        switch (group_cfg) {
+            case 1: k_group<128, 2048><<<c, 128, 0, s>>>(tb, children, a, w, slots, counters, o); break;
+            case 2: k_group<64, 1024><<<c, 64, 0, s>>>(tb, children, a, w, slots, counters, o); break;
+            case 3: k_group<256, 4096><<<c, 256, 0, s>>>(tb, children, a, w, slots, counters, o); break;
+            default: k_group<256, 2048><<<c, 256, 0, s>>>(tb, children, a, w, slots, counters, o); break;
+        }
     }
     return hipGetLastError();
 }

@@ -15,6 +15,8 @@
 // columns (first occurrence of each column in the rank order).
 #include <limits.h>
 
+#include <algorithm>
+
 #include "brm_device.h"
 #include "kernels.h"
 
@@ -145,7 +147,11 @@ hipError_t launch_short_count(const int* rows, int n, const RowArgs& a, int64_t*
     OutArgs o{};
     {
         ProfScope ps_("k_short_count", s);
-        k_short<false><<<(n + SH_WPB - 1) / SH_WPB, SH_NT, 0, s>>>(rows, n, a, o, nnzrow);
+        const int64_t ch = launch_chunk(BRM_WAVE);
+        for (int64_t b = 0; b < n; b += ch) {
+            const int c = (int)std::min<int64_t>(ch, n - b);
+            k_short<false><<<(c + SH_WPB - 1) / SH_WPB, SH_NT, 0, s>>>(rows + b, c, a, o, nnzrow);
+        }
     }
     return hipGetLastError();
 }
@@ -154,7 +160,11 @@ hipError_t launch_short_numeric(const int* rows, int n, const RowArgs& a, const 
     if (n <= 0) return hipSuccess;
     {
         ProfScope ps_("k_short_numeric", s);
-        k_short<true><<<(n + SH_WPB - 1) / SH_WPB, SH_NT, 0, s>>>(rows, n, a, o, nullptr);
+        const int64_t ch = launch_chunk(BRM_WAVE);
+        for (int64_t b = 0; b < n; b += ch) {
+            const int c = (int)std::min<int64_t>(ch, n - b);
+            k_short<true><<<(c + SH_WPB - 1) / SH_WPB, SH_NT, 0, s>>>(rows + b, c, a, o, nullptr);
+        }
     }
     return hipGetLastError();
 }

@@ -23,8 +23,13 @@ TIER_SETTINGS = {
     "no_wave_medium": {"wmed_cap": 0},
     "bitmap_all_long": {"force_long": 1, "group_cfg": 4},
     "bitmap": {"group_cfg": 4},
+    "tiny_launches": {"launch_chunk": 3, "group_cfg": 4},
+    "register_medium_128": {"wmed_cap": 128, "wmed_engine": 1},
+    "register_medium_512": {"wmed_cap": 512, "wmed_engine": 1, "med_cap": 512},
+    "register_medium_1024": {"wmed_cap": 1024, "wmed_engine": 1},
+    "lds_wave_medium": {"wmed_cap": 512, "wmed_engine": 0},
 }
-KNOBS = ("force_long", "leaf_cap", "med_cap", "wmed_cap", "group_cfg")
+KNOBS = ("force_long", "leaf_cap", "med_cap", "wmed_cap", "group_cfg", "launch_chunk", "wmed_engine")
 
 
 @pytest.fixture
@@ -308,3 +313,33 @@ def test_wide_columns_and_hub_columns(gpu, engine, force_long):
         assert_csr_equal(Cu, R)
     finally:
         gpu.set_param("force_long", old)
+
+
+@pytest.mark.parametrize("engine_wm", [0, 1], ids=["lds", "registers"])
+def test_medium_rows_with_deep_column_trees(gpu, engine_wm):
+    """Rows of 64 lists whose B rows share a handful of columns: every output
+    column sums up to 64 contributions through all six tree levels, with
+    absent lists at irregular positions."""
+    saved = {k: gpu.get_param(k) for k in ("wmed_cap", "wmed_engine")}
+    gpu.set_param("wmed_cap", 1024)
+    gpu.set_param("wmed_engine", engine_wm)
+    try:
+        rng = np.random.default_rng(13)
+        K, N = 300, 40
+        rows = []
+        for r in range(K):
+            d = 0 if r % 7 == 3 else int(rng.integers(1, 9))
+            rows.append(np.sort(rng.choice(8, size=d, replace=False) * 5))
+        brpt = np.concatenate([[0], np.cumsum([len(b) for b in rows])]).astype(np.int32)
+        mags = rng.choice([1.0, 2.0 ** 40, 2.0 ** -40, 3.0], size=brpt[-1])
+        B = (brpt, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, brpt[-1]) * mags, N)
+        acols = [np.sort(rng.choice(K, size=sz, replace=False)) for sz in [64] * 40 + [33, 17, 9, 63]]
+        arpt = np.concatenate([[0], np.cumsum([len(a) for a in acols])]).astype(np.int32)
+        A = (arpt, np.concatenate(acols).astype(np.int32), rng.uniform(-1, 1, arpt[-1]), K)
+        R = oracle.brmerge_precise(A, B, nthreads=4)
+        Cp, Cu = run_both(gpu, A, B)
+        assert_csr_equal(Cp, R)
+        assert_csr_equal(Cu, R)
+    finally:
+        for k, v in saved.items():
+            gpu.set_param(k, v)
