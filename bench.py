@@ -88,12 +88,16 @@ def measured_traffic(kernel, workload):
 
 
 def cpu_baseline(A, target_s=20.0):
-    """Time the oracle's BRMerge-Precise on a row panel of A*A."""
+    """Time the oracle's BRMerge-Precise on A*A, or on a row panel of A
+    (every stride-th row) times A when the whole product is too large for
+    the reference's host run: 1 warm-up, then the mean of as many runs as
+    fit in about target_s seconds (at least 1, at most 1000)."""
     import oracle
     from brmerge_amd import CSR
 
     cores = min(16, os.cpu_count() or 1)
-    stride = 64
+    L, ip = row_stats(A, A)
+    stride = 1 if int(ip.sum()) <= 200_000_000 else 64
     rows = np.arange(0, A.M, stride)
     rpt = np.asarray(A.rpt, np.int64)
     lens = rpt[rows + 1] - rpt[rows]
@@ -101,25 +105,24 @@ def cpu_baseline(A, target_s=20.0):
     idx = np.concatenate([np.arange(rpt[r], rpt[r + 1]) for r in rows]) if len(rows) else np.zeros(0, np.int64)
     P = (prpt, A.col[idx], A.val[idx], A.N)
     B = (np.asarray(A.rpt, np.int32), A.col, A.val, A.N)
-    L, ip = row_stats(CSR(prpt, A.col[idx], A.val[idx], len(rows), A.N), A)
-    ip_panel = int(ip.sum())
+    ip_panel = int(ip[rows].sum())
     t0 = time.perf_counter()
     oracle.brmerge_precise(P, B, nthreads=cores)  # warm-up (reference protocol: 1 + mean of runs)
     t1 = time.perf_counter() - t0
-    reps = int(max(1, min(10, round(target_s / max(t1, 1e-3)))))
+    reps = int(max(1, min(1000, round(target_s / max(t1, 1e-4)))))
     ts = []
     for _ in range(reps):
         t0 = time.perf_counter()
         oracle.brmerge_precise(P, B, nthreads=cores)
         ts.append(time.perf_counter() - t0)
     t = float(np.mean(ts))
+    what = "all rows" if stride == 1 else f"rows i%{stride}==0"
     return {
         "value": round(2.0 * ip_panel / t / 1e9, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-        "sample": (f"oracle BRMerge-Precise (OpenMP C restatement of spgemm.h:468-735) on rows i%{stride}==0 "
+        "sample": (f"oracle BRMerge-Precise (OpenMP C restatement of spgemm.h:468-735) on {what} "
                    f"of the same A ({len(rows)} rows, {ip_panel} products) times A; 1 warm-up + mean of {reps}"),
         "seconds_per_run": round(t, 4),
     }
-
 
 def _stats_ip(A, B):
     return int(row_stats(A, B)[1].sum())
@@ -341,6 +344,8 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0, help="serve: offered requests/s (0: half of saturation)")
     ap.add_argument("--max-batch", type=int, default=64, help="serve: requests fused per batch")
     ap.add_argument("--upper", action="store_true", help="also time BRMerge-Upper")
+    ap.add_argument("--vendor", action="store_true",
+                    help="also time rocSPARSE SpGEMM on the same product (int32 indices: nnz(C) < 2^31)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="do not bracket launches with hipEvents")
     ap.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
@@ -472,6 +477,21 @@ def main():
             C.free()
         out["upper"] = {"value": round(2.0 * total_ip / np.mean(ts) / 1e9, 3), "unit": "GFLOP/s",
                         "ms_per_step": round(1000 * float(np.mean(ts)), 3)}
+    if args.vendor:
+        if nnz_c >= 2 ** 31:
+            out["rocsparse"] = {"skipped": "nnz(C) exceeds the int32 indices of the rocSPARSE path"}
+        else:
+            from tests import _rocsparse
+            _rocsparse.spgemm(bm, dA, dB)  # warm-up
+            ts, tc = [], []
+            for _ in range(max(1, args.steps)):
+                _, t_all, t_comp = _rocsparse.spgemm(bm, dA, dB)
+                ts.append(t_all)
+                tc.append(t_comp)
+            out["rocsparse"] = {"gflops_total": round(2.0 * total_ip / float(np.mean(ts)) / 1e9, 3),
+                                "gflops_compute_stage": round(2.0 * total_ip / float(np.mean(tc)) / 1e9, 3),
+                                "ms_total": round(1e3 * float(np.mean(ts)), 3),
+                                "ms_compute_stage": round(1e3 * float(np.mean(tc)), 3)}
     dA.free()
     if dB is not dA:
         dB.free()

@@ -64,7 +64,7 @@ EXPORTED = [
     "brm_spgemm_precise_i32", "brm_spgemm_upper_i32", "brm_spgemm_precise_i64", "brm_spgemm_upper_i64",
     "brm_host_free_i32", "brm_host_free_i64", "brm_host_free_f64",
     "brm_spgemm_precise_dev", "brm_spgemm_upper_dev",
-    "brm_dev_malloc", "brm_dev_free", "brm_dev_trim", "brm_rpt_to_i32",
+    "brm_dev_malloc", "brm_dev_free", "brm_dev_trim", "brm_rpt_to_i32", "brm_csr_transpose_dev",
     "brm_flop_count", "brm_flop_count_dev",
     "brm_set_param", "brm_get_param", "brm_debug_fail_alloc_after",
     "brm_memcpy_h2d", "brm_memcpy_d2h", "brm_prof_reset", "brm_prof_report", "brm_last_stats", "brm_bench_copy",
@@ -110,6 +110,7 @@ def lib():
         "brm_dev_free": ([P], None),
         "brm_dev_trim": ([], None),
         "brm_rpt_to_i32": ([P, I64, P, P], I),
+        "brm_csr_transpose_dev": ([P, P, P, I64, I, I, P, P, P, P], I),
         "brm_flop_count": ([P, P, P, I, I, ctypes.POINTER(I64)], I),
         "brm_flop_count_dev": ([P, P, I64, P, I, I, P, ctypes.POINTER(I64), P], I),
         "brm_set_param": ([ctypes.c_char_p, I64], I),

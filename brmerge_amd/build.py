@@ -16,9 +16,11 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbrmerge.so")
 DRIVER = os.path.join(HERE, "brm_driver")
+XCHECK = os.path.join(HERE, "libbrm_xcheck.so")
 
 HIP_SOURCES = ["alloc.hip", "scan.hip", "flop.hip", "short.hip", "lds.hip", "long.hip", "assemble.hip", "api.hip",
-               "batch.hip", "serve.hip", "bmerge.hip", "rmed.hip"]
+               "batch.hip", "serve.hip", "bmerge.hip", "rmed.hip",
+               "transpose.hip"]
 CXX_SOURCES = ["gen.cpp"]
 
 ARCH = os.environ.get("BRM_OFFLOAD_ARCH", "gfx950")
@@ -73,6 +75,10 @@ def build(verbose=False):
           os.path.join(CSRC, "driver.cpp"), "-o", DRIVER + ".tmp", "-L", HERE, "-lbrmerge",
           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + "/opt/rocm/lib"])
     os.replace(DRIVER + ".tmp", DRIVER)
+    # vendor cross-check (rocSPARSE), a separate test/bench library
+    _run([hipcc, "-O2", "-fPIC", "-shared", "-std=c++17", os.path.join(HERE, "xcheck", "rocsparse_xcheck.cpp"),
+          "-o", XCHECK + ".tmp", "-L/opt/rocm/lib", "-lrocsparse", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(XCHECK + ".tmp", XCHECK)
     return LIB
 
 

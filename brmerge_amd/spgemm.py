@@ -143,6 +143,17 @@ class DeviceCSR:
         for a in (self.rpt, self.col, self.val):
             a.free()
 
+    def transpose(self, stream=None):
+        """A^T as a new DeviceCSR (int32 row pointer), computed on the GPU."""
+        src = self.as_operand(stream)
+        t = DeviceCSR(DeviceArray.alloc(self.N + 1, np.int32), DeviceArray.alloc(self.nnz, np.int32),
+                      DeviceArray.alloc(self.nnz, np.float64), self.N, self.M, self.nnz)
+        _lib.check(_lib.lib().brm_csr_transpose_dev(src.rpt.ptr, src.col.ptr, src.val.ptr, self.nnz, self.M, self.N,
+                                                    t.rpt.ptr, t.col.ptr, t.val.ptr, stream))
+        if src is not self:
+            src.rpt.free()
+        return t
+
     def as_operand(self, stream=None):
         """This CSR with an int32 row pointer (a new device array; the
         column and value arrays are shared), usable as an operand."""

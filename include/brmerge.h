@@ -134,6 +134,12 @@ void *brm_dev_malloc(uint64_t bytes);  /* caching device allocator */
  * product from a _dev entry can be the operand of the next product (e.g.
  * P^T * (A * P)); returns BRM_E_OVERFLOW if a value exceeds INT32_MAX. */
 int brm_rpt_to_i32(const int64_t *in, int64_t n, int *out, void *stream);
+/* Transpose of an M x N device CSR (int32 indices) into caller-provided
+ * device arrays trpt[N+1], tcol[nnz], tval[nnz]; rows of the transpose come
+ * out sorted.  The reference transposes on the host by sorting
+ * (Existing-SpGEMM/inc/convert.h:6-58); brm_csr_transpose is the host form. */
+int brm_csr_transpose_dev(const int *rpt, const int *col, const double *val, int64_t nnz, int M, int N,
+                          int *trpt, int *tcol, double *tval, void *stream);
 void brm_dev_free(void *p);
 void brm_dev_trim(void);               /* return cached device memory to the driver */
 
