@@ -31,7 +31,7 @@
 namespace brm {
 
 Params& params() {
-    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, 0, 512, 0, 1};
+    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, 512, 0, 1};
     return p;
 }
 

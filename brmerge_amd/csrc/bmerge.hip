@@ -42,6 +42,10 @@ constexpr int BG_KMAX = BGROUP_KMAX;
 constexpr int BG_NW = BG_NT / BRM_WAVE;
 constexpr int BG_NSLOT = 8;    // rotating reduction scratch slots
 constexpr int BG_TOPUPS = 2;   // top-up iterations per round
+// Outputs with more contributions than this are summed by a whole wavefront
+// (dense over the 256 list ids) instead of by one thread.
+constexpr int BG_THREAD_M = 8;
+constexpr int BG_BIGMAX = BG_W / (BG_THREAD_M + 1) + 1;
 static_assert(BG_KMAX <= BG_NT, "one child per thread in the window logic");
 static_assert(BG_W <= 65535, "u16 ranks and contribution indices");
 
@@ -52,8 +56,8 @@ struct BgSmem {
     int cnt[BG_W];
     unsigned short wrank[BG_WORDS];
     unsigned short contrib[BG_W];
-    unsigned short contrib2[BG_W];  // contributions ordered by child, per output
     unsigned char cidx[BG_W];
+    unsigned char elid[BG_W];       // list id (node id) of the element's child
     int64_t off[BG_KMAX];
     int64_t len[BG_KMAX];
     int64_t cur[BG_KMAX];
@@ -68,9 +72,13 @@ struct BgSmem {
     unsigned char kind[BG_KMAX];
     unsigned char buf[BG_KMAX];
     unsigned char lid[BG_KMAX];
+    unsigned short big[BG_BIGMAX];  // outputs left to the wavefront sums
+    int nbig;
     int64_t red64[BG_NSLOT][BG_NW];
     int red[BG_NSLOT][BG_NW];
 };
+
+static_assert(sizeof(BgSmem) <= 80 * 1024, "two workgroups per CU");
 
 struct WsCols {
     int* col0;
@@ -203,75 +211,122 @@ __device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const R
     }
 }
 
-// Sum of the m >= 2 contributions contrib[s .. s+m) of one column in the
-// aligned tree over list positions (node ids lid < 256, 8 levels).
-//
-// Each child contributes at most one element per column, so ranking the
-// contributions by child index -- through a 256-bit presence mask -- orders
-// them by list position (window and top-up elements of a child are not
-// adjacent in the buffer); the ordered list goes to contrib2.  Then one
-// pass folds them: when the next leaf l follows leaf p and they first differ
-// at bit d, every pending left subtree below height d on p's path is
-// complete and is added (left + right) to p's side, which becomes the
-// pending left sibling at height d; at the end all pending left subtrees are
-// added in increasing height.  This is the reference's bracketing: siblings
-// added left + right, a node with one present child passes it through.
-__device__ double bg_tree_sum(BgSmem& g, int s, int m) {
-    const unsigned short* c = g.contrib + s;
-    if (m == 2) {
-        int e0 = c[0], e1 = c[1];
-        if (g.cidx[e0] > g.cidx[e1]) {
-            const int tmp = e0;
-            e0 = e1;
-            e1 = tmp;
+// Sum of output bucket [s, s + m), 2 <= m <= BG_THREAD_M, by one thread in
+// registers: the contributions come from distinct children, so sorting
+// them by list id orders them as the tree's leaves; the sorted values are
+// then folded in one pass: when the next leaf l follows leaf p and they
+// first differ at bit d, every pending left subtree below height d on p's
+// path is complete and is added (left + right) to p's side, which becomes
+// the pending left sibling at height d; at the end all pending left subtrees
+// are added in increasing height.  This is the reference's bracketing:
+// siblings added left + right, a node with one present child passes it
+// through.
+__device__ double bg_small_sum(const BgSmem& g, int s, int m) {
+    constexpr int M = BG_THREAD_M;
+    unsigned key[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        key[i] = ~0u;
+        if (i < m) {
+            const int e = g.contrib[s + i];
+            key[i] = ((unsigned)g.elid[e] << 16) | (unsigned)e;
         }
-        return fadd(g.val[e0], g.val[e1]);
     }
-    unsigned long long mk0 = 0ull, mk1 = 0ull, mk2 = 0ull, mk3 = 0ull;
-    for (int i = 0; i < m; ++i) {
-        const int cid = g.cidx[c[i]];
-        const unsigned long long bit = 1ull << (cid & 63);
-        const int wd = cid >> 6;
-        mk0 |= wd == 0 ? bit : 0ull;
-        mk1 |= wd == 1 ? bit : 0ull;
-        mk2 |= wd == 2 ? bit : 0ull;
-        mk3 |= wd == 3 ? bit : 0ull;
-    }
-    unsigned short* c2 = g.contrib2 + s;
-    for (int i = 0; i < m; ++i) {
-        const int e = c[i];
-        const int cid = g.cidx[e];
-        const int wd = cid >> 6;
-        const unsigned long long below = (1ull << (cid & 63)) - 1ull;
-        int r = (wd > 0 ? __popcll(mk0) : 0) + (wd > 1 ? __popcll(mk1) : 0) + (wd > 2 ? __popcll(mk2) : 0);
-        const unsigned long long mw = wd == 0 ? mk0 : wd == 1 ? mk1 : wd == 2 ? mk2 : mk3;
-        r += __popcll(mw & below);
-        c2[r] = (unsigned short)e;
-    }
+    // Batcher's odd-even merge sort network over M keys
+#pragma unroll
+    for (int p = 1; p < M; p <<= 1)
+#pragma unroll
+        for (int k = p; k >= 1; k >>= 1)
+#pragma unroll
+            for (int j = k % p; j + k < M; j += 2 * k)
+#pragma unroll
+                for (int i = 0; i < k; ++i)
+                    if (i + j + k < M && (i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+                        const unsigned x = key[i + j], y = key[i + j + k];
+                        key[i + j] = x < y ? x : y;
+                        key[i + j + k] = x < y ? y : x;
+                    }
+    double v[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) v[i] = i < m ? g.val[key[i] & 0xFFFFu] : 0.0;
     double lv[BPASS_BITS];
     unsigned pend = 0u;
-    int prev = g.lid[g.cidx[c2[0]]];
-    double cur = g.val[c2[0]];
-    for (int i = 1; i < m; ++i) {
-        const int e = c2[i];
-        const int l = g.lid[g.cidx[e]];
-        const int d = 31 - __clz(prev ^ l);
-        double v = cur;
+    int prev = (int)(key[0] >> 16);
+    double cur = v[0];
 #pragma unroll
-        for (int h = 0; h < BPASS_BITS; ++h)
-            if (h < d && ((pend >> h) & 1u)) v = fadd(lv[h], v);
+    for (int i = 1; i < M; ++i) {
+        if (i < m) {
+            const int l = (int)(key[i] >> 16);
+            const int d = 31 - __clz(prev ^ l);
+            double x = cur;
 #pragma unroll
-        for (int h = 0; h < BPASS_BITS; ++h)
-            if (h == d) lv[h] = v;
-        pend = (pend & ~((1u << d) - 1u)) | (1u << d);
-        cur = g.val[e];
-        prev = l;
+            for (int h = 0; h < BPASS_BITS; ++h)
+                if (h < d && ((pend >> h) & 1u)) x = fadd(lv[h], x);
+#pragma unroll
+            for (int h = 0; h < BPASS_BITS; ++h)
+                if (h == d) lv[h] = x;
+            pend = (pend & ~((1u << d) - 1u)) | (1u << d);
+            cur = v[i];
+            prev = l;
+        }
     }
-    double v = cur;
+    double x = cur;
 #pragma unroll
     for (int h = 0; h < BPASS_BITS; ++h)
-        if ((pend >> h) & 1u) v = fadd(lv[h], v);
-    return v;
+        if ((pend >> h) & 1u) x = fadd(lv[h], x);
+    return x;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// a (left) and b (right) of one tree node: both present are added, one
+// present passes through
+__device__ __forceinline__ void node_combine(double& a, bool& pa, double b, bool pb) {
+    if (pa && pb) a = fadd(a, b);
+    else if (pb) a = b;
+    pa = pa || pb;
+}
+
+// Sum of output bucket [s, s + m) by one wavefront: the contributions are
+// placed densely by list id (256 slots in dense/pres, private to the wave,
+// pres all zero on entry and on return), each lane reduces its 4 slots, and
+// the aligned tree over lanes is finished with shuffles.  Returns the sum in
+// every lane.
+__device__ double bg_wave_sum(const BgSmem& g, int s, int m, double* dense, unsigned char* pres) {
+    const int lane = lane_id();
+    for (int i = lane; i < m; i += BRM_WAVE) {
+        const int e = g.contrib[s + i];
+        const int l = g.elid[e];
+        dense[l] = g.val[e];
+        pres[l] = 1;
+    }
+    wave_lds_sync();
+    double v[4];
+    bool p[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        p[i] = pres[4 * lane + i] != 0;
+        v[i] = dense[4 * lane + i];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pres[4 * lane + i] = 0;
+    node_combine(v[0], p[0], v[1], p[1]);
+    node_combine(v[2], p[2], v[3], p[3]);
+    node_combine(v[0], p[0], v[2], p[2]);
+    double x = v[0];
+    bool px = p[0];
+#pragma unroll
+    for (int d = 1; d < BRM_WAVE; d <<= 1) {
+        const double y = __shfl_xor(x, d, BRM_WAVE);
+        const bool py = __shfl_xor((int)px, d, BRM_WAVE) != 0;
+        if ((lane & d) == 0) node_combine(x, px, y, py);
+    }
+    wave_lds_sync();
+    return __shfl(x, 0, BRM_WAVE);
 }
 
 __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ tasks,
@@ -295,6 +350,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         g.take[t] = 0;
     }
     if (t < 32) g.hist[t] = 0;
+    if (t == 0) g.nbig = 0;
 #pragma unroll
     for (int i = 0; i < BG_WPT; ++i) g.bm[t * BG_WPT + i] = 0ull;
     __syncthreads();
@@ -342,6 +398,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
                 g.col[e] = rc[i];
                 g.val[e] = ((vmask >> i) & 1u) ? fmul(rs[i], rv[i]) : rv[i];
                 g.cidx[e] = (unsigned char)cj[i];
+                g.elid[e] = g.lid[cj[i]];
                 if (ci[i] == g.w[cj[i]] - 1) g.lastc[cj[i]] = rc[i];
             }
         }
@@ -394,6 +451,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
                 g.col[pos] = c;
                 g.val[pos] = p.virt ? fmul(g.scale[j], v) : v;
                 g.cidx[pos] = (unsigned char)j;
+                g.elid[pos] = g.lid[j];
                 if (q == g.xamt[j] - 1) g.lastc[j] = c;
             }
             __syncthreads();
@@ -494,23 +552,54 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         }
         __syncthreads();
         mark(6);
-        // ---- one output per thread (strided: coalesced stores); bitmap cleared
-#pragma unroll
-        for (int i = 0; i < BG_WPT; ++i) g.bm[t * BG_WPT + i] = 0ull;
+        // ---- one output per thread (strided: coalesced stores); outputs
+        // with many contributions are left to whole wavefronts
         for (int oi = t; oi < nout; oi += BG_NT) {
             const int s = oi ? g.cnt[oi - 1] : 0;
             const int m = g.cnt[oi] - s;
+            if (m > BG_THREAD_M) {
+                g.big[atomicAdd(&g.nbig, 1)] = (unsigned short)oi;
+                continue;
+            }
             const int e0 = g.contrib[s];
             const int c = g.col[e0];
-            const double v = m == 1 ? g.val[e0] : bg_tree_sum(g, s, m);
+            const double v = m == 1 ? g.val[e0] : bg_small_sum(g, s, m);
             const int64_t q = out_n + oi;
             if (q < dcap) {
                 dc[q] = c;
                 dv[q] = v;
             }
         }
+        __syncthreads();
+        {
+            // the bitmap is free until the next round: wave w uses words
+            // [512 w, 512 w + 256) for dense sums and the next 32 for flags,
+            // then clears its share of the bitmap
+            const int wv = t / BRM_WAVE, lane = lane_id();
+            double* dense = reinterpret_cast<double*>(&g.bm[wv * (BG_WORDS / BG_NW)]);
+            unsigned char* pres = reinterpret_cast<unsigned char*>(&g.bm[wv * (BG_WORDS / BG_NW) + BG_KMAX]);
+            const int nbig = g.nbig;
+            if (nbig > 0) {
+                reinterpret_cast<unsigned*>(pres)[lane] = 0u;
+                wave_lds_sync();
+                for (int b = wv; b < nbig; b += BG_NW) {
+                    const int oi = g.big[b];
+                    const int s = oi ? g.cnt[oi - 1] : 0;
+                    const double v = bg_wave_sum(g, s, g.cnt[oi] - s, dense, pres);
+                    const int64_t q = out_n + oi;
+                    if (lane == 0 && q < dcap) {
+                        dc[q] = g.col[g.contrib[s]];
+                        dv[q] = v;
+                    }
+                }
+            }
+            static_assert(BG_WPT * BRM_WAVE == BG_WORDS / BG_NW, "a wave clears its own bitmap words");
+#pragma unroll
+            for (int i = 0; i < BG_WPT; ++i) g.bm[t * BG_WPT + i] = 0ull;
+        }
         out_n += nout;
         __syncthreads();
+        if (t == 0) g.nbig = 0;  // read by every wave before the barrier above
         mark(7);
     }
 

@@ -157,8 +157,8 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
 /* Tuning knobs (names: "ws_budget_bytes", "med_cap", "leaf_cap",
  * "force_long", "profile", "lds_cfg", "group_cfg", "wmed_cap"); returns
  * BRM_E_INVALID for unknown names or values.  Knobs never change results,
- * only speed.  group_cfg 0-3: merge-path engine shapes for long rows; 4: the
- * bitmap-rank engine. */
+ * only speed.  group_cfg 0-3: merge-path engine shapes for long rows; 4
+ * (default): the bitmap-rank engine. */
 int brm_set_param(const char *name, int64_t value);
 int64_t brm_get_param(const char *name);
 

@@ -16,14 +16,16 @@ pytestmark = pytest.mark.gpu
 
 TIER_SETTINGS = {
     "default": {},
-    "all_long": {"force_long": 1},
-    "tiny_leaves": {"force_long": 1, "leaf_cap": 8},
+    "all_long": {"force_long": 1, "group_cfg": 0},
+    "tiny_leaves": {"force_long": 1, "leaf_cap": 8, "group_cfg": 0},
+    "mergepath": {"group_cfg": 0},
     "small_medium": {"med_cap": 64, "leaf_cap": 100, "wmed_cap": 0},
     "wave_medium": {"wmed_cap": 1024},
     "no_wave_medium": {"wmed_cap": 0},
     "bitmap_all_long": {"force_long": 1, "group_cfg": 4},
     "bitmap": {"group_cfg": 4},
     "tiny_launches": {"launch_chunk": 3, "group_cfg": 4},
+    "tiny_launches_mergepath": {"launch_chunk": 3, "group_cfg": 0},
     "register_medium_128": {"wmed_cap": 128, "wmed_engine": 1},
     "register_medium_512": {"wmed_cap": 512, "wmed_engine": 1, "med_cap": 512},
     "register_medium_1024": {"wmed_cap": 1024, "wmed_engine": 1},
