@@ -57,18 +57,20 @@ def kernel_bytes(name, A, L, ip, nnz_row, tier, st):
     gather per product (col 4 + val 8 = 12 B; symbolic kernels read columns
     only, 4 B), C written once (12 B per nnz).  Upper-level merges read and
     write 12 B per element (k_group: each input element once, each output
-    once).  Ping-pong traffic inside LDS is not counted."""
+    once; k_bgroup likewise, although elements loaded into a round's window
+    beyond the round's bound are read again in the next round).  Ping-pong
+    traffic inside LDS is not counted."""
     sel = {"k_short_numeric": 1, "k_short_count": 1, "k_medium": 2, "k_sym_hash": 2, "k_sym_bitmap": 3,
-           "k_medium_w": 4, "k_sym_hash_w": 4}
+           "k_medium_w": 4, "k_rmed": 4, "k_sym_hash_w": 4}
     if name in sel:
         m = tier == sel[name]
         nA, nip, nc = int(L[m].sum()), int(ip[m].sum()), int(nnz_row[m].sum())
-        if name in ("k_short_numeric", "k_medium", "k_medium_w"):
+        if name in ("k_short_numeric", "k_medium", "k_medium_w", "k_rmed"):
             return 20 * nA + 12 * nip + 12 * nc + 16 * int(m.sum())
         return 12 * nA + 4 * nip + 8 * int(m.sum())
     if name == "k_leaf":
         return 12 * st["leaf_ip"] + 12 * st["leaf_out"] + 20 * st["leaf_tasks"]
-    if name == "k_group":
+    if name in ("k_group", "k_bgroup"):
         return 12 * st["group_in"] + 12 * st["group_out"] + 48 * st["group_tasks"]
     return None
 

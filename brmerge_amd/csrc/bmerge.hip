@@ -64,6 +64,8 @@ struct BgSmem {
     double scale[BG_KMAX];
     int w[BG_KMAX];         // elements of the child loaded this round
     int woff[BG_KMAX + 1];  // proportional windows: offsets in the buffer
+    unsigned long long wstart[BG_W / 64];  // bit woff[j] set for every child with a window
+    unsigned char wchild[BG_KMAX];         // the children with windows, in order
     int lastc[BG_KMAX];     // last loaded column of the child
     int xoff[BG_KMAX + 1];  // top-up offsets (relative to the fill level)
     int xamt[BG_KMAX];
@@ -142,6 +144,14 @@ __device__ __forceinline__ int block_scan_i(BgSmem& g, Slots& sl, int x, int* to
     return block_excl_scan_1b<BG_NT>(x, g.red[sl.next()], total);
 }
 
+// Buffer element i (0 <= i < BG_IPT) of the calling thread: wavefront w
+// owns elements [w * IPT * 64, (w + 1) * IPT * 64), and element i of every
+// lane of the wavefront is one contiguous run of 64 (conflict-free LDS
+// accesses, coalesced loads).
+__device__ __forceinline__ int bg_elem(int i) {
+    return (int)(threadIdx.x / BRM_WAVE) * (BG_IPT * BRM_WAVE) + i * BRM_WAVE + lane_id();
+}
+
 // Proportional windows for the next round: everything left if it fits,
 // else a share of BG_W - BG_KMAX proportional to what each child has left
 // (at least one element).  Returns true when nothing is left.  The windows
@@ -161,13 +171,19 @@ __device__ __forceinline__ bool bg_windows(BgSmem& g, Slots& sl, int k) {
             wi = (int)x;
         }
     }
+    // one scan gives the offsets (low 16 bits) and the rank among children
+    // with a window (high bits)
     int tot;
-    const int ex = block_scan_i(g, sl, wi, &tot);
+    const int ex = block_scan_i(g, sl, wi | ((wi > 0 ? 1 : 0) << 16), &tot);
     if (t < k) {
         g.w[t] = wi;
-        g.woff[t] = ex;
+        g.woff[t] = ex & 0xFFFF;
+        if (wi > 0) {
+            g.wchild[ex >> 16] = (unsigned char)t;
+            atomicOr(&g.wstart[(ex & 0xFFFF) >> 6], 1ull << (ex & 63));
+        }
     }
-    if (t == BG_NT - 1) g.woff[k] = tot;
+    if (t == BG_NT - 1) g.woff[k] = tot & 0xFFFF;
     return R == 0;
 }
 
@@ -183,21 +199,28 @@ __device__ __forceinline__ SrcPtr src_of(const BgSmem& g, int j, const RowArgs& 
                   virt};
 }
 
-// Loads this thread's elements of the proportional windows (buffer range
-// t*IPT ..) into registers; cj = child, ci = index in the child's window.
+// Loads this thread's elements of the proportional windows (buffer
+// positions bg_elem(i)) into registers; cj = child, ci = index in the
+// child's window.  The child of element e is the last window starting at or
+// before e: its rank is the number of wstart bits up to e, less one (the 64
+// elements of one step of a wavefront share one wstart word).
 __device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const RowArgs& a, const WsCols& ws,
                                          int (&rc)[BG_IPT], double (&rv)[BG_IPT], double (&rs)[BG_IPT],
                                          int (&cj)[BG_IPT], int (&ci)[BG_IPT], unsigned& vmask) {
-    const int e0 = threadIdx.x * BG_IPT;
+    static_assert(BG_W / 64 <= BRM_WAVE, "one wstart word per lane");
     vmask = 0u;
-    int j = 0;
-    if (e0 < Wn) j = upper_idx(g.woff, 0, k, e0);
+    const int lane = lane_id();
+    const int pc = lane < BG_W / 64 ? __popcll(g.wstart[lane]) : 0;
+    const int before = wave_incl_scan(pc) - pc;  // window starts in earlier words
+    const unsigned long long upto = lane == BRM_WAVE - 1 ? ~0ull : (2ull << lane) - 1ull;
 #pragma unroll
     for (int i = 0; i < BG_IPT; ++i) {
-        const int e = e0 + i;
+        const int e = bg_elem(i);
+        const int wd = e >> 6;
+        const int r = __shfl(before, wd, BRM_WAVE) + __popcll(g.wstart[wd] & upto) - 1;
         cj[i] = -1;
         if (e < Wn) {
-            while (j + 1 < k && g.woff[j + 1] <= e) ++j;
+            const int j = g.wchild[r];
             const int q = e - g.woff[j];
             const SrcPtr p = src_of(g, j, a, ws);
             const int64_t x = g.off[j] + g.cur[j] + q;
@@ -350,6 +373,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         g.take[t] = 0;
     }
     if (t < 32) g.hist[t] = 0;
+    if (t < BG_W / 64) g.wstart[t] = 0ull;
     if (t == 0) g.nbig = 0;
 #pragma unroll
     for (int i = 0; i < BG_WPT; ++i) g.bm[t * BG_WPT + i] = 0ull;
@@ -393,7 +417,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         // ---- the proportional windows into the buffer (products rounded here)
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i) {
-            const int e = t * BG_IPT + i;
+            const int e = bg_elem(i);
             if (e < Wn) {
                 g.col[e] = rc[i];
                 g.val[e] = ((vmask >> i) & 1u) ? fmul(rs[i], rv[i]) : rv[i];
@@ -403,6 +427,9 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             }
         }
         __syncthreads();
+        // the window starts were read by the fetch of this round's windows,
+        // before the barrier above; the next windows are set in phase 3
+        if (t < BG_W / 64) g.wstart[t] = 0ull;
         const int64_t rem = t < k ? g.len[t] - g.cur[t] : 0;
         const int c_lo = block_min_i(g, sl, (t < k && g.w[t] > 0) ? g.col[g.woff[t]] : INT_MAX);
         int filled = Wn;
@@ -467,7 +494,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         unsigned taken = 0u;
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i) {
-            const int e = t * BG_IPT + i;
+            const int e = bg_elem(i);
             if (e < filled) {
                 const int c = g.col[e];
                 if (c <= bound) {
@@ -512,7 +539,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         for (int i = 0; i < BG_IPT; ++i) {
             pos[i] = 0;
             if ((taken >> i) & 1u) {
-                const int e = t * BG_IPT + i;
+                const int e = bg_elem(i);
                 const int bit = g.col[e] - c_lo;
                 const int wd = bit >> 6;
                 pos[i] = g.wrank[wd] + __popcll(g.bm[wd] & ((1ull << (bit & 63)) - 1ull));
@@ -547,7 +574,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         for (int i = 0; i < BG_IPT; ++i) {
             if ((taken >> i) & 1u) {
                 const int slot = atomicAdd(&g.cnt[pos[i]], 1);
-                g.contrib[slot] = (unsigned short)(t * BG_IPT + i);
+                g.contrib[slot] = (unsigned short)bg_elem(i);
             }
         }
         __syncthreads();
