@@ -46,6 +46,7 @@ constexpr int BG_TOPUPS = 2;   // top-up iterations per round
 // (dense over the 256 list ids) instead of by one thread.
 constexpr int BG_THREAD_M = 8;
 constexpr int BG_BIGMAX = BG_W / (BG_THREAD_M + 1) + 1;
+constexpr int BG_MIDMAX = BG_W / 3 + 1;  // outputs with 3 .. BG_THREAD_M contributions
 static_assert(BG_KMAX <= BG_NT, "one child per thread in the window logic");
 static_assert(BG_W <= 65535, "u16 ranks and contribution indices");
 
@@ -75,7 +76,9 @@ struct BgSmem {
     unsigned char buf[BG_KMAX];
     unsigned char lid[BG_KMAX];
     unsigned short big[BG_BIGMAX];  // outputs left to the wavefront sums
+    unsigned short mid[BG_MIDMAX];  // outputs left to the second thread pass
     int nbig;
+    int nmid;
     int64_t red64[BG_NSLOT][BG_NW];
     int red[BG_NSLOT][BG_NW];
 };
@@ -218,7 +221,12 @@ __device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const R
         const int e = bg_elem(i);
         const int wd = e >> 6;
         const int r = __shfl(before, wd, BRM_WAVE) + __popcll(g.wstart[wd] & upto) - 1;
+        // every register is written, so nothing of the previous round stays live
         cj[i] = -1;
+        ci[i] = 0;
+        rc[i] = 0;
+        rv[i] = 0.0;
+        rs[i] = 0.0;
         if (e < Wn) {
             const int j = g.wchild[r];
             const int q = e - g.woff[j];
@@ -374,7 +382,10 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
     }
     if (t < 32) g.hist[t] = 0;
     if (t < BG_W / 64) g.wstart[t] = 0ull;
-    if (t == 0) g.nbig = 0;
+    if (t == 0) {
+        g.nbig = 0;
+        g.nmid = 0;
+    }
 #pragma unroll
     for (int i = 0; i < BG_WPT; ++i) g.bm[t * BG_WPT + i] = 0ull;
     __syncthreads();
@@ -414,6 +425,10 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
     };
     while (!done) {
         rounds++;
+        // every load of the prefetched windows is complete here (vmcnt 0):
+        // stated explicitly so that no load is left in flight into the next
+        // prefetch, whose destination registers are reused as temporaries
+        __builtin_amdgcn_s_waitcnt(0x0F70);
         // ---- the proportional windows into the buffer (products rounded here)
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i) {
@@ -467,19 +482,40 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             }
             if (t == BG_NT - 1) g.xoff[k] = tot;
             __syncthreads();
-            for (int e = t; e < tot; e += BG_NT) {
-                const int j = upper_idx(g.xoff, 0, k, e);
-                const int q = e - g.xoff[j];
-                const SrcPtr p = src_of(g, j, a, ws);
-                const int64_t x = g.off[j] + g.cur[j] + g.w[j] + q;
-                const int c = p.c[x];
-                const double v = p.v[x];
-                const int pos = filled + e;
-                g.col[pos] = c;
-                g.val[pos] = p.virt ? fmul(g.scale[j], v) : v;
-                g.cidx[pos] = (unsigned char)j;
-                g.elid[pos] = g.lid[j];
-                if (q == g.xamt[j] - 1) g.lastc[j] = c;
+            // tot <= BG_W - filled: at most BG_IPT top-up elements per
+            // thread; all loads are issued before the first is used
+            int tc[BG_IPT], tj[BG_IPT], tq[BG_IPT];
+            double tv[BG_IPT];
+#pragma unroll
+            for (int i = 0; i < BG_IPT; ++i) {
+                const int e = t + i * BG_NT;
+                tc[i] = 0;
+                tj[i] = 0;
+                tq[i] = 0;
+                tv[i] = 0.0;
+                if (e < tot) {
+                    const int j = upper_idx(g.xoff, 0, k, e);
+                    const int q = e - g.xoff[j];
+                    const SrcPtr p = src_of(g, j, a, ws);
+                    const int64_t x = g.off[j] + g.cur[j] + g.w[j] + q;
+                    tc[i] = p.c[x];
+                    tv[i] = p.v[x];
+                    tj[i] = j;
+                    tq[i] = q;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < BG_IPT; ++i) {
+                const int e = t + i * BG_NT;
+                if (e < tot) {
+                    const int j = tj[i];
+                    const int pos = filled + e;
+                    g.col[pos] = tc[i];
+                    g.val[pos] = g.kind[j] == SRC_VIRTUAL ? fmul(g.scale[j], tv[i]) : tv[i];
+                    g.cidx[pos] = (unsigned char)j;
+                    g.elid[pos] = g.lid[j];
+                    if (tq[i] == g.xamt[j] - 1) g.lastc[j] = tc[i];
+                }
             }
             __syncthreads();
             if (t < k) g.w[t] += amt;
@@ -579,25 +615,45 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         }
         __syncthreads();
         mark(6);
-        // ---- one output per thread (strided: coalesced stores); outputs
-        // with many contributions are left to whole wavefronts
+        // ---- one output per thread (strided: coalesced stores) for one
+        // or two contributions; outputs with more are queued, so that the
+        // lanes of a wavefront take the same path: up to BG_THREAD_M
+        // contributions for a second thread pass, more for whole wavefronts
         for (int oi = t; oi < nout; oi += BG_NT) {
             const int s = oi ? g.cnt[oi - 1] : 0;
             const int m = g.cnt[oi] - s;
-            if (m > BG_THREAD_M) {
-                g.big[atomicAdd(&g.nbig, 1)] = (unsigned short)oi;
+            if (m > 2) {
+                if (m > BG_THREAD_M) g.big[atomicAdd(&g.nbig, 1)] = (unsigned short)oi;
+                else g.mid[atomicAdd(&g.nmid, 1)] = (unsigned short)oi;
                 continue;
             }
-            const int e0 = g.contrib[s];
-            const int c = g.col[e0];
-            const double v = m == 1 ? g.val[e0] : bg_small_sum(g, s, m);
+            int e0 = g.contrib[s];
+            double v = g.val[e0];
+            if (m == 2) {
+                const int e1 = g.contrib[s + 1];
+                const double v1 = g.val[e1];
+                v = g.elid[e0] < g.elid[e1] ? fadd(v, v1) : fadd(v1, v);
+            }
             const int64_t q = out_n + oi;
             if (q < dcap) {
-                dc[q] = c;
+                dc[q] = g.col[e0];
                 dv[q] = v;
             }
         }
         __syncthreads();
+        {
+            const int nmid = g.nmid;
+            for (int b = t; b < nmid; b += BG_NT) {
+                const int oi = g.mid[b];
+                const int s = oi ? g.cnt[oi - 1] : 0;
+                const double v = bg_small_sum(g, s, g.cnt[oi] - s);
+                const int64_t q = out_n + oi;
+                if (q < dcap) {
+                    dc[q] = g.col[g.contrib[s]];
+                    dv[q] = v;
+                }
+            }
+        }
         {
             // the bitmap is free until the next round: wave w uses words
             // [512 w, 512 w + 256) for dense sums and the next 32 for flags,
@@ -626,7 +682,10 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         }
         out_n += nout;
         __syncthreads();
-        if (t == 0) g.nbig = 0;  // read by every wave before the barrier above
+        if (t == 0) {  // read by every wave before the barrier above
+            g.nbig = 0;
+            g.nmid = 0;
+        }
         mark(7);
     }
 
