@@ -423,6 +423,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             tprev = now;
         }
     };
+    int64_t R = in_total;  // elements not yet taken
     while (!done) {
         rounds++;
         // every load of the prefetched windows is complete here (vmcnt 0):
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         }
         __syncthreads();
         // the window starts were read by the fetch of this round's windows,
-        // before the barrier above; the next windows are set in phase 3
+        // before the barrier above; the next windows set them in phase 5
         if (t < BG_W / 64) g.wstart[t] = 0ull;
         const int64_t rem = t < k ? g.len[t] - g.cur[t] : 0;
         const int c_lo = block_min_i(g, sl, (t < k && g.w[t] > 0) ? g.col[g.woff[t]] : INT_MAX);
@@ -543,14 +544,14 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             g.cnt[e] = 0;
         }
         __syncthreads();
+        int64_t remt = 0;
         if (t < k) {
             g.cur[t] += g.take[t];
             g.take[t] = 0;
+            remt = g.len[t] - g.cur[t];
         }
         mark(2);
-        // ---- next windows (published by the scan barrier below)
-        done = bg_windows(g, sl, k);
-        // ---- ranks of the bitmap words
+        // ---- ranks of the bitmap words; with them the number taken
         int pc[BG_WPT];
         int sum = 0;
 #pragma unroll
@@ -559,14 +560,19 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             sum += pc[i];
         }
         int nout;
-        int wex = block_scan_i(g, sl, sum, &nout);
-        Wn = g.woff[k];
-        if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, ci, vmask);
+        {
+            int tot;
+            const int ex = block_scan_i(g, sl, sum | (__popc(taken) << 16), &tot);
+            nout = tot & 0xFFFF;
+            R -= tot >> 16;
+            int wex = ex & 0xFFFF;
 #pragma unroll
-        for (int i = 0; i < BG_WPT; ++i) {
-            g.wrank[t * BG_WPT + i] = (unsigned short)wex;
-            wex += pc[i];
+            for (int i = 0; i < BG_WPT; ++i) {
+                g.wrank[t * BG_WPT + i] = (unsigned short)wex;
+                wex += pc[i];
+            }
         }
+        done = R == 0;
         __syncthreads();
         mark(3);
         // ---- output position of every taken element; contributions per position
@@ -584,28 +590,57 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         }
         __syncthreads();
         mark(4);
-        // ---- bucket offsets (exclusive scan of the counts)
+        // ---- bucket offsets (exclusive scan of the counts) and, in the
+        // same scan, the next round's proportional windows: everything left
+        // if it fits, else a share of BG_W - BG_KMAX proportional to what
+        // each child has left (at least one element)
         {
+            int wi = 0;
+            if (remt > 0) {
+                if (R <= BG_W) {
+                    wi = (int)remt;
+                } else {
+                    int64_t x = (int64_t)((double)(BG_W - BG_KMAX) * (double)remt / (double)R);
+                    if (x < 1) x = 1;
+                    if (x > remt) x = remt;
+                    wi = (int)x;
+                }
+            }
             int loc[BG_IPT];
-            int s = 0;
+            int sloc = 0;
 #pragma unroll
             for (int i = 0; i < BG_IPT; ++i) {
                 const int oi = t * BG_IPT + i;
                 loc[i] = oi < nout ? g.cnt[oi] : 0;
-                s += loc[i];
+                sloc += loc[i];
             }
-            int tot;
-            int ex = block_scan_i(g, sl, s, &tot);
+            // bits 0-15: bucket offset, 16-31: window offset, 32-: window rank
+            const int64_t packed = (int64_t)sloc | ((int64_t)wi << 16) | ((int64_t)(wi > 0 ? 1 : 0) << 32);
+            int64_t tot;
+            const int64_t ex = block_excl_scan_1b<BG_NT>(packed, g.red64[sl.next()], &tot);
+            int bex = (int)(ex & 0xFFFF);
 #pragma unroll
             for (int i = 0; i < BG_IPT; ++i) {
                 const int oi = t * BG_IPT + i;
-                if (oi < nout) g.cnt[oi] = ex;
-                ex += loc[i];
+                if (oi < nout) g.cnt[oi] = bex;
+                bex += loc[i];
             }
+            const int wo = (int)((ex >> 16) & 0xFFFF);
+            if (t < k) {
+                g.w[t] = wi;
+                g.woff[t] = wo;
+                if (wi > 0) {
+                    g.wchild[(int)(ex >> 32)] = (unsigned char)t;
+                    atomicOr(&g.wstart[wo >> 6], 1ull << (wo & 63));
+                }
+            }
+            Wn = (int)((tot >> 16) & 0xFFFF);
+            if (t == BG_NT - 1) g.woff[k] = Wn;
         }
         __syncthreads();
         mark(5);
-        // ---- bucket the contributions (cnt[p] ends as the bucket's end)
+        // ---- bucket the contributions (cnt[p] ends as the bucket's end);
+        // the next round's windows are fetched into registers
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i) {
             if ((taken >> i) & 1u) {
@@ -613,6 +648,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
                 g.contrib[slot] = (unsigned short)bg_elem(i);
             }
         }
+        if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, ci, vmask);
         __syncthreads();
         mark(6);
         // ---- one output per thread (strided: coalesced stores) for one
