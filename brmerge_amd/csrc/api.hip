@@ -122,6 +122,10 @@ int d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
 int check_dev_err(const int* derr, hipStream_t s, const char* stage) {
     int e = 0;
     BRMC(d2h(&e, derr, sizeof(int), s));
+    return dev_err_status(e, stage);
+}
+
+int dev_err_status(int e, const char* stage) {
     if (e & DERR_INVALID) {
         set_err("%s: malformed CSR input (row pointers, column range or unsorted/duplicate B columns)", stage);
         return BRM_E_INVALID;
@@ -355,6 +359,20 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
 }  // namespace
 
 // ---------------------------------------------------------------- pipeline
+namespace {
+// Row bins (device) under the given tier parameters; the sizes stay on the device.
+hipError_t bin_launch(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int force_long, hipStream_t s) {
+    const LdsShape shape = lds_shape(lds_cfg);
+    p.lds_cfg = lds_cfg;
+    p.med_cap = std::min(med_cap, shape.cap);
+    p.wmed_cap = std::min(wmed_cap, WMED_CAP);
+    hipError_t e = launch_fill_i32(p.bin_cnt, BIN_COUNT, 0, s);
+    if (e != hipSuccess) return e;
+    return launch_rowstats(p.a, p.med_cap, shape.lmax, p.wmed_cap, force_long, p.iprow, p.rowcnt, p.bins,
+                           p.bin_cnt, p.derr, s);
+}
+}  // namespace
+
 int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p) {
     if (in.M < 0 || in.K < 0 || in.N < 0 || in.nnzA < 0 || in.nnzB < 0) {
         set_err("negative dimension");
@@ -390,33 +408,57 @@ int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p)
     a.N = in.N;
     HIPC(launch_validate(a, in.nnzA, in.b_checked ? -1 : in.nnzB, p.derr, s));
     if (!in.b_checked) HIPC(launch_validate_sorted(in.brpt, in.bcol, in.K, p.derr, s));
-    BRMC(check_dev_err(p.derr, s, "validation"));
 
+    // The product scan reads no B row pointer outside the array and the
+    // row binning skips every row once validation has flagged the input, so
+    // one host round trip after binning serves validation, the product
+    // total and the bin sizes.
     if (clk) clk->begin(P_FLOP);
-    HIPC(scan_alen(in.acol, in.brpt, in.nnzA, p.ipA, s));
-    BRMC(d2h(&p.total_ip, p.ipA + in.nnzA, sizeof(int64_t), s));
+    HIPC(scan_alen(in.acol, in.brpt, in.K, in.nnzA, p.ipA, s));
 
     if (clk) clk->begin(P_LB);
     const LdsShape shape = lds_shape(params().lds_cfg);
-    return pattern_bin(p, params().lds_cfg, std::min(params().med_cap, shape.cap), params().wmed_cap,
-                       params().force_long, s);
+    HIPC(bin_launch(p, params().lds_cfg, std::min(params().med_cap, shape.cap), params().wmed_cap,
+                    params().force_long, s));
+    int e = 0;
+    HIPC(hipMemcpyAsync(&e, p.derr, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(&p.total_ip, p.ipA + in.nnzA, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(p.bc, p.bin_cnt, sizeof(p.bc), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    return dev_err_status(e, "validation");
 }
 
 int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int force_long, hipStream_t s) {
-    const LdsShape shape = lds_shape(lds_cfg);
-    p.lds_cfg = lds_cfg;
-    p.med_cap = std::min(med_cap, shape.cap);
-    p.wmed_cap = std::min(wmed_cap, WMED_CAP);
-    HIPC(launch_fill_i32(p.bin_cnt, BIN_COUNT, 0, s));
-    HIPC(launch_rowstats(p.a, p.med_cap, shape.lmax, p.wmed_cap, force_long, p.iprow, p.rowcnt, p.bins, p.bin_cnt,
-                         s));
+    HIPC(bin_launch(p, lds_cfg, med_cap, wmed_cap, force_long, s));
     BRMC(d2h(p.bc, p.bin_cnt, sizeof(p.bc), s));
     return BRM_OK;
 }
 
-int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk) {
+int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_short) {
     if (clk) clk->begin(P_SYM);
-    HIPC(launch_short_count(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, p.rowcnt, s));
+    const int64_t n32 = (int64_t)p.bc[BIN_TINY] * 32, n64 = (int64_t)p.bc[BIN_SHORT] * 64;
+    p.stashed = stash_short && n32 + n64 > 0 && 12 * (n32 + n64) <= SHORT_STASH_MAX_BYTES;
+    if (p.stashed) {
+        BRMC(p.sc.alloc(&p.stash_col[0], n32, "short-row stash"));
+        BRMC(p.sc.alloc(&p.stash_val[0], n32, "short-row stash"));
+        BRMC(p.sc.alloc(&p.stash_col[1], n64, "short-row stash"));
+        BRMC(p.sc.alloc(&p.stash_val[1], n64, "short-row stash"));
+        OutArgs o{};
+        o.err = p.derr;
+        o.upper = 1;
+        o.rowlen = p.rowcnt;
+        o.slot = 32;
+        o.ccol = p.stash_col[0];
+        o.cval = p.stash_val[0];
+        HIPC(launch_short_numeric(p.rows(BIN_TINY), p.bc[BIN_TINY], p.a, o, s, 32));
+        o.slot = 64;
+        o.ccol = p.stash_col[1];
+        o.cval = p.stash_val[1];
+        HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
+    } else {
+        HIPC(launch_short_count(p.rows(BIN_TINY), p.bc[BIN_TINY], p.a, p.rowcnt, s, 32));
+        HIPC(launch_short_count(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, p.rowcnt, s));
+    }
     HIPC(launch_wmed_count(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, p.rowcnt, s, p.wmed_cap));
     HIPC(launch_medium_count(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, p.rowcnt, s));
     HIPC(launch_long_count(p.rows(BIN_LONG), p.bc[BIN_LONG], p.a, p.rowcnt, s));
@@ -440,7 +482,15 @@ OutArgs precise_out(const Pattern& p, int* ccol, double* cval) {
 
 int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t s) {
     const OutArgs o = precise_out(p, ccol, cval);
-    HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
+    if (p.stashed) {
+        HIPC(launch_short_copy(p.rows(BIN_TINY), p.bc[BIN_TINY], p.stash_col[0], p.stash_val[0], p.crpt, ccol, cval,
+                               s, 32));
+        HIPC(launch_short_copy(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.stash_col[1], p.stash_val[1], p.crpt, ccol,
+                               cval, s, 64));
+    } else {
+        HIPC(launch_short_numeric(p.rows(BIN_TINY), p.bc[BIN_TINY], p.a, o, s, 32));
+        HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
+    }
     HIPC(launch_wmed_numeric(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, o, s, p.wmed_cap));
     HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
     return BRM_OK;
@@ -463,6 +513,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     const int M = in.M;
     for (int b = 0; b < 4; ++b) t_stats.rows_by_tier[b] = p.bc[b];
     t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_WMED];
+    t_stats.rows_by_tier[BIN_SHORT] += p.bc[BIN_TINY];
     t_stats.total_ip = p.total_ip;
     Scratch& sc = p.sc;
     int* ccol = nullptr;
@@ -470,7 +521,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     int64_t nnzC = 0;
 
     if (!upper) {
-        BRMC(pattern_symbolic(p, s, &clk));
+        BRMC(pattern_symbolic(p, s, &clk, true));
         nnzC = p.nnzC;
         if (nnzC > max_nnz) {
             set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
@@ -500,6 +551,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         o.ccol = bcol_bar;
         o.cval = bval_bar;
         o.rowlen = p.rowcnt;
+        HIPC(launch_short_numeric(p.rows(BIN_TINY), p.bc[BIN_TINY], p.a, o, s, 32));
         HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
         HIPC(launch_wmed_numeric(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, o, s, p.wmed_cap));
         HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
@@ -825,7 +877,7 @@ int brm_flop_count_dev(const int* arpt, const int* acol, int64_t nnzA, const int
     Scratch sc;
     int64_t* ipA;
     BRMC(sc.alloc(&ipA, nnzA + 1, "ipA"));
-    HIPC(scan_alen(acol, brpt, nnzA, ipA, s));
+    HIPC(scan_alen(acol, brpt, K, nnzA, ipA, s));
     if (row_ip && M > 0) {
         RowArgs a{};
         a.arpt = arpt;

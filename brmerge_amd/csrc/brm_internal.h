@@ -16,10 +16,12 @@ enum RowBin : int {
     BIN_MEDIUM = 2,  // ip <= MED_CAP and L <= MED_LMAX: one workgroup per row, merge in LDS
     BIN_LONG = 3,    // otherwise: LDS leaf subtrees + HBM-streamed upper tree levels
     BIN_WMED = 4,    // ip <= wmed_cap and L <= WMED_LMAX: one wavefront per row, merge in LDS
-    BIN_COUNT = 5
+    BIN_TINY = 5,    // ip <= TINY_MAX: two rows per wavefront, merge in registers
+    BIN_COUNT = 6
 };
 
 constexpr int SHORT_MAX = 64;
+constexpr int TINY_MAX = 32;
 
 // Wavefront-sized LDS merges for small medium rows (several rows resident
 // per CU instead of one 256-thread workgroup per row).
@@ -197,7 +199,8 @@ struct ProfScope {
 
 // ------------------------------------------------------------------ launchers
 // Exclusive scans: out[0..n] with out[n] = total.
-hipError_t scan_alen(const int* acol, const int* brpt, int64_t n, int64_t* out, hipStream_t s);
+// Product count of each A entry (|B row acol[j]|, 0 for a column outside [0, K)).
+hipError_t scan_alen(const int* acol, const int* brpt, int K, int64_t n, int64_t* out, hipStream_t s);
 hipError_t scan_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s);
 hipError_t scan_i32(const int* in, int64_t n, int64_t* out, hipStream_t s);
 

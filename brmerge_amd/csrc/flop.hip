@@ -39,49 +39,80 @@ __global__ void k_validate_range(const int* __restrict__ col, int64_t nnz, int l
 __global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __restrict__ col, int n, int* err) {
     const int r = blockIdx.x * (NT / BRM_WAVE) + threadIdx.x / BRM_WAVE;
     if (r >= n) return;
+    if (*err) return;  // row pointers already flagged (checked by the launches before): not followed
     const int b = rpt[r], e = rpt[r + 1];
     bool bad = false;
     for (int k = b + lane_id(); k + 1 < e; k += BRM_WAVE) bad |= col[k] >= col[k + 1];
     if (__ballot(bad) != 0ull && lane_id() == 0) atomicOr(err, (int)DERR_INVALID);
 }
 
+// Row tiers.  A block takes RS_IT * NT rows; bin slots are counted in LDS and
+// claimed with one global atomic per bin and block.
+constexpr int RS_IT = 4;
+
 __global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int med_lmax, int wmed_cap, int force_long,
                                                  int64_t* __restrict__ iprow,
                                                  int64_t* __restrict__ rowcnt, int* __restrict__ bins,
-                                                 int* __restrict__ bin_cnt) {
-    const int i = blockIdx.x * NT + threadIdx.x;
-    int bin = -1;
-    if (i < a.M) {
-        const int a0 = a.arpt[i], a1 = a.arpt[i + 1];
-        const int L = a1 - a0;
-        const int64_t ip = a.ipA[a1] - a.ipA[a0];
-        iprow[i] = ip;
-        if (ip == 0) {
-            bin = BIN_EMPTY;
-            rowcnt[i] = 0;
-        } else if (force_long) {
-            bin = BIN_LONG;
-        } else if (ip <= SHORT_MAX) {
-            bin = BIN_SHORT;
-        } else if (ip <= wmed_cap && L <= WMED_LMAX) {
-            bin = BIN_WMED;
-        } else if (ip <= med_cap && L <= med_lmax) {
-            bin = BIN_MEDIUM;
-        } else {
-            bin = BIN_LONG;
-        }
-    }
+                                                 int* __restrict__ bin_cnt, const int* __restrict__ err) {
+    __shared__ int lcnt[BIN_COUNT];
+    __shared__ int gbase[BIN_COUNT];
+    const int t = threadIdx.x;
+    if (t < BIN_COUNT) lcnt[t] = 0;
+    __syncthreads();
+    const bool bad = err && *err;  // malformed input: its row pointers may index anywhere
     const int lane = lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
+    int bin[RS_IT], lpos[RS_IT];
 #pragma unroll
-    for (int b = 0; b < BIN_COUNT; ++b) {
-        const uint64_t m = __ballot(bin == b);
-        if (m == 0ull) continue;
-        const int leader = __ffsll((long long)m) - 1;
-        int pos = 0;
-        if (lane == leader) pos = atomicAdd(&bin_cnt[b], __popcll(m));
-        pos = __shfl(pos, leader, BRM_WAVE);
-        if (b > 0 && bin == b) bins[(int64_t)(b - 1) * a.M + pos + __popcll(m & below)] = i;
+    for (int it = 0; it < RS_IT; ++it) {
+        const int64_t i = ((int64_t)blockIdx.x * RS_IT + it) * NT + t;
+        int b_ = -1;
+        if (i < a.M && bad) {
+            b_ = BIN_EMPTY;
+            iprow[i] = 0;
+            rowcnt[i] = 0;
+        } else if (i < a.M) {
+            const int a0 = a.arpt[i], a1 = a.arpt[i + 1];
+            const int L = a1 - a0;
+            const int64_t ip = a.ipA[a1] - a.ipA[a0];
+            iprow[i] = ip;
+            if (ip == 0) {
+                b_ = BIN_EMPTY;
+                rowcnt[i] = 0;
+            } else if (force_long) {
+                b_ = BIN_LONG;
+            } else if (ip <= TINY_MAX) {
+                b_ = BIN_TINY;
+            } else if (ip <= SHORT_MAX) {
+                b_ = BIN_SHORT;
+            } else if (ip <= wmed_cap && L <= WMED_LMAX) {
+                b_ = BIN_WMED;
+            } else if (ip <= med_cap && L <= med_lmax) {
+                b_ = BIN_MEDIUM;
+            } else {
+                b_ = BIN_LONG;
+            }
+        }
+        bin[it] = b_;
+        lpos[it] = 0;
+#pragma unroll
+        for (int b = 0; b < BIN_COUNT; ++b) {
+            const uint64_t m = __ballot(b_ == b);
+            if (m == 0ull) continue;
+            const int leader = __ffsll((long long)m) - 1;
+            int pos = 0;
+            if (lane == leader) pos = atomicAdd(&lcnt[b], __popcll(m));
+            pos = __shfl(pos, leader, BRM_WAVE);
+            if (b_ == b) lpos[it] = pos + __popcll(m & below);
+        }
+    }
+    __syncthreads();
+    if (t < BIN_COUNT) gbase[t] = lcnt[t] ? atomicAdd(&bin_cnt[t], lcnt[t]) : 0;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < RS_IT; ++it) {
+        const int b = bin[it];
+        if (b > 0) bins[(int64_t)(b - 1) * a.M + gbase[b] + lpos[it]] = (int)(((int64_t)blockIdx.x * RS_IT + it) * NT + t);
     }
 }
 
@@ -134,12 +165,12 @@ hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* er
 }
 
 hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int wmed_cap, int force_long, int64_t* iprow,
-                           int64_t* rowcnt, int* bins, int* bin_cnt, hipStream_t s) {
+                           int64_t* rowcnt, int* bins, int* bin_cnt, const int* err, hipStream_t s) {
     if (a.M <= 0) return hipSuccess;
     {
         ProfScope ps_("k_rowstats", s);
-        k_rowstats<<<nblk(a.M, NT), NT, 0, s>>>(a, med_cap, med_lmax, wmed_cap, force_long, iprow, rowcnt, bins,
-                                                bin_cnt);
+        k_rowstats<<<nblk(a.M, NT * RS_IT), NT, 0, s>>>(a, med_cap, med_lmax, wmed_cap, force_long, iprow, rowcnt,
+                                                        bins, bin_cnt, err);
     }
     return hipGetLastError();
 }

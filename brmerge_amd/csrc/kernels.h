@@ -25,18 +25,26 @@ struct OutArgs {
     double* cval;
     int64_t* rowlen;      // Upper
     int* err;             // DevErr bits
+    int slot;             // Upper, short tiers: row k of the row list writes at k * slot (0: at ipA)
 };
 
 // flop.hip
 hipError_t launch_validate(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* err, hipStream_t s);
 hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* err, hipStream_t s);
 // bins: [BIN_COUNT-1][M] row lists (bin b at (b-1)*M); bin_cnt: [BIN_COUNT]
+// Rows are all treated as empty when *err is set (err may be null).
 hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int wmed_cap, int force_long, int64_t* iprow,
-                           int64_t* nnzrow, int* bins, int* bin_cnt, hipStream_t s);
+                           int64_t* nnzrow, int* bins, int* bin_cnt, const int* err, hipStream_t s);
 
 // short.hip
-hipError_t launch_short_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
-hipError_t launch_short_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s);
+// group: lanes per row, 64 (ip <= 64) or 32 (ip <= 32, two rows per wavefront)
+hipError_t launch_short_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s,
+                              int group = 64);
+hipError_t launch_short_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
+                                int group = 64);
+// C rows of short rows from their slots (row k of the list at k * group)
+hipError_t launch_short_copy(const int* rows, int n, const int* scol, const double* sval, const int64_t* crpt,
+                             int* ccol, double* cval, hipStream_t s, int group);
 
 // lds.hip
 hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);

@@ -15,6 +15,7 @@ int hip_status(hipError_t e, const char* what);
 int use_device();
 int d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
 int check_dev_err(const int* derr, hipStream_t s, const char* stage);
+int dev_err_status(int err_flags, const char* stage);
 double now_s();
 brm_stats& thread_stats();
 
@@ -118,8 +119,16 @@ struct Pattern {
     int lds_cfg = 0;
     int med_cap = MED_CAP;
     int wmed_cap = 0;
+    // Short rows merged once, in the symbolic phase (pattern_symbolic with
+    // stash_short): row k of the tiny / short list at k * 32 / k * 64 here
+    bool stashed = false;
+    int* stash_col[2] = {};
+    double* stash_val[2] = {};
     const int* rows(int bin) const { return bins + (int64_t)(bin - 1) * a.M; }
 };
+
+// Largest stash of merged short rows (12 bytes per slot).
+constexpr int64_t SHORT_STASH_MAX_BYTES = (int64_t)8 << 30;
 
 // Validation, product offsets (flop scan) and row bins under the current
 // tier parameters.
@@ -128,7 +137,11 @@ int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p)
 // wavefront-tier capacity `wmed_cap`.
 int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int force_long, hipStream_t s);
 // Exact nnz per row, C's row pointer and nnz(C) (BRMerge-Precise symbolic).
-int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk);
+// With stash_short, rows of at most 64 products are merged here already
+// (their lengths are the counts) when the stash fits SHORT_STASH_MAX_BYTES;
+// the numeric phase then copies them.  A Pattern whose values change
+// between numeric passes must not stash.
+int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_short = false);
 // Numeric phase of BRMerge-Precise into C (ccol/cval of nnz(C) elements).
 // The short and medium tiers only enqueue kernels (capturable in a graph);
 // the long tier needs host round trips between its launches.

@@ -19,11 +19,15 @@ constexpr int SCAN_NT = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int64_t SCAN_TILE = SCAN_NT * SCAN_ITEMS;
 
+// A column outside [0, K) counts 0 products (validation reports it), so no
+// B row pointer outside the array is read.
 struct AlenF {
     const int* acol;
     const int* brpt;
+    int K;
     __device__ int64_t operator()(int64_t j) const {
-        int c = acol[j];
+        const int c = acol[j];
+        if ((unsigned)c >= (unsigned)K) return 0;
         return (int64_t)(brpt[c + 1] - brpt[c]);
     }
 };
@@ -116,8 +120,8 @@ hipError_t scan_impl(F f, int64_t n, int64_t* out, hipStream_t s) {
 
 }  // namespace
 
-hipError_t scan_alen(const int* acol, const int* brpt, int64_t n, int64_t* out, hipStream_t s) {
-    return scan_impl(AlenF{acol, brpt}, n, out, s);
+hipError_t scan_alen(const int* acol, const int* brpt, int K, int64_t n, int64_t* out, hipStream_t s) {
+    return scan_impl(AlenF{acol, brpt, K}, n, out, s);
 }
 hipError_t scan_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s) {
     return scan_impl(I64F{in}, n, out, s);

@@ -358,7 +358,7 @@ int brm_server_submit(brm_server* S, int b_id, const int* arpt, const int* acol,
         a.N = B.N;
         HIPC(launch_validate(a, nnzA, -1, derr, S->aux));
         BRMC(check_dev_err(derr, S->aux, "request"));
-        HIPC(scan_alen(acol, brpt, nnzA, ipA, S->aux));
+        HIPC(scan_alen(acol, brpt, B.K, nnzA, ipA, S->aux));
         BRMC(d2h(&r->ip, ipA + nnzA, sizeof(int64_t), S->aux));
     }
     r->b = b_id;

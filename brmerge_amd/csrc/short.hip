@@ -35,94 +35,133 @@ __device__ __forceinline__ double permute_f64(int dst_lane, double x) {
     return __hiloint2double(hi, lo);
 }
 
-// Locates product `lane` of a row: returns its list position (A entry - a0).
-// Uniform control flow across the wavefront (shuffles inside).
-__device__ __forceinline__ int locate_product(const RowArgs& a, int a0, int L, int64_t base, int ip, int lane) {
-    if (L <= BRM_WAVE) {
-        int off = lane < L ? (int)(a.ipA[a0 + lane] - base) : INT_MAX;
-        int lo = 0;
-#pragma unroll
-        for (int step = 32; step >= 1; step >>= 1) {
-            int cand = lo + step;
-            int oc = __shfl(off, cand & (BRM_WAVE - 1), BRM_WAVE);
-            if (cand < L && oc <= lane) lo = cand;
-        }
-        return lo;
-    }
-    if (lane >= ip) return 0;
-    return upper_idx(a.ipA + a0, 0, L, base + lane);
-}
-
-template <bool NUMERIC>
-__global__ __launch_bounds__(SH_NT) void k_short(const int* __restrict__ rows, int nrows, RowArgs a, OutArgs o,
-                                                 int64_t* __restrict__ nnzrow) {
-    const int wid = blockIdx.x * SH_WPB + threadIdx.x / BRM_WAVE;
-    if (wid >= nrows) return;  // whole wavefront
+// G = 64: one row per wavefront; G = 32: two rows per wavefront, each in its
+// own half (every shuffle, ballot mask and rank stays inside the half).
+template <bool NUMERIC, int G>
+__global__ __launch_bounds__(SH_NT) void k_short(const int* __restrict__ rows, int nrows, int64_t wbase, RowArgs a,
+                                                 OutArgs o, int64_t* __restrict__ nnzrow) {
+    constexpr int RPW = BRM_WAVE / G;
     const int lane = lane_id();
-    const int row = rows[wid];
-    const int a0 = a.arpt[row];
-    const int L = a.arpt[row + 1] - a0;
-    const int64_t base = a.ipA[a0];
-    const int ip = (int)(a.ipA[a0 + L] - base);
+    const int gl = lane & (G - 1);
+    const int gbase = lane & ~(G - 1);
+    const uint64_t gmask = G == BRM_WAVE ? ~0ull : (((1ull << G) - 1ull) << gbase);
+    const int wid = (blockIdx.x * SH_WPB + (int)(threadIdx.x / BRM_WAVE)) * RPW + lane / G;
+    if (__ballot(wid < nrows) == 0ull) return;  // whole wavefront
+    const bool has = wid < nrows;  // a group past the end takes part with an empty row
+    int row = 0, a0 = 0, L = 0, ip = 0;
+    int64_t base = 0;
+    if (has) {
+        row = rows[wid];
+        a0 = a.arpt[row];
+        L = a.arpt[row + 1] - a0;
+        base = a.ipA[a0];
+        ip = (int)(a.ipA[a0 + L] - base);
+    }
 
-    const int p = locate_product(a, a0, L, base, ip, lane);
-    const bool act = lane < ip;
+    // list lanes (gl < L) load their list's product offset, B row start and
+    // A value; product gl finds its list p (largest p with off[p] <= gl) by
+    // a binary search over the group's lanes and takes p's descriptor by
+    // shuffles.  Rows with more lists than lanes search ipA in memory.
+    int off = INT_MAX, lbst = 0;
+    double lav = 0.0;
+    if (gl < L) {
+        off = (int)(a.ipA[a0 + gl] - base);
+        lbst = a.brpt[a.acol[a0 + gl]];
+        if (NUMERIC) lav = a.aval[a0 + gl];
+    }
+    int p = 0;
+#pragma unroll
+    for (int step = G / 2; step >= 1; step >>= 1) {
+        const int cand = p + step;
+        const int oc = __shfl(off, gbase + (cand & (G - 1)), BRM_WAVE);
+        if (cand < L && oc <= gl) p = cand;
+    }
+    const int src = gbase + (p & (G - 1));
+    const int p_off = __shfl(off, src, BRM_WAVE);
+    const int p_bst = __shfl(lbst, src, BRM_WAVE);
+    const double p_av = __shfl(lav, src, BRM_WAVE);
+    const bool act = gl < ip;
     int col = INT_MAX;
     double val = 0.0;
     if (act) {
-        const int ai = a0 + p;
-        const int bi = a.brpt[a.acol[ai]] + (int)(base + lane - a.ipA[ai]);
+        int bi;
+        double av = p_av;
+        if (L <= G) {
+            bi = p_bst + (gl - p_off);
+        } else {
+            p = upper_idx(a.ipA + a0, 0, L, base + gl);
+            const int ai = a0 + p;
+            bi = a.brpt[a.acol[ai]] + (int)(base + gl - a.ipA[ai]);
+            if (NUMERIC) av = a.aval[ai];
+        }
         col = a.bcol[bi];
-        if (NUMERIC) val = fmul(a.aval[ai], a.bval[bi]);
+        if (NUMERIC) val = fmul(av, a.bval[bi]);
     }
     const uint64_t key = act ? (((uint64_t)(uint32_t)col << 32) | (uint32_t)p) : ~0ull;
     const uint32_t klo_me = (uint32_t)key, khi_me = (uint32_t)(key >> 32);
 
+    // rank by (column, position) inside the group; inactive lanes hold the
+    // largest key, so they are never counted
     int rank = 0;
     bool dup = false;
-    for (int k = 0; k < ip; ++k) {
-        uint32_t klo = (uint32_t)__builtin_amdgcn_readlane((int)klo_me, k);
-        uint32_t khi = (uint32_t)__builtin_amdgcn_readlane((int)khi_me, k);
-        uint64_t kk = ((uint64_t)khi << 32) | klo;
-        bool lt = kk < key;
-        rank += lt;
-        dup |= lt && (khi == khi_me);
+    if (G == BRM_WAVE) {
+        const int n = __builtin_amdgcn_readfirstlane(ip);
+        for (int k = 0; k < n; ++k) {
+            const uint32_t klo = (uint32_t)__builtin_amdgcn_readlane((int)klo_me, k);
+            const uint32_t khi = (uint32_t)__builtin_amdgcn_readlane((int)khi_me, k);
+            const uint64_t kk = ((uint64_t)khi << 32) | klo;
+            const bool lt = kk < key;
+            rank += lt;
+            dup |= lt && (khi == khi_me);
+        }
+    } else {
+#pragma unroll 8
+        for (int k = 0; k < G; ++k) {
+            const uint32_t klo = (uint32_t)__shfl((int)klo_me, gbase + k, BRM_WAVE);
+            const uint32_t khi = (uint32_t)__shfl((int)khi_me, gbase + k, BRM_WAVE);
+            const uint64_t kk = ((uint64_t)khi << 32) | klo;
+            const bool lt = kk < key;
+            rank += lt;
+            dup |= lt && (khi == khi_me);
+        }
     }
 
     if (!NUMERIC) {
-        uint64_t first = __ballot(act && !dup);
-        if (lane == 0) nnzrow[row] = (int64_t)__popcll(first);
+        const uint64_t first = __ballot(act && !dup) & gmask;
+        if (has && gl == 0) nnzrow[row] = (int64_t)__popcll(first);
         return;
     }
 
-    const int dst = act ? rank : lane;
+    const int dst = act ? gbase + rank : lane;
     col = permute_i32(dst, col);
     int pos = permute_i32(dst, p);
     val = permute_f64(dst, val);
-    bool alive = lane < ip;
+    bool alive = gl < ip;
 
-    const int nlev = L <= 1 ? 0 : 32 - __clz(L - 1);
-    for (int l = 0; l < nlev; ++l) {
-        const uint64_t am = __ballot(alive);
-        const uint64_t above = (lane == BRM_WAVE - 1) ? 0ull : (am & (~0ull << (lane + 1)));
+    const uint64_t below_me = (1ull << lane) - 1ull;
+    const uint64_t above_me = lane == BRM_WAVE - 1 ? 0ull : ~0ull << (lane + 1);
+    for (int l = 0; l < 31; ++l) {
+        const uint64_t am = __ballot(alive) & gmask;
+        const uint64_t above = am & above_me;
         const int nxt = above ? (__ffsll((long long)above) - 1) : lane;
         const int ncol = __shfl(col, nxt, BRM_WAVE);
         const int npos = __shfl(pos, nxt, BRM_WAVE);
         const double nval = __shfl(val, nxt, BRM_WAVE);
         const bool same = alive && above != 0ull && ncol == col;
-        if (__ballot(same) == 0ull) break;  // no column left to combine
+        if (__ballot(same) == 0ull) break;  // no column left to combine in any group
         const bool comb = same && ((npos >> (l + 1)) == (pos >> (l + 1)));
         const uint64_t cm = __ballot(comb);
-        const uint64_t below = am & ((1ull << lane) - 1ull);
+        const uint64_t below = am & below_me;
         const int prv = below ? (63 - __clzll((long long)below)) : -1;
         const bool dies = prv >= 0 && ((cm >> prv) & 1ull);
         if (comb) val = fadd(val, nval);
         alive = alive && !dies;
     }
 
-    const uint64_t am = __ballot(alive);
-    const int idx = __popcll(am & ((1ull << lane) - 1ull));
+    const uint64_t am = __ballot(alive) & gmask;
+    const int idx = __popcll(am & below_me);
     const int nout = __popcll(am);
+    if (!has) return;
     if (!o.upper) {
         const int64_t cb = o.crpt[row];
         const int64_t cap = o.crpt[row + 1] - cb;
@@ -130,41 +169,93 @@ __global__ __launch_bounds__(SH_NT) void k_short(const int* __restrict__ rows, i
             o.ccol[cb + idx] = col;
             o.cval[cb + idx] = val;
         }
-        if (lane == 0 && (int64_t)nout != cap) atomicOr(o.err, (int)DERR_COUNT_MISMATCH);
+        if (gl == 0 && (int64_t)nout != cap) atomicOr(o.err, (int)DERR_COUNT_MISMATCH);
     } else {
+        const int64_t ob = o.slot ? (wbase + wid) * o.slot : base;
         if (alive) {
-            o.ccol[base + idx] = col;
-            o.cval[base + idx] = val;
+            o.ccol[ob + idx] = col;
+            o.cval[ob + idx] = val;
         }
-        if (lane == 0) o.rowlen[row] = nout;
+        if (gl == 0) o.rowlen[row] = nout;
+    }
+}
+
+// C row of short row k of the list, from its slot at k * G
+template <int G>
+__global__ __launch_bounds__(SH_NT) void k_short_copy(const int* __restrict__ rows, int nrows, int64_t wbase,
+                                                      const int* __restrict__ scol, const double* __restrict__ sval,
+                                                      const int64_t* __restrict__ crpt, int* __restrict__ ccol,
+                                                      double* __restrict__ cval) {
+    constexpr int RPW = BRM_WAVE / G;
+    const int lane = lane_id();
+    const int gl = lane & (G - 1);
+    const int wid = (blockIdx.x * SH_WPB + (int)(threadIdx.x / BRM_WAVE)) * RPW + lane / G;
+    if (wid >= nrows) return;
+    const int row = rows[wid];
+    const int64_t cb = crpt[row];
+    const int len = (int)(crpt[row + 1] - cb);
+    const int64_t sb = (wbase + wid) * G;
+    if (gl < len) {
+        ccol[cb + gl] = scol[sb + gl];
+        cval[cb + gl] = sval[sb + gl];
+    }
+}
+
+template <bool NUMERIC, int G>
+void short_launch(const int* rows, int n, const RowArgs& a, const OutArgs& o, int64_t* nnzrow, hipStream_t s) {
+    constexpr int RPB = SH_WPB * (BRM_WAVE / G);  // rows per block
+    const int64_t ch = launch_chunk(G);
+    for (int64_t b = 0; b < n; b += ch) {
+        const int c = (int)std::min<int64_t>(ch, n - b);
+        k_short<NUMERIC, G><<<(c + RPB - 1) / RPB, SH_NT, 0, s>>>(rows + b, c, b, a, o, nnzrow);
     }
 }
 
 }  // namespace
 
-hipError_t launch_short_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {
+hipError_t launch_short_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s,
+                              int group) {
     if (n <= 0) return hipSuccess;
     OutArgs o{};
     {
         ProfScope ps_("k_short_count", s);
-        const int64_t ch = launch_chunk(BRM_WAVE);
-        for (int64_t b = 0; b < n; b += ch) {
-            const int c = (int)std::min<int64_t>(ch, n - b);
-            k_short<false><<<(c + SH_WPB - 1) / SH_WPB, SH_NT, 0, s>>>(rows + b, c, a, o, nnzrow);
-        }
+        if (group == 32) short_launch<false, 32>(rows, n, a, o, nnzrow, s);
+        else short_launch<false, 64>(rows, n, a, o, nnzrow, s);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_short_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
+hipError_t launch_short_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
+                                int group) {
     if (n <= 0) return hipSuccess;
     {
         ProfScope ps_("k_short_numeric", s);
-        const int64_t ch = launch_chunk(BRM_WAVE);
-        for (int64_t b = 0; b < n; b += ch) {
-            const int c = (int)std::min<int64_t>(ch, n - b);
-            k_short<true><<<(c + SH_WPB - 1) / SH_WPB, SH_NT, 0, s>>>(rows + b, c, a, o, nullptr);
-        }
+        if (group == 32) short_launch<true, 32>(rows, n, a, o, nullptr, s);
+        else short_launch<true, 64>(rows, n, a, o, nullptr, s);
+    }
+    return hipGetLastError();
+}
+
+namespace {
+template <int G>
+void short_copy_t(const int* rows, int n, const int* scol, const double* sval, const int64_t* crpt, int* ccol,
+                  double* cval, hipStream_t s) {
+    constexpr int RPB = SH_WPB * (BRM_WAVE / G);
+    const int64_t ch = launch_chunk(G);
+    for (int64_t b = 0; b < n; b += ch) {
+        const int c = (int)std::min<int64_t>(ch, n - b);
+        k_short_copy<G><<<(c + RPB - 1) / RPB, SH_NT, 0, s>>>(rows + b, c, b, scol, sval, crpt, ccol, cval);
+    }
+}
+}  // namespace
+
+hipError_t launch_short_copy(const int* rows, int n, const int* scol, const double* sval, const int64_t* crpt,
+                             int* ccol, double* cval, hipStream_t s, int group) {
+    if (n <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_short_copy", s);
+        if (group == 32) short_copy_t<32>(rows, n, scol, sval, crpt, ccol, cval, s);
+        else short_copy_t<64>(rows, n, scol, sval, crpt, ccol, cval, s);
     }
     return hipGetLastError();
 }
