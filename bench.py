@@ -424,7 +424,10 @@ def main():
     last.free()
     nnz_row = np.diff(crpt)
 
-    tier = tiers(L, ip, bm.get_param("med_cap"), bm.get_param("wmed_cap"))
+    wcap = bm.get_param("wmed_cap")
+    if wcap < 0:  # the library's default: 1024 products while B has at most 65,536 columns
+        wcap = 1024 if bm.get_param("wmed_engine") == 1 and B.N <= 65536 else 512
+    tier = tiers(L, ip, bm.get_param("med_cap"), wcap)
     ms_per_step = 1000.0 * elapsed / args.steps
     gflops = 2.0 * total_ip * args.steps / elapsed / 1e9
     bytes_alg = 4 * (A.M + 1) + 20 * A.nnz + 12 * total_ip + 8 * (A.M + 1) + 12 * nnz_c

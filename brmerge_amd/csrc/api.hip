@@ -31,7 +31,7 @@
 namespace brm {
 
 Params& params() {
-    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, 512, 0, 1};
+    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, -1, 0, 1};
     return p;
 }
 
@@ -365,6 +365,7 @@ hipError_t bin_launch(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int fo
     const LdsShape shape = lds_shape(lds_cfg);
     p.lds_cfg = lds_cfg;
     p.med_cap = std::min(med_cap, shape.cap);
+    if (wmed_cap < 0) wmed_cap = params().wmed_engine == 1 ? rmed_auto_cap(p.a.N) : 512;
     p.wmed_cap = std::min(wmed_cap, WMED_CAP);
     hipError_t e = launch_fill_i32(p.bin_cnt, BIN_COUNT, 0, s);
     if (e != hipSuccess) return e;
@@ -932,7 +933,7 @@ int brm_set_param(const char* name, int64_t value) {
     else if (!strcmp(name, "profile")) { p.profile = value ? 1 : 0; }
     else if (!strcmp(name, "lds_cfg")) { if (value < 0 || value >= N_LDS_CFG) return BRM_E_INVALID; p.lds_cfg = (int)value; }
     else if (!strcmp(name, "group_cfg")) { if (value < 0 || value >= N_GROUP_CFG) return BRM_E_INVALID; p.group_cfg = (int)value; }
-    else if (!strcmp(name, "wmed_cap")) { if (value < 0 || value > WMED_CAP) return BRM_E_INVALID; p.wmed_cap = (int)value; }
+    else if (!strcmp(name, "wmed_cap")) { if (value < -1 || value > WMED_CAP) return BRM_E_INVALID; p.wmed_cap = (int)value; }
     else if (!strcmp(name, "launch_chunk")) { if (value < 0) return BRM_E_INVALID; p.launch_chunk = value; }
     else if (!strcmp(name, "wmed_engine")) { if (value < 0 || value > 1) return BRM_E_INVALID; p.wmed_engine = (int)value; }
     else return BRM_E_INVALID;

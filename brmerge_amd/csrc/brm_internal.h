@@ -158,7 +158,7 @@ struct Params {
     int profile;               // bracket every kernel launch with hipEvents
     int lds_cfg;               // LDS tier shape (threads, elements, lists): see lds_shape()
     int group_cfg;             // group-merge shape (threads, window elements): see group_shape()
-    int wmed_cap;              // wavefront LDS tier: products per row (0 = off, <= WMED_CAP)
+    int wmed_cap;              // wavefront tier: products per row (0 = off, <= WMED_CAP, -1 = by N)
     int64_t launch_chunk;      // test hook: rows/tasks per launch (0 = the grid limit)
     int wmed_engine;           // wavefront tier numeric: 0 LDS merge levels, 1 register bitonic sort
 };

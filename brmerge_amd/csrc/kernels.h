@@ -67,6 +67,9 @@ hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children
 
 // rmed.hip (register tier; cap = products per row, <= WMED_CAP)
 hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s, int cap);
+// Wavefront-tier capacity chosen when wmed_cap is -1 (register engine): 1024
+// products while 32-bit sort keys hold N columns, else 512.
+int rmed_auto_cap(int N);
 
 // bmerge.hip
 hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,

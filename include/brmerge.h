@@ -155,10 +155,12 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
                        int64_t *row_ip, int64_t *total, void *stream);
 
 /* Tuning knobs (names: "ws_budget_bytes", "med_cap", "leaf_cap",
- * "force_long", "profile", "lds_cfg", "group_cfg", "wmed_cap"); returns
- * BRM_E_INVALID for unknown names or values.  Knobs never change results,
- * only speed.  group_cfg 0-3: merge-path engine shapes for long rows; 4
- * (default): the bitmap-rank engine. */
+ * "force_long", "profile", "lds_cfg", "group_cfg", "wmed_cap",
+ * "wmed_engine", "launch_chunk"); returns BRM_E_INVALID for unknown names or
+ * values.  Knobs never change results, only speed.  group_cfg 0-3:
+ * merge-path engine shapes for long rows; 4 (default): the bitmap-rank
+ * engine.  wmed_cap: products per row of the wavefront tier (0 off, up to
+ * 1024; -1, the default: 1024 when B has at most 65,536 columns, else 512). */
 int brm_set_param(const char *name, int64_t value);
 int64_t brm_get_param(const char *name);
 

@@ -317,6 +317,40 @@ def test_wide_columns_and_hub_columns(gpu, engine, force_long):
         gpu.set_param("force_long", old)
 
 
+@pytest.mark.parametrize("ncols", [1 << 16, (1 << 16) + 1, 1 << 17, (1 << 17) + 1, 1 << 20], ids=lambda n: f"N{n}")
+@pytest.mark.parametrize("wmed_cap", [128, 512, 1024])
+def test_register_medium_key_widths(gpu, ncols, wmed_cap):
+    """Medium rows of the register tier with columns up to N - 1 on both
+    sides of the 32-bit key limit (2^16 columns at 1024 products, 2^17 at
+    512): keys of either width give the same bits."""
+    saved = {k: gpu.get_param(k) for k in ("wmed_cap", "wmed_engine")}
+    gpu.set_param("wmed_cap", wmed_cap)
+    gpu.set_param("wmed_engine", 1)
+    try:
+        rng = np.random.default_rng(ncols + wmed_cap)
+        K = 400
+        rows = []
+        for r in range(K):
+            d = int(rng.integers(0, 24))
+            top = rng.choice([ncols - 1, ncols - 2]) if r % 3 == 0 else None
+            c = rng.choice(ncols, size=d, replace=False)
+            if top is not None:
+                c = np.unique(np.concatenate([c, [top]]))
+            rows.append(np.sort(c))
+        brpt = np.concatenate([[0], np.cumsum([len(b) for b in rows])]).astype(np.int32)
+        B = (brpt, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, brpt[-1]), ncols)
+        acols = [np.sort(rng.choice(K, size=int(rng.integers(2, 64)), replace=False)) for _ in range(300)]
+        arpt = np.concatenate([[0], np.cumsum([len(a) for a in acols])]).astype(np.int32)
+        A = (arpt, np.concatenate(acols).astype(np.int32), rng.uniform(-1, 1, arpt[-1]), K)
+        R = oracle.brmerge_precise(A, B, nthreads=4)
+        Cp, Cu = run_both(gpu, A, B)
+        assert_csr_equal(Cp, R)
+        assert_csr_equal(Cu, R)
+    finally:
+        for k, v in saved.items():
+            gpu.set_param(k, v)
+
+
 @pytest.mark.parametrize("engine_wm", [0, 1], ids=["lds", "registers"])
 def test_medium_rows_with_deep_column_trees(gpu, engine_wm):
     """Rows of 64 lists whose B rows share a handful of columns: every output
