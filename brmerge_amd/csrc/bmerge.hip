@@ -691,14 +691,18 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             }
         }
         {
-            // the bitmap is free until the next round: wave w uses words
-            // [512 w, 512 w + 256) for dense sums and the next 32 for flags,
+            // the bitmap and the word ranks are free until the next round:
+            // wave w sums densely in the first 256 words of its share of the
+            // bitmap, with flags in bytes [256 w, 256 w + 256) of the ranks,
             // then clears its share of the bitmap
+            static_assert(BG_KMAX <= BG_WORDS / BG_NW, "dense sums fit a wave's bitmap share");
+            static_assert(BG_NW * BG_KMAX <= (int)sizeof(BgSmem::wrank), "flags fit the word ranks");
             const int wv = t / BRM_WAVE, lane = lane_id();
             double* dense = reinterpret_cast<double*>(&g.bm[wv * (BG_WORDS / BG_NW)]);
-            unsigned char* pres = reinterpret_cast<unsigned char*>(&g.bm[wv * (BG_WORDS / BG_NW) + BG_KMAX]);
+            unsigned char* pres = reinterpret_cast<unsigned char*>(g.wrank) + wv * BG_KMAX;
             const int nbig = g.nbig;
             if (nbig > 0) {
+                static_assert(BG_KMAX == 4 * BRM_WAVE, "one flag word per lane");
                 reinterpret_cast<unsigned*>(pres)[lane] = 0u;
                 wave_lds_sync();
                 for (int b = wv; b < nbig; b += BG_NW) {
