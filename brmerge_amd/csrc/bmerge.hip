@@ -321,11 +321,39 @@ __device__ __forceinline__ void node_combine(double& a, bool& pa, double b, bool
     pa = pa || pb;
 }
 
+// Value of lane (lane - shift) for the DPP control CTRL (row_shr:s within a
+// row of 16 lanes, row_bcast15 / row_bcast31 across rows); lanes without a
+// source keep 0.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+    const int lo = dpp_i<CTRL>(__double2loint(x));
+    const int hi = dpp_i<CTRL>(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+
+// One tree level over lanes: lane i with i % (2 s) == 2 s - 1 holds the
+// right node and takes the left one from lane i - s (CTRL moves it there).
+template <int CTRL, int S>
+__device__ __forceinline__ void dpp_level(double& x, bool& px, int lane) {
+    double l = dpp_d<CTRL>(x);
+    bool pl = dpp_i<CTRL>((int)px) != 0;
+    if ((lane & (2 * S - 1)) == 2 * S - 1) {
+        node_combine(l, pl, x, px);
+        x = l;
+        px = pl;
+    }
+}
+
 // Sum of output bucket [s, s + m) by one wavefront: the contributions are
 // placed densely by list id (256 slots in dense/pres, private to the wave,
 // pres all zero on entry and on return), each lane reduces its 4 slots, and
-// the aligned tree over lanes is finished with shuffles.  Returns the sum in
-// every lane.
+// the aligned tree over lanes is finished with DPP moves (no LDS round
+// trips): lanes 2s-1 mod 2s combine at level s, the root ends in lane 63.
+// Returns the sum in every lane.
 __device__ double bg_wave_sum(const BgSmem& g, int s, int m, double* dense, unsigned char* pres) {
     const int lane = lane_id();
     for (int i = lane; i < m; i += BRM_WAVE) {
@@ -350,14 +378,14 @@ __device__ double bg_wave_sum(const BgSmem& g, int s, int m, double* dense, unsi
     node_combine(v[0], p[0], v[2], p[2]);
     double x = v[0];
     bool px = p[0];
-#pragma unroll
-    for (int d = 1; d < BRM_WAVE; d <<= 1) {
-        const double y = __shfl_xor(x, d, BRM_WAVE);
-        const bool py = __shfl_xor((int)px, d, BRM_WAVE) != 0;
-        if ((lane & d) == 0) node_combine(x, px, y, py);
-    }
+    dpp_level<0x111, 1>(x, px, lane);   // row_shr:1
+    dpp_level<0x112, 2>(x, px, lane);   // row_shr:2
+    dpp_level<0x114, 4>(x, px, lane);   // row_shr:4
+    dpp_level<0x118, 8>(x, px, lane);   // row_shr:8
+    dpp_level<0x142, 16>(x, px, lane);  // row_bcast15: lanes 31, 63 get 15, 47
+    dpp_level<0x143, 32>(x, px, lane);  // row_bcast31: lane 63 gets 31
     wave_lds_sync();
-    return __shfl(x, 0, BRM_WAVE);
+    return __shfl(x, BRM_WAVE - 1, BRM_WAVE);
 }
 
 __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ tasks,
