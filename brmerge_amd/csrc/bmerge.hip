@@ -348,34 +348,42 @@ __device__ __forceinline__ void dpp_level(double& x, bool& px, int lane) {
     }
 }
 
-// Sum of output bucket [s, s + m) by one wavefront: the contributions are
-// placed densely by list id (256 slots in dense/pres, private to the wave,
-// pres all zero on entry and on return), each lane reduces its 4 slots, and
-// the aligned tree over lanes is finished with DPP moves (no LDS round
-// trips): lanes 2s-1 mod 2s combine at level s, the root ends in lane 63.
-// Returns the sum in every lane.
-__device__ double bg_wave_sum(const BgSmem& g, int s, int m, double* dense, unsigned char* pres) {
+// Sums of two output buckets by one wavefront, one per half of 32 lanes
+// (m = 0: no bucket).  Each half places its bucket's contributions densely
+// by list id (256 slots of its own in dense/pres, pres all zero on entry and
+// on return), each lane reduces its 8 slots, and the aligned tree over the
+// half's lanes is finished with DPP moves (no LDS round trips): lanes
+// 2s-1 mod 2s combine at level s, the roots end in lanes 31 and 63.
+// Returns the first sum in r0 and the second in r1, in every lane.
+__device__ void bg_wave_sum2(const BgSmem& g, int s0, int m0, int s1, int m1, double* dense,
+                             unsigned char* pres, double& r0, double& r1) {
     const int lane = lane_id();
-    for (int i = lane; i < m; i += BRM_WAVE) {
+    const int h = lane >> 5, hl = lane & 31;
+    const int s = h ? s1 : s0, m = h ? m1 : m0;
+    double* dn = dense + h * BG_KMAX;
+    unsigned char* pr = pres + h * BG_KMAX;
+    for (int i = hl; i < m; i += 32) {
         const int e = g.contrib[s + i];
         const int l = g.elid[e];
-        dense[l] = g.val[e];
-        pres[l] = 1;
+        dn[l] = g.val[e];
+        pr[l] = 1;
     }
     wave_lds_sync();
-    double v[4];
-    bool p[4];
+    constexpr int SPL = BG_KMAX / 32;  // slots per lane
+    double v[SPL];
+    bool p[SPL];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        p[i] = pres[4 * lane + i] != 0;
-        v[i] = dense[4 * lane + i];
+    for (int i = 0; i < SPL; ++i) {
+        p[i] = pr[SPL * hl + i] != 0;
+        v[i] = dn[SPL * hl + i];
     }
     wave_lds_sync();
 #pragma unroll
-    for (int i = 0; i < 4; ++i) pres[4 * lane + i] = 0;
-    node_combine(v[0], p[0], v[1], p[1]);
-    node_combine(v[2], p[2], v[3], p[3]);
-    node_combine(v[0], p[0], v[2], p[2]);
+    for (int i = 0; i < SPL; ++i) pr[SPL * hl + i] = 0;
+#pragma unroll
+    for (int w = 1; w < SPL; w <<= 1)
+#pragma unroll
+        for (int i = 0; i + w < SPL; i += 2 * w) node_combine(v[i], p[i], v[i + w], p[i + w]);
     double x = v[0];
     bool px = p[0];
     dpp_level<0x111, 1>(x, px, lane);   // row_shr:1
@@ -383,9 +391,9 @@ __device__ double bg_wave_sum(const BgSmem& g, int s, int m, double* dense, unsi
     dpp_level<0x114, 4>(x, px, lane);   // row_shr:4
     dpp_level<0x118, 8>(x, px, lane);   // row_shr:8
     dpp_level<0x142, 16>(x, px, lane);  // row_bcast15: lanes 31, 63 get 15, 47
-    dpp_level<0x143, 32>(x, px, lane);  // row_bcast31: lane 63 gets 31
     wave_lds_sync();
-    return __shfl(x, BRM_WAVE - 1, BRM_WAVE);
+    r0 = __shfl(x, 31, BRM_WAVE);
+    r1 = __shfl(x, 63, BRM_WAVE);
 }
 
 __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ tasks,
@@ -720,27 +728,34 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         }
         {
             // the bitmap and the word ranks are free until the next round:
-            // wave w sums densely in the first 256 words of its share of the
-            // bitmap, with flags in bytes [256 w, 256 w + 256) of the ranks,
-            // then clears its share of the bitmap
-            static_assert(BG_KMAX <= BG_WORDS / BG_NW, "dense sums fit a wave's bitmap share");
-            static_assert(BG_NW * BG_KMAX <= (int)sizeof(BgSmem::wrank), "flags fit the word ranks");
+            // wave w sums two buckets at a time densely in the first 512
+            // words of its share of the bitmap, with flags in bytes
+            // [512 w, 512 w + 512) of the ranks, then clears its share of the
+            // bitmap
+            static_assert(2 * BG_KMAX <= BG_WORDS / BG_NW, "dense sums fit a wave's bitmap share");
+            static_assert(BG_NW * 2 * BG_KMAX <= (int)sizeof(BgSmem::wrank), "flags fit the word ranks");
             const int wv = t / BRM_WAVE, lane = lane_id();
             double* dense = reinterpret_cast<double*>(&g.bm[wv * (BG_WORDS / BG_NW)]);
-            unsigned char* pres = reinterpret_cast<unsigned char*>(g.wrank) + wv * BG_KMAX;
+            unsigned char* pres = reinterpret_cast<unsigned char*>(g.wrank) + wv * 2 * BG_KMAX;
             const int nbig = g.nbig;
             if (nbig > 0) {
-                static_assert(BG_KMAX == 4 * BRM_WAVE, "one flag word per lane");
+                static_assert(2 * BG_KMAX == 8 * BRM_WAVE, "two flag words per lane");
                 reinterpret_cast<unsigned*>(pres)[lane] = 0u;
+                reinterpret_cast<unsigned*>(pres)[BRM_WAVE + lane] = 0u;
                 wave_lds_sync();
-                for (int b = wv; b < nbig; b += BG_NW) {
-                    const int oi = g.big[b];
-                    const int s = oi ? g.cnt[oi - 1] : 0;
-                    const double v = bg_wave_sum(g, s, g.cnt[oi] - s, dense, pres);
+                for (int b = 2 * wv; b < nbig; b += 2 * BG_NW) {
+                    const int oi0 = g.big[b];
+                    const int oi1 = b + 1 < nbig ? g.big[b + 1] : -1;
+                    const int s0 = oi0 ? g.cnt[oi0 - 1] : 0;
+                    const int s1 = oi1 > 0 ? g.cnt[oi1 - 1] : 0;
+                    const int m1 = oi1 >= 0 ? g.cnt[oi1] - s1 : 0;
+                    double v0, v1;
+                    bg_wave_sum2(g, s0, g.cnt[oi0] - s0, s1, m1, dense, pres, v0, v1);
+                    const int oi = lane == 0 ? oi0 : oi1;
                     const int64_t q = out_n + oi;
-                    if (lane == 0 && q < dcap) {
-                        dc[q] = g.col[g.contrib[s]];
-                        dv[q] = v;
+                    if ((lane == 0 || (lane == 32 && oi1 >= 0)) && q < dcap) {
+                        dc[q] = g.col[g.contrib[lane == 0 ? s0 : s1]];
+                        dv[q] = lane == 0 ? v0 : v1;
                     }
                 }
             }
