@@ -162,6 +162,76 @@ __device__ void emit_group(const LongArgs& la, int row, int a0, const int64_t* i
     la.tasks[la.task_seg[j] + (int64_t)idx] = T;
 }
 
+// Without a frontier (frontier_cap 0: every non-empty list is a child in
+// place) the children of pass-j node (s, h_j) are exactly the non-empty
+// nodes at height h_{j-1} inside it, at most PL_NT of them: the workgroup
+// emits each node's children together, thread t taking child slot t, with
+// a block scan for their order.
+__device__ void plan_no_frontier(const LongArgs& la, int row, int a0, const int64_t* ipr, int L, int H, int J,
+                                 int64_t wsb, int64_t slb, int64_t final_off, int* err) {
+    __shared__ int64_t sm[PL_NT / BRM_WAVE + 1];
+    __shared__ int64_t cbase;
+    const int t = threadIdx.x;
+    const int pb = la.pass_bits;
+    for (int j = 1; j <= J; ++j) {
+        const int hj = pass_height(j, H, pb);
+        const int hp = j == 1 ? 0 : pass_height(j - 1, H, pb);
+        const bool root = hj == H;
+        const int nnodes = root ? 1 : ((L - 1) >> hj) + 1;
+        for (int nd = 0; nd < nnodes; ++nd) {
+            const int s = nd << hj;
+            const NodeK u = node_kind(ipr, L, s, hj);  // uniform
+            if (u.kind == SRC_EMPTY || (u.kind == SRC_VIRTUAL && !root)) continue;
+            const int e = node_end(L, s, hj);
+            const int q = s + (t << hp);
+            Src src{};
+            src.kind = SRC_EMPTY;
+            if (q < e) src = make_src(la, a0, ipr, L, H, q, hp, (j - 1) & 1, wsb, slb);
+            const int64_t has = src.kind != SRC_EMPTY ? 1 : 0;
+            int64_t nc;
+            const int64_t rank = block_excl_scan<PL_NT>(has, sm, &nc);
+            if (t == 0) {
+                const unsigned long long cb =
+                    atomicAdd((unsigned long long*)&la.counters[CNT_CHILD], (unsigned long long)nc);
+                cbase = (int64_t)cb;
+                if ((int64_t)cb + nc > la.child_cap) atomicOr(err, (int)DERR_CAPACITY);
+            }
+            __syncthreads();
+            const int64_t cb = cbase;
+            __syncthreads();  // cbase is rewritten for the next node
+            if (cb + nc > la.child_cap) continue;  // uniform
+            if (has) {
+                Child c;
+                c.src = src;
+                c.lid = t;  // (q - s) >> hp
+                c.pad = 0;
+                la.children[cb + rank] = c;
+            }
+            if (t == 0) {
+                GroupTask T;
+                T.row = row;
+                T.nchild = (int)nc;
+                T.cbase = cb;
+                if (root) {
+                    T.final_dst = 1;
+                    T.dst_buf = 0;
+                    T.dst_off = final_off;
+                    T.slot = -1;
+                } else {
+                    T.final_dst = 0;
+                    T.dst_buf = j & 1;
+                    T.dst_off = wsb + (ipr[s] - ipr[0]);
+                    T.slot = slb + heap_idx(H, s, hj);
+                }
+                const unsigned long long idx =
+                    atomicAdd((unsigned long long*)&la.counters[CNT_TASK0 + j], 1ull);
+                if ((int64_t)idx >= la.task_seg[j + 1] - la.task_seg[j]) atomicOr(err, (int)DERR_CAPACITY);
+                else la.tasks[la.task_seg[j] + (int64_t)idx] = T;
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int r0, int* err) {
     const Fit cap{la.frontier_cap, la.lmax, la.log_lmax};
     const int r = r0 + blockIdx.x;
@@ -174,6 +244,11 @@ __global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int r0, int* err) {
     const int J = H == 0 ? 1 : (H + pb - 1) / pb;
     const int64_t wsb = la.ws_base[r], slb = la.slot_base[r];
     const int64_t final_off = la.final_off ? la.final_off[row] : ipr[0];
+    if (la.frontier_cap == 0 && (1 << pb) <= PL_NT) {
+        plan_no_frontier(la, row, a0, ipr, L, H, J, wsb, slb, final_off, err);
+        if (threadIdx.x == 0) atomicMax((unsigned long long*)&la.counters[CNT_MAXP], (unsigned long long)J);
+        return;
+    }
 
     for (int p = threadIdx.x; p < L; p += PL_NT) {
         const int hf = frontier_height(ipr, L, H, p, cap);
