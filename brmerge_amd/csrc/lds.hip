@@ -219,17 +219,23 @@ __global__ __launch_bounds__(HS_NT) void k_sym_hash(const int* __restrict__ rows
 
 // Long rows: distinct columns counted with an LDS bitmap of up to 2^20
 // columns per pass (128 KB); wider matrices take several column passes.
+// Lists are taken in chunks of BM_CHUNK: their product offsets and B row
+// starts go to LDS first, so a product's column is one global load, and
+// each thread handles BM_U consecutive products per step with their loads
+// issued together.
 constexpr int BM_NT = 1024;
 constexpr int BM_BITS = 1 << 20;
 constexpr int BM_WORDS = BM_BITS / 32;
-constexpr int BM_CHUNK = 4096;
+constexpr int BM_CHUNK = 2048;
+constexpr int BM_U = 8;
 
 __global__ __launch_bounds__(BM_NT) void k_sym_bitmap(const int* __restrict__ rows, int nrows, RowArgs a,
                                                       int64_t* __restrict__ nnzrow) {
     extern __shared__ __align__(16) unsigned char smraw[];
     unsigned* bm = reinterpret_cast<unsigned*>(smraw);
     int* loff = reinterpret_cast<int*>(smraw + sizeof(unsigned) * BM_WORDS);
-    int64_t* red = reinterpret_cast<int64_t*>(smraw + sizeof(unsigned) * BM_WORDS + sizeof(int) * (BM_CHUNK + 8));
+    int* bst = loff + BM_CHUNK + 8;
+    int64_t* red = reinterpret_cast<int64_t*>(bst + BM_CHUNK + 8);
     const int t = threadIdx.x;
     const int row = rows[blockIdx.x];
     const int ab = a.arpt[row], ae = a.arpt[row + 1];
@@ -242,16 +248,31 @@ __global__ __launch_bounds__(BM_NT) void k_sym_bitmap(const int* __restrict__ ro
         for (int cs = ab; cs < ae; cs += BM_CHUNK) {
             const int nl = min(BM_CHUNK, ae - cs);
             const int64_t cb = a.ipA[cs];
-            for (int k = t; k <= nl; k += BM_NT) loff[k] = (int)(a.ipA[cs + k] - cb);
+            for (int k = t; k <= nl; k += BM_NT) {
+                loff[k] = (int)(a.ipA[cs + k] - cb);
+                if (k < nl) bst[k] = a.brpt[a.acol[cs + k]];
+            }
             __syncthreads();
             const int Sc = loff[nl];
             int hint = 0;
-            for (int e = t; e < Sc; e += BM_NT) {
-                hint = gallop_idx(loff, hint, nl, e);
-                const int ai = cs + hint;
-                const int col = a.bcol[a.brpt[a.acol[ai]] + (e - loff[hint])];
-                const int64_t rel = (int64_t)col - c0;
-                if (rel >= 0 && rel < span) atomicOr(&bm[rel >> 5], 1u << (rel & 31));
+            for (int e0 = t * BM_U; e0 < Sc; e0 += BM_NT * BM_U) {
+                hint = gallop_idx(loff, hint, nl, e0);
+                int col[BM_U];
+                int h = hint;
+#pragma unroll
+                for (int u = 0; u < BM_U; ++u) {
+                    const int e = e0 + u;
+                    col[u] = -1;
+                    if (e < Sc) {
+                        while (loff[h + 1] <= e) ++h;  // next non-empty list
+                        col[u] = a.bcol[bst[h] + (e - loff[h])];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < BM_U; ++u) {
+                    const int64_t rel = (int64_t)col[u] - c0;
+                    if (col[u] >= 0 && rel >= 0 && rel < span) atomicOr(&bm[rel >> 5], 1u << (rel & 31));
+                }
             }
             __syncthreads();
         }
@@ -264,8 +285,9 @@ __global__ __launch_bounds__(BM_NT) void k_sym_bitmap(const int* __restrict__ ro
 }
 
 constexpr size_t bitmap_smem() {
-    return sizeof(unsigned) * BM_WORDS + sizeof(int) * (BM_CHUNK + 8) + sizeof(int64_t) * (BM_NT / BRM_WAVE + 2);
+    return sizeof(unsigned) * BM_WORDS + 2 * sizeof(int) * (BM_CHUNK + 8) + sizeof(int64_t) * (BM_NT / BRM_WAVE + 2);
 }
+static_assert(bitmap_smem() <= 160 * 1024, "LDS of one CU");
 
 template <int NT, int CAP, int LMAX>
 hipError_t medium_numeric_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s,
