@@ -201,6 +201,7 @@ def run_batched(args, bm):
         "gflops": round(2.0 * ip / el / 1e9, 3),
         "one_call_per_product": {"products_per_s": round(loop_rate, 2), "sample": ns},
         "speedup_vs_loop": round(rate / loop_rate, 2),
+        "plan_speedup_vs_loop": round(plan_out["products_per_s"] / loop_rate, 2) if plan_out else None,
         "plan": plan_out, "kernels_per_step": kernels, "stats": st,
     }
 

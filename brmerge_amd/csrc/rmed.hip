@@ -138,11 +138,30 @@ template <int R, typename KT>
 __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t base, const OutArgs& o,
                             RbSmem<R>& b) {
     using Key = RmKey<R, KT>;
-    constexpr int WPL = RB_WORDS / BRM_WAVE;  // bitmap words per lane
+    constexpr int WPL = RB_WORDS / BRM_WAVE;  // bitmap words per lane, at most
     const int lane = lane_id();
+    // only the words between the row's smallest and largest column are used:
+    // lane l owns words [wb, we), cpl consecutive ones
+    int cmn = INT_MAX, cmx = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (r * 64 + lane < ip) {
+            const int c = (int)Key::col(key[r]);
+            cmn = min(cmn, c);
+            cmx = max(cmx, c);
+        }
+    }
+#pragma unroll
+    for (int d = BRM_WAVE / 2; d > 0; d >>= 1) {
+        cmn = min(cmn, __shfl_xor(cmn, d, BRM_WAVE));
+        cmx = max(cmx, __shfl_xor(cmx, d, BRM_WAVE));
+    }
+    const int w0 = cmn >> 6, w1 = min((cmx >> 6) + 1, nw);
+    const int cpl = (w1 - w0 + BRM_WAVE - 1) / BRM_WAVE;
+    const int wb = w0 + lane * cpl, we = min(wb + cpl, w1);
 #pragma unroll
     for (int i = 0; i < WPL; ++i)
-        if (lane * WPL + i < nw) b.u.bm[lane * WPL + i] = 0ull;
+        if (wb + i < we) b.u.bm[wb + i] = 0ull;
     wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -156,7 +175,7 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
     int sum = 0;
 #pragma unroll
     for (int i = 0; i < WPL; ++i) {
-        pc[i] = lane * WPL + i < nw ? __popcll(b.u.bm[lane * WPL + i]) : 0;
+        pc[i] = wb + i < we ? __popcll(b.u.bm[wb + i]) : 0;
         sum += pc[i];
     }
     const int inc = wave_incl_scan(sum);
@@ -164,7 +183,7 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
     int ex = inc - sum;
 #pragma unroll
     for (int i = 0; i < WPL; ++i) {
-        if (lane * WPL + i < nw) b.w.wrank[lane * WPL + i] = (unsigned short)ex;
+        if (wb + i < we) b.w.wrank[wb + i] = (unsigned short)ex;
         ex += pc[i];
     }
     wave_lds_sync();
@@ -324,6 +343,10 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = r * 64 + lane;
+        lp[r] = 0;
+        bi[r] = 0;
+        sv[r] = 0.0;
+        if (r * 64 >= ip) continue;  // uniform: registers past the row's products
         int p = 0;
 #pragma unroll
         for (int step = 32; step >= 1; step >>= 1) {  // largest list p with off[p] <= j
@@ -349,7 +372,8 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
         }
     }
 #pragma unroll
-    for (int r = 0; r < R; ++r) sm.val[r * 64 + lane] = fmul(sv[r], val[r]);
+    for (int r = 0; r < R; ++r)
+        if (r * 64 < ip) sm.val[r * 64 + lane] = fmul(sv[r], val[r]);
 
     // ---- B with at most RB_SPAN columns: bitmap ranks instead of a sort
     if (BITMAP) {
