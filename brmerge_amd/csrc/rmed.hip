@@ -330,18 +330,47 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
     const int ip = (int)(a.ipA[a0 + L] - base);
 
     // ---- 1. list descriptors in lanes, products in registers
-    int off = INT_MAX, bst = 0;
+    int off = INT_MAX, bst = 0, len = 0;
     double sc = 0.0;
     if (lane < L) {
         off = (int)(a.ipA[a0 + lane] - base);
+        len = (int)(a.ipA[a0 + lane + 1] - base) - off;
         bst = a.brpt[a.acol[a0 + lane]];
         sc = a.aval[a0 + lane];
     }
     KT key[R];
     int bi[R], lp[R];
     double sv[R];
+    if constexpr (BITMAP) {
+        // every list lane writes its products' B index and list position to
+        // LDS (in the bitmap area, unused until the ranks), one read each
+        static_assert(64 * R * 5 <= (int)sizeof(RbSmem<R>::u), "product map fits the bitmap area");
+        auto& b = *reinterpret_cast<RbSmem<R>*>(&shm);
+        int* pbi = reinterpret_cast<int*>(&b.u);
+        unsigned char* plp = reinterpret_cast<unsigned char*>(pbi + 64 * R);
+        for (int k = 0; k < len; ++k) {
+            pbi[off + k] = bst + k;
+            plp[off + k] = (unsigned char)lane;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = r * 64 + lane;
+            lp[r] = 0;
+            bi[r] = 0;
+            sv[r] = 0.0;
+            if (r * 64 >= ip) continue;  // uniform
+            if (j < ip) {
+                bi[r] = pbi[j];
+                lp[r] = plp[j];
+            }
+            sv[r] = __shfl(sc, lp[r], BRM_WAVE);
+        }
+        wave_lds_sync();  // the map is read before the bitmap area is reused
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
+        if constexpr (BITMAP) break;
         const int j = r * 64 + lane;
         lp[r] = 0;
         bi[r] = 0;
