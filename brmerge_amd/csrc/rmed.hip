@@ -341,13 +341,21 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
     KT key[R];
     int bi[R], lp[R];
     double sv[R];
-    if constexpr (BITMAP) {
+    {
         // every list lane writes its products' B index and list position to
-        // LDS (in the bitmap area, unused until the ranks), one read each
-        static_assert(64 * R * 5 <= (int)sizeof(RbSmem<R>::u), "product map fits the bitmap area");
-        auto& b = *reinterpret_cast<RbSmem<R>*>(&shm);
-        int* pbi = reinterpret_cast<int*>(&b.u);
-        unsigned char* plp = reinterpret_cast<unsigned char*>(pbi + 64 * R);
+        // LDS (areas unused until later: the bitmap area, or the keys and
+        // values of the sort), one read each
+        int* pbi;
+        unsigned char* plp;
+        if constexpr (BITMAP) {
+            static_assert(64 * R * 5 <= (int)sizeof(RbSmem<R>::u), "product map fits the bitmap area");
+            auto& b = *reinterpret_cast<RbSmem<R>*>(&shm);
+            pbi = reinterpret_cast<int*>(&b.u);
+            plp = reinterpret_cast<unsigned char*>(pbi + 64 * R);
+        } else {
+            pbi = reinterpret_cast<int*>(sm.key);
+            plp = reinterpret_cast<unsigned char*>(sm.val);
+        }
         for (int k = 0; k < len; ++k) {
             pbi[off + k] = bst + k;
             plp[off + k] = (unsigned char)lane;
@@ -366,28 +374,7 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
             }
             sv[r] = __shfl(sc, lp[r], BRM_WAVE);
         }
-        wave_lds_sync();  // the map is read before the bitmap area is reused
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        if constexpr (BITMAP) break;
-        const int j = r * 64 + lane;
-        lp[r] = 0;
-        bi[r] = 0;
-        sv[r] = 0.0;
-        if (r * 64 >= ip) continue;  // uniform: registers past the row's products
-        int p = 0;
-#pragma unroll
-        for (int step = 32; step >= 1; step >>= 1) {  // largest list p with off[p] <= j
-            const int cand = p + step;
-            const int oc = __shfl(off, cand & (BRM_WAVE - 1), BRM_WAVE);
-            if (cand < L && oc <= j) p = cand;
-        }
-        const int o_p = __shfl(off, p, BRM_WAVE);
-        const int b_p = __shfl(bst, p, BRM_WAVE);
-        sv[r] = __shfl(sc, p, BRM_WAVE);
-        lp[r] = p;
-        bi[r] = b_p + (j - o_p);
+        wave_lds_sync();  // the map is read before its areas are reused
     }
     double val[R];
 #pragma unroll
