@@ -92,6 +92,22 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Inclusive scan over the wavefront with DPP moves (Hillis-Steele within
+// rows of 16 lanes, then row broadcasts): no LDS round trips.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_add(int x) {
+    return x + __builtin_amdgcn_update_dpp(0, x, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ int wave_incl_scan_dpp(int x) {
+    x = dpp_add<0x111, 0xF>(x);  // row_shr:1
+    x = dpp_add<0x112, 0xF>(x);  // row_shr:2
+    x = dpp_add<0x114, 0xF>(x);  // row_shr:4
+    x = dpp_add<0x118, 0xF>(x);  // row_shr:8
+    x = dpp_add<0x142, 0xA>(x);  // row_bcast15 into rows 1 and 3
+    x = dpp_add<0x143, 0xC>(x);  // row_bcast31 into rows 2 and 3
+    return x;
+}
+
 // Sum of one bucket (m >= 2 contributions from distinct lists) in the
 // aligned tree over list positions: leaves are visited in increasing list
 // position (the bits of `mask`), each found in the bucket by a scan; the
@@ -140,8 +156,7 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
     using Key = RmKey<R, KT>;
     constexpr int WPL = RB_WORDS / BRM_WAVE;  // bitmap words per lane, at most
     const int lane = lane_id();
-    // only the words between the row's smallest and largest column are used:
-    // lane l owns words [wb, we), cpl consecutive ones
+    // only the words between the row's smallest and largest column are used
     int cmn = INT_MAX, cmx = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -156,12 +171,14 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
         cmn = min(cmn, __shfl_xor(cmn, d, BRM_WAVE));
         cmx = max(cmx, __shfl_xor(cmx, d, BRM_WAVE));
     }
+    // words w0 + 64 i + lane (consecutive lanes on consecutive words)
     const int w0 = cmn >> 6, w1 = min((cmx >> 6) + 1, nw);
-    const int cpl = (w1 - w0 + BRM_WAVE - 1) / BRM_WAVE;
-    const int wb = w0 + lane * cpl, we = min(wb + cpl, w1);
+    const int nit = (w1 - w0 + BRM_WAVE - 1) / BRM_WAVE;
 #pragma unroll
-    for (int i = 0; i < WPL; ++i)
-        if (wb + i < we) b.u.bm[wb + i] = 0ull;
+    for (int i = 0; i < WPL; ++i) {
+        const int w = w0 + i * BRM_WAVE + lane;
+        if (i < nit && w < w1) b.u.bm[w] = 0ull;
+    }
     wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -171,20 +188,15 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
         }
     }
     wave_lds_sync();
-    int pc[WPL];
-    int sum = 0;
+    int nout = 0;  // ranks: a scan per step of 64 words, carried
 #pragma unroll
     for (int i = 0; i < WPL; ++i) {
-        pc[i] = wb + i < we ? __popcll(b.u.bm[wb + i]) : 0;
-        sum += pc[i];
-    }
-    const int inc = wave_incl_scan(sum);
-    const int nout = __shfl(inc, BRM_WAVE - 1, BRM_WAVE);
-    int ex = inc - sum;
-#pragma unroll
-    for (int i = 0; i < WPL; ++i) {
-        if (wb + i < we) b.w.wrank[wb + i] = (unsigned short)ex;
-        ex += pc[i];
+        if (i >= nit) break;  // uniform
+        const int w = w0 + i * BRM_WAVE + lane;
+        const int pc = w < w1 ? __popcll(b.u.bm[w]) : 0;
+        const int inc = wave_incl_scan_dpp(pc);
+        if (w < w1) b.w.wrank[w] = (unsigned short)(nout + inc - pc);
+        nout += __shfl(inc, BRM_WAVE - 1, BRM_WAVE);
     }
     wave_lds_sync();
     int pos[R];
@@ -199,7 +211,8 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
     }
     wave_lds_sync();  // bitmap and ranks are dead from here on
 #pragma unroll
-    for (int i = 0; i < R; ++i) b.u.c.cnt[lane * R + i] = 0;
+    for (int i = 0; i < R; ++i)
+        if (i * BRM_WAVE < nout) b.u.c.cnt[i * BRM_WAVE + lane] = 0;
     wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -210,19 +223,15 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
     }
     wave_lds_sync();
     {
-        int loc[R];
-        int s = 0;
+        int carry = 0;  // bucket offsets: a scan per step of 64 buckets, carried
 #pragma unroll
         for (int i = 0; i < R; ++i) {
-            loc[i] = b.u.c.cnt[lane * R + i];
-            s += loc[i];
-        }
-        const int inc2 = wave_incl_scan(s);
-        int ex2 = inc2 - s;
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            b.u.c.cnt[lane * R + i] = ex2;
-            ex2 += loc[i];
+            if (i * BRM_WAVE >= nout) break;  // uniform
+            const int q = i * BRM_WAVE + lane;
+            const int c = b.u.c.cnt[q];
+            const int inc = wave_incl_scan_dpp(c);
+            b.u.c.cnt[q] = carry + inc - c;
+            carry += __shfl(inc, BRM_WAVE - 1, BRM_WAVE);
         }
     }
     wave_lds_sync();
