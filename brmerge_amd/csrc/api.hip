@@ -460,8 +460,9 @@ int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_shor
         HIPC(launch_short_count(p.rows(BIN_TINY), p.bc[BIN_TINY], p.a, p.rowcnt, s, 32));
         HIPC(launch_short_count(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, p.rowcnt, s));
     }
-    HIPC(launch_wmed_count(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, p.rowcnt, s, std::min(p.wmed_cap, WMED_SPLIT)));
-    HIPC(launch_wmed_count(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, p.rowcnt, s, p.wmed_cap));
+    HIPC(launch_wmed_count(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, p.rowcnt, s, std::min(p.wmed_cap, WMED_SPLIT0)));
+    HIPC(launch_wmed_count(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, p.rowcnt, s, std::min(p.wmed_cap, WMED_SPLIT)));
+    HIPC(launch_wmed_count(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, p.rowcnt, s, p.wmed_cap));
     HIPC(launch_medium_count(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, p.rowcnt, s));
     HIPC(launch_long_count(p.rows(BIN_LONG), p.bc[BIN_LONG], p.a, p.rowcnt, s));
     if (clk) clk->begin(P_PNNZ);
@@ -493,8 +494,9 @@ int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t
         HIPC(launch_short_numeric(p.rows(BIN_TINY), p.bc[BIN_TINY], p.a, o, s, 32));
         HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
     }
-    HIPC(launch_wmed_numeric(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
-    HIPC(launch_wmed_numeric(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, o, s, p.wmed_cap));
+    HIPC(launch_wmed_numeric(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT0)));
+    HIPC(launch_wmed_numeric(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
+    HIPC(launch_wmed_numeric(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, o, s, p.wmed_cap));
     HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
     return BRM_OK;
 }
@@ -515,7 +517,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     BRMC(pattern_prepare(in, s, &clk, p));
     const int M = in.M;
     for (int b = 0; b < 4; ++b) t_stats.rows_by_tier[b] = p.bc[b];
-    t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_WMED] + p.bc[BIN_WMED2];
+    t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_WMED] + p.bc[BIN_WMED2] + p.bc[BIN_WMED3];
     t_stats.rows_by_tier[BIN_SHORT] += p.bc[BIN_TINY];
     t_stats.total_ip = p.total_ip;
     Scratch& sc = p.sc;
@@ -556,8 +558,9 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         o.rowlen = p.rowcnt;
         HIPC(launch_short_numeric(p.rows(BIN_TINY), p.bc[BIN_TINY], p.a, o, s, 32));
         HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
-        HIPC(launch_wmed_numeric(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
-        HIPC(launch_wmed_numeric(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, o, s, p.wmed_cap));
+        HIPC(launch_wmed_numeric(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT0)));
+        HIPC(launch_wmed_numeric(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
+        HIPC(launch_wmed_numeric(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, o, s, p.wmed_cap));
         HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
         BRMC(run_long(p.a, p.rows(BIN_LONG), p.bc[BIN_LONG], o, nullptr, s, sc));
         BRMC(check_dev_err(p.derr, s, "numeric"));

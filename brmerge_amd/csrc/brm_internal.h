@@ -15,10 +15,11 @@ enum RowBin : int {
     BIN_SHORT = 1,   // ip <= 64: one wavefront per row, merge in registers
     BIN_MEDIUM = 2,  // ip <= MED_CAP and L <= MED_LMAX: one workgroup per row, merge in LDS
     BIN_LONG = 3,    // otherwise: LDS leaf subtrees + HBM-streamed upper tree levels
-    BIN_WMED = 4,    // ip <= min(wmed_cap, WMED_SPLIT) and L <= WMED_LMAX: one wavefront per row
+    BIN_WMED = 4,    // ip <= min(wmed_cap, WMED_SPLIT0) and L <= WMED_LMAX: one wavefront per row
     BIN_TINY = 5,    // ip <= TINY_MAX: two rows per wavefront, merge in registers
-    BIN_WMED2 = 6,   // WMED_SPLIT < ip <= wmed_cap, L <= WMED_LMAX: the wavefront tier's larger rows
-    BIN_COUNT = 7
+    BIN_WMED2 = 6,   // WMED_SPLIT0 < ip <= min(wmed_cap, WMED_SPLIT), L <= WMED_LMAX
+    BIN_WMED3 = 7,   // WMED_SPLIT < ip <= wmed_cap, L <= WMED_LMAX
+    BIN_COUNT = 8
 };
 
 constexpr int SHORT_MAX = 64;
@@ -28,8 +29,9 @@ constexpr int TINY_MAX = 32;
 // per CU instead of one 256-thread workgroup per row).
 constexpr int WMED_LMAX = 64;
 constexpr int WMED_CAP = 1024;
-// The wavefront tier runs its rows in two launches split at this many
-// products, so that the smaller rows get the smaller kernel shape.
+// The wavefront tier runs its rows in three launches split at these
+// product counts, so that the smaller rows get the smaller kernel shapes.
+constexpr int WMED_SPLIT0 = 256;
 constexpr int WMED_SPLIT = 512;
 
 // LDS tree-merge configurations (threads, element capacity, list capacity).
