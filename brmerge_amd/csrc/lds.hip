@@ -378,7 +378,10 @@ hipError_t launch_wmed_count(const int* rows, int n, const RowArgs& a, int64_t* 
         const int64_t ch = launch_chunk(BRM_WAVE);
         for (int64_t b = 0; b < n; b += ch) {
             const int c = (int)std::min<int64_t>(ch, n - b);
-            k_sym_hash<BRM_WAVE, 2 * WMED_CAP, WMED_LMAX><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+            // table of twice the row's largest product count (smaller LDS, more waves)
+            if (wmed_cap <= 256) k_sym_hash<BRM_WAVE, 512, WMED_LMAX><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+            else if (wmed_cap <= 512) k_sym_hash<BRM_WAVE, 1024, WMED_LMAX><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+            else k_sym_hash<BRM_WAVE, 2 * WMED_CAP, WMED_LMAX><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
         }
     }
     return hipGetLastError();
