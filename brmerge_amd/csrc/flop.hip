@@ -20,19 +20,34 @@ namespace {
 
 constexpr int NT = 256;
 
-__global__ void k_validate_rpt(const int* __restrict__ rpt, int n, int64_t nnz, int* err) {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
-    if (i > n) return;
-    const int v = rpt[i];
-    bool bad = (i == 0 && v != 0) || (i == n && (int64_t)v != nnz) || (i < n && v > rpt[i + 1]);
+// All range checks in one launch: A's and B's row pointers (first 0, last
+// nnz, nondecreasing), then A's columns in [0, K) and B's in [0, N).  B is
+// skipped when nnzB < 0.  Grid-stride over the concatenated index space.
+__global__ __launch_bounds__(NT) void k_validate_all(RowArgs a, int64_t nnzA, int64_t nnzB, int* err) {
+    const int64_t n0 = (int64_t)a.M + 1;
+    const int64_t n1 = nnzB >= 0 ? (int64_t)a.K + 1 : 0;
+    const int64_t n2 = nnzA;
+    const int64_t n3 = nnzB > 0 ? nnzB : 0;
+    const int64_t total = n0 + n1 + n2 + n3;
+    bool bad = false;
+    for (int64_t g = (int64_t)blockIdx.x * NT + threadIdx.x; g < total; g += (int64_t)gridDim.x * NT) {
+        if (g < n0) {
+            const int64_t i = g;
+            const int v = a.arpt[i];
+            bad |= (i == 0 && v != 0) || (i == a.M && (int64_t)v != nnzA) || (i < a.M && v > a.arpt[i + 1]);
+        } else if (g < n0 + n1) {
+            const int64_t i = g - n0;
+            const int v = a.brpt[i];
+            bad |= (i == 0 && v != 0) || (i == a.K && (int64_t)v != nnzB) || (i < a.K && v > a.brpt[i + 1]);
+        } else if (g < n0 + n1 + n2) {
+            const int v = a.acol[g - n0 - n1];
+            bad |= v < 0 || v >= a.K;
+        } else {
+            const int v = a.bcol[g - n0 - n1 - n2];
+            bad |= v < 0 || v >= a.N;
+        }
+    }
     if (bad) atomicOr(err, (int)DERR_INVALID);
-}
-
-__global__ void k_validate_range(const int* __restrict__ col, int64_t nnz, int limit, int* err) {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
-    if (i >= nnz) return;
-    const int v = col[i];
-    if (v < 0 || v >= limit) atomicOr(err, (int)DERR_INVALID);
 }
 
 // Strictly increasing columns within each row; one wavefront per row.
@@ -130,23 +145,12 @@ inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per
 }  // namespace
 
 hipError_t launch_validate(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* err, hipStream_t s) {
-    // Row pointers first: the sortedness check indexes columns through them.
-    // nnzB < 0: B was validated before and is skipped.
+    // Row pointers are checked before the sortedness check (a later launch)
+    // indexes columns through them.  nnzB < 0: B was validated before.
+    const int64_t total = (int64_t)a.M + 1 + (nnzB >= 0 ? (int64_t)a.K + 1 + nnzB : 0) + nnzA;
     {
-        ProfScope ps_("k_validate_rpt", s);
-        k_validate_rpt<<<nblk((int64_t)a.M + 1, NT), NT, 0, s>>>(a.arpt, a.M, nnzA, err);
-    }
-    if (nnzB >= 0) {
-        ProfScope ps_("k_validate_rpt", s);
-        k_validate_rpt<<<nblk((int64_t)a.K + 1, NT), NT, 0, s>>>(a.brpt, a.K, nnzB, err);
-    }
-    if (nnzA > 0) {
-        ProfScope ps_("k_validate_range", s);
-        k_validate_range<<<nblk(nnzA, NT), NT, 0, s>>>(a.acol, nnzA, a.K, err);
-    }
-    if (nnzB > 0) {
-        ProfScope ps_("k_validate_range", s);
-        k_validate_range<<<nblk(nnzB, NT), NT, 0, s>>>(a.bcol, nnzB, a.N, err);
+        ProfScope ps_("k_validate", s);
+        k_validate_all<<<(unsigned)std::min<int64_t>(nblk(total, NT), 8192), NT, 0, s>>>(a, nnzA, nnzB, err);
     }
     return hipGetLastError();
 }

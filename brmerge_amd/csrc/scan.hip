@@ -87,6 +87,30 @@ __global__ __launch_bounds__(SCAN_NT) void k_scan_down(F f, int64_t n, const int
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = part[nb];
 }
 
+// Down pass that also forms its own block's carry: the sum of the tile
+// totals part[0 .. blockIdx.x), reduced by the block (for up to
+// SCAN_FUSE_TILES tiles, so that this costs a few loads per thread).  Saves
+// the one-block scan of the tile totals and its launch.
+constexpr int64_t SCAN_FUSE_TILES = 8192;
+template <typename F>
+__global__ __launch_bounds__(SCAN_NT) void k_scan_down_fused(F f, int64_t n, const int64_t* part, int64_t* out) {
+    __shared__ int64_t sm[SCAN_NT / BRM_WAVE + 1];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int64_t pre = 0;
+    for (int64_t b = threadIdx.x; b < blockIdx.x; b += SCAN_NT) pre += part[b];
+    int64_t carry = block_sum<SCAN_NT>(pre, sm);
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        int64_t idx = base + it * SCAN_NT + threadIdx.x;
+        if (base + it * SCAN_NT >= n) break;  // uniform across the block
+        int64_t x = idx < n ? f(idx) : 0;
+        int64_t tot;
+        int64_t ex = block_excl_scan<SCAN_NT>(x, sm, &tot);
+        if (idx < n) out[idx] = carry + ex;
+        carry += tot;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = carry;
+}
+
 __global__ void k_zero1(int64_t* out) { out[0] = 0; }
 
 template <typename F>
@@ -105,11 +129,14 @@ hipError_t scan_impl(F f, int64_t n, int64_t* out, hipStream_t s) {
         ProfScope ps_("k_scan_reduce", s);
         k_scan_reduce<F><<<(unsigned)nb, SCAN_NT, 0, s>>>(f, n, part);
     }
-    {
-        ProfScope ps_("k_scan_partials", s);
-        k_scan_partials<<<1, SCAN_NT, 0, s>>>(part, nb);
-    }
-    {
+    if (nb <= SCAN_FUSE_TILES) {
+        ProfScope ps_("k_scan_down", s);
+        k_scan_down_fused<F><<<(unsigned)nb, SCAN_NT, 0, s>>>(f, n, part, out);
+    } else {
+        {
+            ProfScope ps_("k_scan_partials", s);
+            k_scan_partials<<<1, SCAN_NT, 0, s>>>(part, nb);
+        }
         ProfScope ps_("k_scan_down", s);
         k_scan_down<F><<<(unsigned)nb, SCAN_NT, 0, s>>>(f, n, part, nb, out);
     }
