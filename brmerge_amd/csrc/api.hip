@@ -435,7 +435,7 @@ int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int force_lo
     return BRM_OK;
 }
 
-int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_short) {
+int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_short, bool read_nnz) {
     if (clk) clk->begin(P_SYM);
     const int64_t n32 = (int64_t)p.bc[BIN_TINY] * 32, n64 = (int64_t)p.bc[BIN_SHORT] * 64;
     p.stashed = stash_short && n32 + n64 > 0 && 12 * (n32 + n64) <= SHORT_STASH_MAX_BYTES;
@@ -467,7 +467,8 @@ int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_shor
     HIPC(launch_long_count(p.rows(BIN_LONG), p.bc[BIN_LONG], p.a, p.rowcnt, s));
     if (clk) clk->begin(P_PNNZ);
     HIPC(scan_i64(p.rowcnt, p.a.M, p.crpt, s));
-    BRMC(d2h(&p.nnzC, p.crpt + p.a.M, sizeof(int64_t), s));
+    p.nnzC = -1;
+    if (read_nnz) BRMC(d2h(&p.nnzC, p.crpt + p.a.M, sizeof(int64_t), s));
     return BRM_OK;
 }
 
@@ -526,22 +527,36 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     int64_t nnzC = 0;
 
     if (!upper) {
-        BRMC(pattern_symbolic(p, s, &clk, true));
+        // Small products size C by the product count (an upper bound of
+        // nnz(C), at most SMALL_IP elements) and read nnz(C) with the
+        // numeric phase's error flag: one host round trip less.
+        constexpr int64_t SMALL_IP = (int64_t)1 << 24;
+        const bool small = p.total_ip <= SMALL_IP && p.total_ip <= max_nnz;
+        BRMC(pattern_symbolic(p, s, &clk, true, !small));
         nnzC = p.nnzC;
-        if (nnzC > max_nnz) {
+        if (!small && nnzC > max_nnz) {
             set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
             return BRM_E_OVERFLOW;
         }
+        const int64_t cap = small ? p.total_ip : nnzC;
         clk.begin(P_ALLOC);
-        BRMC(sc.alloc(&ccol, nnzC, "C col"));
-        BRMC(sc.alloc(&cval, nnzC, "C val"));
-        tr.mark("C allocated", nnzC);
+        BRMC(sc.alloc(&ccol, cap, "C col"));
+        BRMC(sc.alloc(&cval, cap, "C val"));
+        tr.mark("C allocated", cap);
         clk.begin(P_COMPUTE);
         BRMC(pattern_numeric_tiers(p, ccol, cval, s));
         tr.mark("short+medium numeric");
         BRMC(pattern_numeric_long(p, ccol, cval, s));
         tr.mark("long numeric");
-        BRMC(check_dev_err(p.derr, s, "numeric"));
+        if (small) {
+            int e = 0;
+            HIPC(hipMemcpyAsync(&e, p.derr, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPC(hipMemcpyAsync(&nnzC, p.crpt + M, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            HIPC(hipStreamSynchronize(s));
+            BRMC(dev_err_status(e, "numeric"));
+        } else {
+            BRMC(check_dev_err(p.derr, s, "numeric"));
+        }
     } else {
         clk.begin(P_ALLOC);
         int* bcol_bar;

@@ -141,7 +141,8 @@ int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int force_lo
 // (their lengths are the counts) when the stash fits SHORT_STASH_MAX_BYTES;
 // the numeric phase then copies them.  A Pattern whose values change
 // between numeric passes must not stash.
-int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_short = false);
+// With read_nnz false, nnz(C) stays on the device (p.nnzC = -1).
+int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_short = false, bool read_nnz = true);
 // Numeric phase of BRMerge-Precise into C (ccol/cval of nnz(C) elements).
 // The short and medium tiers only enqueue kernels (capturable in a graph);
 // the long tier needs host round trips between its launches.
