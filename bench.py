@@ -357,8 +357,8 @@ def main():
                     help="only run the library's plain copy kernel over GIB GiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); multi-GPU sharding is "
-              f"described in DESIGN.md but not built", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): the product would shard by rows "
+              f"of C across GPUs, but only the one-GPU path is built", file=sys.stderr)
         sys.exit(2)
 
     import brmerge_amd as bm

@@ -104,6 +104,22 @@ def test_rmat(gpu, engine, scale):
     assert_csr_equal(gpu.brmerge_upper(A, A), R)
 
 
+def test_flop_count_many_tiles(gpu):
+    """Product count of an A with 40M entries: the prefix sum takes its
+    three-kernel path (more tiles than the down pass sums by itself), and
+    the total matches numpy."""
+    from types import SimpleNamespace
+    rng = np.random.default_rng(5)
+    M, per, K = 10_000_000, 4, 1000
+    blen = rng.integers(0, 6, size=K)
+    brpt = np.concatenate([[0], np.cumsum(blen)]).astype(np.int32)
+    acol = rng.integers(0, K, size=M * per).astype(np.int32)
+    arpt = (np.arange(M + 1, dtype=np.int64) * per).astype(np.int32)
+    A = SimpleNamespace(rpt=arpt, col=acol, M=M)
+    B = SimpleNamespace(rpt=brpt, M=K)
+    assert gpu.compute_flop(A, B) == int(blen[acol].sum())
+
+
 def test_amg_chain(gpu):
     n = 21
     S, P = gpu.gen.stencil27(n), gpu.gen.aggregation(n)
