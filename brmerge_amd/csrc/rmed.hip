@@ -2,11 +2,13 @@
 // products held in registers (R per lane, ip <= 64 R <= 1024, at most 64 lists).
 //
 //   1. every lane l < L holds list l's product offset, B row start and A
-//      value; each product locates its list with a shuffle binary search,
-//      loads its B entry and forms key = (column, list position, index)
-//      and value = a * b (rounded, BRMerge-SpGEMM/inc/spgemm.h:621).  Keys
-//      are 32-bit when the column fits beside 6 bits of list position and
-//      log2(64 R) bits of index (half the shuffles and compares), else 64-bit;
+//      value, and writes the B index and list position of each of its
+//      list's products to an LDS map; each product reads its entry, loads
+//      its B entry and forms key = (column, list position, index) and
+//      value = a * b (rounded, BRMerge-SpGEMM/inc/spgemm.h:621).  Keys are
+//      32-bit when the column fits beside 6 bits of list position and
+//      log2(64 R) bits of index (half the shuffles and compares), else 64-bit.
+//   Sort variant:
 //   2. a bitonic sort over the 64 R keys (cross-lane steps with shuffles,
 //      intra-lane steps between registers) orders the products by column,
 //      and within a column by list position;
@@ -16,8 +18,12 @@
 //      child passes it through, spgemm.h:639-707) -- a single pass that
 //      keeps one pending left subtree per height;
 //   4. run heads are compacted with ballots and written to C (Precise), or
-//      to C-bar with the row length (Upper).  (The symbolic count of these
-//      rows uses the wavefront hash kernel of lds.hip.)
+//      to C-bar with the row length (Upper).
+//   Bitmap variant (B of at most RB_SPAN columns, R >= 8): the columns set
+//   bits of an LDS bitmap over the row's column span, popcount ranks give
+//   each product its output position, products are bucketed by position and
+//   each bucket is folded in list-position order -- no sort.
+// The symbolic count of these rows uses the wavefront hash kernel of lds.hip.
 #include <limits.h>
 
 #include <algorithm>
