@@ -1,8 +1,10 @@
 // Short-row tier: rows with at most 64 intermediate products, one
-// wavefront per row, everything in registers.
+// wavefront per row (two per wavefront, 32 lanes each, for rows of at most
+// 32 products), everything in registers.
 //
 // Each lane holds one product (column, list position, value).  A rank sort
-// by (column, position) over the wavefront (v_readlane broadcasts) places the
+// by (column, position) over the row's lanes (v_readlane broadcasts, or
+// shuffles inside a half-wavefront) places the
 // products in column order with equal columns adjacent and ordered by
 // position; ds_permute moves every lane's product to its rank.  The merge tree
 // of BRMerge (BRMerge-SpGEMM/inc/spgemm.h:639-707: lists paired (2j, 2j+1)
