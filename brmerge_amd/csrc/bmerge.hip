@@ -92,10 +92,15 @@ struct WsCols {
     double* val1;
 };
 
+// Minimum over the wavefront (DPP prefix minimum, then the last lane).
 __device__ __forceinline__ int wave_min(int x) {
-#pragma unroll
-    for (int d = BRM_WAVE / 2; d > 0; d >>= 1) x = min(x, __shfl_xor(x, d, BRM_WAVE));
-    return x;
+    x = min(x, dpp_mov<0x111, 0xF>(x, INT_MAX));  // row_shr:1
+    x = min(x, dpp_mov<0x112, 0xF>(x, INT_MAX));  // row_shr:2
+    x = min(x, dpp_mov<0x114, 0xF>(x, INT_MAX));  // row_shr:4
+    x = min(x, dpp_mov<0x118, 0xF>(x, INT_MAX));  // row_shr:8
+    x = min(x, dpp_mov<0x142, 0xA>(x, INT_MAX));  // row_bcast15 into rows 1 and 3
+    x = min(x, dpp_mov<0x143, 0xC>(x, INT_MAX));  // row_bcast31 into rows 2 and 3
+    return wave_last(x);
 }
 
 // Block reductions and scans with one barrier each.  Every call takes the
@@ -216,11 +221,12 @@ __device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const R
     const int pc = lane < BG_W / 64 ? __popcll(g.wstart[lane]) : 0;
     const int before = wave_incl_scan(pc) - pc;  // window starts in earlier words
     const unsigned long long upto = lane == BRM_WAVE - 1 ? ~0ull : (2ull << lane) - 1ull;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / BRM_WAVE));
 #pragma unroll
     for (int i = 0; i < BG_IPT; ++i) {
         const int e = bg_elem(i);
-        const int wd = e >> 6;
-        const int r = __shfl(before, wd, BRM_WAVE) + __popcll(g.wstart[wd] & upto) - 1;
+        const int wd = wv * BG_IPT + i;  // e >> 6, the same for the whole wavefront
+        const int r = __builtin_amdgcn_readlane(before, wd) + __popcll(g.wstart[wd] & upto) - 1;
         // every register is written, so nothing of the previous round stays live
         cj[i] = -1;
         ci[i] = 0;
@@ -392,8 +398,10 @@ __device__ void bg_wave_sum2(const BgSmem& g, int s0, int m0, int s1, int m1, do
     dpp_level<0x118, 8>(x, px, lane);   // row_shr:8
     dpp_level<0x142, 16>(x, px, lane);  // row_bcast15: lanes 31, 63 get 15, 47
     wave_lds_sync();
-    r0 = __shfl(x, 31, BRM_WAVE);
-    r1 = __shfl(x, 63, BRM_WAVE);
+    r0 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), 31),
+                          __builtin_amdgcn_readlane(__double2loint(x), 31));
+    r1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), 63),
+                          __builtin_amdgcn_readlane(__double2loint(x), 63));
 }
 
 __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ tasks,

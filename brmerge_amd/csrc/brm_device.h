@@ -1,13 +1,15 @@
 // Device-side building blocks shared by the BRMerge kernels (gfx950, wave64).
 //
-// Everything here is plain HIP C++ for CDNA4: 64-lane wavefronts, LDS scans,
-// ds_bpermute/ds_permute lane exchange.  Value arithmetic uses explicit
+// Everything here is plain HIP C++ for CDNA4: 64-lane wavefronts, DPP lane
+// moves for wave scans, LDS for block scans.  Value arithmetic uses explicit
 // round-to-nearest multiply and add (no contraction into FMA), because the
 // reference stores each product aval*bval before any add
 // (BRMerge-SpGEMM/inc/spgemm.h:621,653).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #define BRM_WAVE 64
 
@@ -19,22 +21,57 @@ __device__ __forceinline__ double fadd(double a, double b) { return __dadd_rn(a,
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
 // ---------------------------------------------------------------- wave scans
+// DPP lane moves (no LDS round trips).  Lane i takes the value of lane i - s
+// of its row of 16 lanes (row_shr:s), or lane 15 / lane 31 (row_bcast15 /
+// row_bcast31, written only into the rows of ROWS); lanes without a source
+// take `id`.  Callers run with the whole wavefront active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_mov(int x, int id) {
+    return __builtin_amdgcn_update_dpp(id, x, CTRL, ROWS, 0xF, false);
+}
+
+template <int CTRL, int ROWS, typename T>
+__device__ __forceinline__ T dpp_take(T x) {
+    if constexpr (sizeof(T) == 4) {
+        return (T)dpp_mov<CTRL, ROWS>((int)x, 0);
+    } else {
+        const uint64_t u = (uint64_t)x;
+        const uint32_t lo = (uint32_t)dpp_mov<CTRL, ROWS>((int)(uint32_t)u, 0);
+        const uint32_t hi = (uint32_t)dpp_mov<CTRL, ROWS>((int)(uint32_t)(u >> 32), 0);
+        return (T)(((uint64_t)hi << 32) | lo);
+    }
+}
+
+// Inclusive sum over the lanes: Hillis-Steele within rows of 16, then the
+// row totals are carried by the two row broadcasts.
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T x) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < BRM_WAVE; d <<= 1) {
-        T y = __shfl_up(x, d, BRM_WAVE);
-        if (lane >= d) x += y;
-    }
+    static_assert(std::is_integral<T>::value && (sizeof(T) == 4 || sizeof(T) == 8), "integer scans");
+    x += dpp_take<0x111, 0xF>(x);  // row_shr:1
+    x += dpp_take<0x112, 0xF>(x);  // row_shr:2
+    x += dpp_take<0x114, 0xF>(x);  // row_shr:4
+    x += dpp_take<0x118, 0xF>(x);  // row_shr:8
+    x += dpp_take<0x142, 0xA>(x);  // row_bcast15 into rows 1 and 3
+    x += dpp_take<0x143, 0xC>(x);  // row_bcast31 into rows 2 and 3
     return x;
+}
+
+// Value of the last lane, in every lane.
+template <typename T>
+__device__ __forceinline__ T wave_last(T x) {
+    if constexpr (sizeof(T) == 4) {
+        return (T)__builtin_amdgcn_readlane((int)x, BRM_WAVE - 1);
+    } else {
+        const uint64_t u = (uint64_t)x;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, BRM_WAVE - 1);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), BRM_WAVE - 1);
+        return (T)(((uint64_t)hi << 32) | lo);
+    }
 }
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T x) {
-#pragma unroll
-    for (int d = BRM_WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, BRM_WAVE);
-    return x;
+    return wave_last(wave_incl_scan(x));
 }
 
 // ---------------------------------------------------------------- block scans
