@@ -79,6 +79,7 @@ struct BgSmem {
     unsigned short mid[BG_MIDMAX];  // outputs left to the second thread pass
     int nbig;
     int nmid;
+    int bnext;  // next big bucket to claim
     int64_t red64[BG_NSLOT][BG_NW];
     int red[BG_NSLOT][BG_NW];
 };
@@ -429,6 +430,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
     if (t == 0) {
         g.nbig = 0;
         g.nmid = 0;
+        g.bnext = 0;
     }
 #pragma unroll
     for (int i = 0; i < BG_WPT; ++i) g.bm[t * BG_WPT + i] = 0ull;
@@ -739,7 +741,8 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             // wave w sums two buckets at a time densely in the first 512
             // words of its share of the bitmap, with flags in bytes
             // [512 w, 512 w + 512) of the ranks, then clears its share of the
-            // bitmap
+            // bitmap.  The pairs are claimed from a shared counter, so the
+            // waves with no second-pass buckets above take more of them.
             static_assert(2 * BG_KMAX <= BG_WORDS / BG_NW, "dense sums fit a wave's bitmap share");
             static_assert(BG_NW * 2 * BG_KMAX <= (int)sizeof(BgSmem::wrank), "flags fit the word ranks");
             const int wv = t / BRM_WAVE, lane = lane_id();
@@ -751,7 +754,11 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
                 reinterpret_cast<unsigned*>(pres)[lane] = 0u;
                 reinterpret_cast<unsigned*>(pres)[BRM_WAVE + lane] = 0u;
                 wave_lds_sync();
-                for (int b = 2 * wv; b < nbig; b += 2 * BG_NW) {
+                for (;;) {
+                    int b = 0;
+                    if (lane == 0) b = atomicAdd(&g.bnext, 2);
+                    b = __builtin_amdgcn_readfirstlane(b);  // lane 0's claim
+                    if (b >= nbig) break;
                     const int oi0 = g.big[b];
                     const int oi1 = b + 1 < nbig ? g.big[b + 1] : -1;
                     const int s0 = oi0 ? g.cnt[oi0 - 1] : 0;
@@ -776,6 +783,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         if (t == 0) {  // read by every wave before the barrier above
             g.nbig = 0;
             g.nmid = 0;
+            g.bnext = 0;
         }
         mark(7);
     }
