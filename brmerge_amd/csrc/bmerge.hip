@@ -69,11 +69,10 @@ struct BgSmem {
     unsigned char wchild[BG_KMAX];         // the children with windows, in order
     int lastc[BG_KMAX];     // last loaded column of the child
     int xoff[BG_KMAX + 1];  // top-up offsets (relative to the fill level)
-    int xamt[BG_KMAX];
     int take[BG_KMAX];
     int hist[32];
-    unsigned char kind[BG_KMAX];
-    unsigned char buf[BG_KMAX];
+    int64_t fbase[BG_KMAX];  // off + cur - woff: element e of the child's window is at fbase + e
+    unsigned char src[BG_KMAX];  // SRC_WS0 / SRC_WS1 / SRC_B
     unsigned char lid[BG_KMAX];
     unsigned short big[BG_BIGMAX];  // outputs left to the wavefront sums
     unsigned short mid[BG_MIDMAX];  // outputs left to the second thread pass
@@ -187,6 +186,7 @@ __device__ __forceinline__ bool bg_windows(BgSmem& g, Slots& sl, int k) {
     if (t < k) {
         g.w[t] = wi;
         g.woff[t] = ex & 0xFFFF;
+        g.fbase[t] = g.off[t] + g.cur[t] - (ex & 0xFFFF);
         if (wi > 0) {
             g.wchild[ex >> 16] = (unsigned char)t;
             atomicOr(&g.wstart[(ex & 0xFFFF) >> 6], 1ull << (ex & 63));
@@ -196,26 +196,28 @@ __device__ __forceinline__ bool bg_windows(BgSmem& g, Slots& sl, int k) {
     return R == 0;
 }
 
-// Global source of child j's elements.
+// Global source of a child's elements: a workspace buffer (a node of the
+// previous pass) or a row of B read in place and scaled.
+constexpr unsigned char SRC_WS0 = 0, SRC_WS1 = 1, SRC_B = 2;
 struct SrcPtr {
     const int* c;
     const double* v;
     bool virt;
 };
-__device__ __forceinline__ SrcPtr src_of(const BgSmem& g, int j, const RowArgs& a, const WsCols& ws) {
-    const bool virt = g.kind[j] == SRC_VIRTUAL;
-    return SrcPtr{virt ? a.bcol : (g.buf[j] ? ws.col1 : ws.col0), virt ? a.bval : (g.buf[j] ? ws.val1 : ws.val0),
-                  virt};
+__device__ __forceinline__ SrcPtr src_ptr(unsigned char sk, const RowArgs& a, const WsCols& ws) {
+    const bool virt = sk == SRC_B;
+    return SrcPtr{virt ? a.bcol : (sk == SRC_WS1 ? ws.col1 : ws.col0),
+                  virt ? a.bval : (sk == SRC_WS1 ? ws.val1 : ws.val0), virt};
 }
 
 // Loads this thread's elements of the proportional windows (buffer
-// positions bg_elem(i)) into registers; cj = child, ci = index in the
-// child's window.  The child of element e is the last window starting at or
-// before e: its rank is the number of wstart bits up to e, less one (the 64
-// elements of one step of a wavefront share one wstart word).
+// positions bg_elem(i)) into registers; cj = child.  The child of element e
+// is the last window starting at or before e: its rank is the number of
+// wstart bits up to e, less one (the 64 elements of one step of a wavefront
+// share one wstart word).
 __device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const RowArgs& a, const WsCols& ws,
                                          int (&rc)[BG_IPT], double (&rv)[BG_IPT], double (&rs)[BG_IPT],
-                                         int (&cj)[BG_IPT], int (&ci)[BG_IPT], unsigned& vmask) {
+                                         int (&cj)[BG_IPT], unsigned& vmask) {
     static_assert(BG_W / 64 <= BRM_WAVE, "one wstart word per lane");
     vmask = 0u;
     const int lane = lane_id();
@@ -230,21 +232,18 @@ __device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const R
         const int r = __builtin_amdgcn_readlane(before, wd) + __popcll(g.wstart[wd] & upto) - 1;
         // every register is written, so nothing of the previous round stays live
         cj[i] = -1;
-        ci[i] = 0;
         rc[i] = 0;
         rv[i] = 0.0;
         rs[i] = 0.0;
         if (e < Wn) {
             const int j = g.wchild[r];
-            const int q = e - g.woff[j];
-            const SrcPtr p = src_of(g, j, a, ws);
-            const int64_t x = g.off[j] + g.cur[j] + q;
+            const SrcPtr p = src_ptr(g.src[j], a, ws);
+            const int64_t x = g.fbase[j] + e;
             rc[i] = p.c[x];
             rv[i] = p.v[x];
             rs[i] = g.scale[j];
             if (p.virt) vmask |= 1u << i;
             cj[i] = j;
-            ci[i] = q;
         }
     }
 }
@@ -416,8 +415,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
     Slots sl;
     if (t < k) {
         const Child c = children[T.cbase + t];
-        g.kind[t] = (unsigned char)c.src.kind;
-        g.buf[t] = (unsigned char)c.src.buf;
+        g.src[t] = c.src.kind == SRC_VIRTUAL ? SRC_B : c.src.buf ? SRC_WS1 : SRC_WS0;
         g.lid[t] = (unsigned char)c.lid;
         g.off[t] = c.src.off;
         g.scale[t] = c.src.scale;
@@ -450,13 +448,13 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
     }
     const int64_t in_total = block_sum_l(g, sl, t < k ? g.len[t] : 0);
 
-    int rc[BG_IPT], cj[BG_IPT], ci[BG_IPT];
+    int rc[BG_IPT], cj[BG_IPT];
     double rv[BG_IPT], rs[BG_IPT];
     unsigned vmask = 0u;
     bool done = bg_windows(g, sl, k);
     __syncthreads();
     int Wn = g.woff[k];
-    if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, ci, vmask);
+    if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, vmask);
     int64_t out_n = 0;
     int rounds = 0;
     // shader cycles per round phase, seen by thread 0 (barrier waits included)
@@ -485,10 +483,12 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
                 g.val[e] = ((vmask >> i) & 1u) ? fmul(rs[i], rv[i]) : rv[i];
                 g.cidx[e] = (unsigned char)cj[i];
                 g.elid[e] = g.lid[cj[i]];
-                if (ci[i] == g.w[cj[i]] - 1) g.lastc[cj[i]] = rc[i];
             }
         }
         __syncthreads();
+        // last loaded column of every child with a window (read by the
+        // child's own thread until the next barrier)
+        if (t < k && g.w[t] > 0) g.lastc[t] = g.col[g.woff[t] + g.w[t] - 1];
         // the window starts were read by the fetch of this round's windows,
         // before the barrier above; the next windows set them in phase 5
         if (t < BG_W / 64) g.wstart[t] = 0ull;
@@ -523,32 +523,27 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             const int amt = shorty ? (int)min((int64_t)add, rem - g.w[t]) : 0;
             int tot;
             const int ex = block_scan_i(g, sl, amt, &tot);
-            if (t < k) {
-                g.xoff[t] = ex;
-                g.xamt[t] = amt;
-            }
+            if (t < k) g.xoff[t] = ex;
             if (t == BG_NT - 1) g.xoff[k] = tot;
             __syncthreads();
             // tot <= BG_W - filled: at most BG_IPT top-up elements per
             // thread; all loads are issued before the first is used
-            int tc[BG_IPT], tj[BG_IPT], tq[BG_IPT];
+            int tc[BG_IPT], tj[BG_IPT];
             double tv[BG_IPT];
 #pragma unroll
             for (int i = 0; i < BG_IPT; ++i) {
                 const int e = t + i * BG_NT;
                 tc[i] = 0;
                 tj[i] = 0;
-                tq[i] = 0;
                 tv[i] = 0.0;
                 if (e < tot) {
                     const int j = upper_idx(g.xoff, 0, k, e);
                     const int q = e - g.xoff[j];
-                    const SrcPtr p = src_of(g, j, a, ws);
+                    const SrcPtr p = src_ptr(g.src[j], a, ws);
                     const int64_t x = g.off[j] + g.cur[j] + g.w[j] + q;
                     tc[i] = p.c[x];
                     tv[i] = p.v[x];
                     tj[i] = j;
-                    tq[i] = q;
                 }
             }
 #pragma unroll
@@ -558,14 +553,16 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
                     const int j = tj[i];
                     const int pos = filled + e;
                     g.col[pos] = tc[i];
-                    g.val[pos] = g.kind[j] == SRC_VIRTUAL ? fmul(g.scale[j], tv[i]) : tv[i];
+                    g.val[pos] = g.src[j] == SRC_B ? fmul(g.scale[j], tv[i]) : tv[i];
                     g.cidx[pos] = (unsigned char)j;
                     g.elid[pos] = g.lid[j];
-                    if (tq[i] == g.xamt[j] - 1) g.lastc[j] = tc[i];
                 }
             }
             __syncthreads();
-            if (t < k) g.w[t] += amt;
+            if (t < k && amt > 0) {
+                g.lastc[t] = g.col[filled + ex + amt - 1];
+                g.w[t] += amt;
+            }
             filled += tot;
         }
         mark(1);
@@ -675,6 +672,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             if (t < k) {
                 g.w[t] = wi;
                 g.woff[t] = wo;
+                g.fbase[t] = g.off[t] + g.cur[t] - wo;
                 if (wi > 0) {
                     g.wchild[(int)(ex >> 32)] = (unsigned char)t;
                     atomicOr(&g.wstart[wo >> 6], 1ull << (wo & 63));
@@ -694,7 +692,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
                 g.contrib[slot] = (unsigned short)bg_elem(i);
             }
         }
-        if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, ci, vmask);
+        if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, vmask);
         __syncthreads();
         mark(6);
         // ---- one output per thread (strided: coalesced stores) for one
