@@ -57,7 +57,6 @@ struct BgSmem {
     int cnt[BG_W];
     unsigned short wrank[BG_WORDS];
     unsigned short contrib[BG_W];
-    unsigned char cidx[BG_W];
     unsigned char elid[BG_W];       // list id (node id) of the element's child
     int64_t off[BG_KMAX];
     int64_t len[BG_KMAX];
@@ -69,7 +68,6 @@ struct BgSmem {
     unsigned char wchild[BG_KMAX];         // the children with windows, in order
     int lastc[BG_KMAX];     // last loaded column of the child
     int xoff[BG_KMAX + 1];  // top-up offsets (relative to the fill level)
-    int take[BG_KMAX];
     int hist[32];
     int64_t fbase[BG_KMAX];  // off + cur - woff: element e of the child's window is at fbase + e
     unsigned char src[BG_KMAX];  // SRC_WS0 / SRC_WS1 / SRC_B
@@ -421,7 +419,6 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         g.scale[t] = c.src.scale;
         g.len[t] = c.src.kind == SRC_VIRTUAL ? c.src.len : slots[c.src.slot];
         g.cur[t] = 0;
-        g.take[t] = 0;
     }
     if (t < 32) g.hist[t] = 0;
     if (t < BG_W / 64) g.wstart[t] = 0ull;
@@ -481,7 +478,6 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             if (e < Wn) {
                 g.col[e] = rc[i];
                 g.val[e] = ((vmask >> i) & 1u) ? fmul(rs[i], rv[i]) : rv[i];
-                g.cidx[e] = (unsigned char)cj[i];
                 g.elid[e] = g.lid[cj[i]];
             }
         }
@@ -495,8 +491,16 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         const int64_t rem = t < k ? g.len[t] - g.cur[t] : 0;
         const int c_lo = block_min_i(g, sl, (t < k && g.w[t] > 0) ? g.col[g.woff[t]] : INT_MAX);
         int filled = Wn;
+        // this child's loaded elements: its window, then one run per top-up
+        // (buffer offset and length), columns ascending across them
+        int sgo[1 + BG_TOPUPS], sgn[1 + BG_TOPUPS];
+        sgo[0] = t < k ? g.woff[t] : 0;
+        sgn[0] = t < k ? g.w[t] : 0;
+#pragma unroll
+        for (int it = 0; it < BG_TOPUPS; ++it) sgo[1 + it] = sgn[1 + it] = 0;
         mark(0);
         // ---- top-ups: children stopping short of the target share the free space
+#pragma unroll
         for (int it = 0; it < BG_TOPUPS; ++it) {
             const bool partial = t < k && g.w[t] > 0 && (int64_t)g.w[t] < rem;
             if (partial) atomicAdd(&g.hist[31 - __clz(g.lastc[t] - c_lo + 1)], 1);
@@ -554,7 +558,6 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
                     const int pos = filled + e;
                     g.col[pos] = tc[i];
                     g.val[pos] = g.src[j] == SRC_B ? fmul(g.scale[j], tv[i]) : tv[i];
-                    g.cidx[pos] = (unsigned char)j;
                     g.elid[pos] = g.lid[j];
                 }
             }
@@ -562,6 +565,8 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             if (t < k && amt > 0) {
                 g.lastc[t] = g.col[filled + ex + amt - 1];
                 g.w[t] += amt;
+                sgo[1 + it] = filled + ex;
+                sgn[1 + it] = amt;
             }
             filled += tot;
         }
@@ -570,7 +575,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         const bool partial = t < k && g.w[t] > 0 && (int64_t)g.w[t] < rem;
         const int lm = block_min_i(g, sl, partial ? g.lastc[t] : INT_MAX);
         const int bound = (int)min((int64_t)lm, (int64_t)c_lo + BG_SPAN - 1);
-        // ---- taken elements: bitmap bits, per-child counts; counters cleared
+        // ---- taken elements: bitmap bits; counters cleared
         unsigned taken = 0u;
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i) {
@@ -581,18 +586,37 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
                     taken |= 1u << i;
                     const int bit = c - c_lo;
                     atomicOr(&g.bm[bit >> 6], 1ull << (bit & 63));
-                    atomicAdd(&g.take[g.cidx[e]], 1);
                 }
             }
             g.cnt[e] = 0;
         }
-        __syncthreads();
+        // ---- this child's taken count: its elements with column <= bound,
+        // a prefix of its runs (binary search in the first run that ends
+        // above the bound)
         int64_t remt = 0;
         if (t < k) {
-            g.cur[t] += g.take[t];
-            g.take[t] = 0;
+            int ntk = 0;
+#pragma unroll
+            for (int r = 0; r <= BG_TOPUPS; ++r) {
+                const int o = sgo[r], n = sgn[r];
+                if (n == 0) continue;
+                if (g.col[o + n - 1] <= bound) {
+                    ntk += n;
+                    continue;
+                }
+                int lo = 0, hi = n - 1;  // col[o + hi] > bound
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (g.col[o + mid] <= bound) lo = mid + 1;
+                    else hi = mid;
+                }
+                ntk += lo;
+                break;
+            }
+            g.cur[t] += ntk;
             remt = g.len[t] - g.cur[t];
         }
+        __syncthreads();
         mark(2);
         // ---- ranks of the bitmap words; with them the number taken
         int pc[BG_WPT];
