@@ -258,15 +258,16 @@ __device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const R
 // through.
 __device__ double bg_small_sum(const BgSmem& g, int s, int m) {
     constexpr int M = BG_THREAD_M;
+    // unconditional reads (indices clamped into the bucket), so that each
+    // batch of LDS reads is issued together; slots past m get the top byte
+    // set and sort last
+    int ce[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) ce[i] = g.contrib[s + min(i, m - 1)];
     unsigned key[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i) {
-        key[i] = ~0u;
-        if (i < m) {
-            const int e = g.contrib[s + i];
-            key[i] = ((unsigned)g.elid[e] << 16) | (unsigned)e;
-        }
-    }
+    for (int i = 0; i < M; ++i)
+        key[i] = ((unsigned)g.elid[ce[i]] << 16) | (unsigned)ce[i] | (i < m ? 0u : 0xFF000000u);
     // Batcher's odd-even merge sort network over M keys
 #pragma unroll
     for (int p = 1; p < M; p <<= 1)
@@ -283,7 +284,7 @@ __device__ double bg_small_sum(const BgSmem& g, int s, int m) {
                     }
     double v[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i) v[i] = i < m ? g.val[key[i] & 0xFFFFu] : 0.0;
+    for (int i = 0; i < M; ++i) v[i] = g.val[key[i] & 0xFFFFu];  // used for i < m only
     double lv[BPASS_BITS];
     unsigned pend = 0u;
     int prev = (int)(key[0] >> 16);
@@ -724,20 +725,22 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         // lanes of a wavefront take the same path: up to BG_THREAD_M
         // contributions for a second thread pass, more for whole wavefronts
         for (int oi = t; oi < nout; oi += BG_NT) {
-            const int s = oi ? g.cnt[oi - 1] : 0;
+            const int sp = g.cnt[oi > 0 ? oi - 1 : 0];
+            const int s = oi > 0 ? sp : 0;
             const int m = g.cnt[oi] - s;
             if (m > 2) {
                 if (m > BG_THREAD_M) g.big[atomicAdd(&g.nbig, 1)] = (unsigned short)oi;
                 else g.mid[atomicAdd(&g.nmid, 1)] = (unsigned short)oi;
                 continue;
             }
-            int e0 = g.contrib[s];
-            double v = g.val[e0];
-            if (m == 2) {
-                const int e1 = g.contrib[s + 1];
-                const double v1 = g.val[e1];
-                v = g.elid[e0] < g.elid[e1] ? fadd(v, v1) : fadd(v1, v);
-            }
+            // both reads in every lane (e1 == e0 for one contribution), so
+            // that the LDS reads are issued together
+            const int e0 = g.contrib[s];
+            const int e1 = g.contrib[s + m - 1];
+            const double v0 = g.val[e0], v1 = g.val[e1];
+            const unsigned l0 = g.elid[e0], l1 = g.elid[e1];
+            double v = v0;
+            if (m == 2) v = l0 < l1 ? fadd(v0, v1) : fadd(v1, v0);
             const int64_t q = out_n + oi;
             if (q < dcap) {
                 dc[q] = g.col[e0];
