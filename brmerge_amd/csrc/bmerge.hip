@@ -209,13 +209,16 @@ __device__ __forceinline__ SrcPtr src_ptr(unsigned char sk, const RowArgs& a, co
 }
 
 // Loads this thread's elements of the proportional windows (buffer
-// positions bg_elem(i)) into registers; cj = child.  The child of element e
-// is the last window starting at or before e: its rank is the number of
-// wstart bits up to e, less one (the 64 elements of one step of a wavefront
-// share one wstart word).
-__device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const RowArgs& a, const WsCols& ws,
+// positions bg_elem(i)) into registers; rl = list id of the element's child.
+// The child of element e is the last window starting at or before e: its
+// rank is the number of wstart bits up to e, less one (the 64 elements of
+// one step of a wavefront share one wstart word).  Elements at or past the
+// window end (Wn >= 1) load element Wn - 1 again and are not stored, so
+// that nothing here branches and every batch of LDS reads is issued
+// together.
+__device__ __forceinline__ void bg_fetch(const BgSmem& g, int Wn, const RowArgs& a, const WsCols& ws,
                                          int (&rc)[BG_IPT], double (&rv)[BG_IPT], double (&rs)[BG_IPT],
-                                         int (&cj)[BG_IPT], unsigned& vmask) {
+                                         int (&rl)[BG_IPT], unsigned& vmask) {
     static_assert(BG_W / 64 <= BRM_WAVE, "one wstart word per lane");
     vmask = 0u;
     const int lane = lane_id();
@@ -223,26 +226,21 @@ __device__ __forceinline__ void bg_fetch(const BgSmem& g, int k, int Wn, const R
     const int before = wave_incl_scan(pc) - pc;  // window starts in earlier words
     const unsigned long long upto = lane == BRM_WAVE - 1 ? ~0ull : (2ull << lane) - 1ull;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / BRM_WAVE));
+    int j[BG_IPT];
 #pragma unroll
     for (int i = 0; i < BG_IPT; ++i) {
-        const int e = bg_elem(i);
-        const int wd = wv * BG_IPT + i;  // e >> 6, the same for the whole wavefront
-        const int r = __builtin_amdgcn_readlane(before, wd) + __popcll(g.wstart[wd] & upto) - 1;
-        // every register is written, so nothing of the previous round stays live
-        cj[i] = -1;
-        rc[i] = 0;
-        rv[i] = 0.0;
-        rs[i] = 0.0;
-        if (e < Wn) {
-            const int j = g.wchild[r];
-            const SrcPtr p = src_ptr(g.src[j], a, ws);
-            const int64_t x = g.fbase[j] + e;
-            rc[i] = p.c[x];
-            rv[i] = p.v[x];
-            rs[i] = g.scale[j];
-            if (p.virt) vmask |= 1u << i;
-            cj[i] = j;
-        }
+        const int wd = wv * BG_IPT + i;  // bg_elem(i) >> 6, the same for the whole wavefront
+        j[i] = g.wchild[__builtin_amdgcn_readlane(before, wd) + __popcll(g.wstart[wd] & upto) - 1];
+    }
+#pragma unroll
+    for (int i = 0; i < BG_IPT; ++i) {
+        const SrcPtr p = src_ptr(g.src[j[i]], a, ws);
+        const int64_t x = g.fbase[j[i]] + min(bg_elem(i), Wn - 1);
+        rc[i] = p.c[x];
+        rv[i] = p.v[x];
+        rs[i] = g.scale[j[i]];
+        rl[i] = g.lid[j[i]];
+        if (p.virt) vmask |= 1u << i;
     }
 }
 
@@ -446,13 +444,13 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
     }
     const int64_t in_total = block_sum_l(g, sl, t < k ? g.len[t] : 0);
 
-    int rc[BG_IPT], cj[BG_IPT];
+    int rc[BG_IPT], rl[BG_IPT];
     double rv[BG_IPT], rs[BG_IPT];
     unsigned vmask = 0u;
     bool done = bg_windows(g, sl, k);
     __syncthreads();
     int Wn = g.woff[k];
-    if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, vmask);
+    if (!done) bg_fetch(g, Wn, a, ws, rc, rv, rs, rl, vmask);
     int64_t out_n = 0;
     int rounds = 0;
     // shader cycles per round phase, seen by thread 0 (barrier waits included)
@@ -479,7 +477,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             if (e < Wn) {
                 g.col[e] = rc[i];
                 g.val[e] = ((vmask >> i) & 1u) ? fmul(rs[i], rv[i]) : rv[i];
-                g.elid[e] = g.lid[cj[i]];
+                g.elid[e] = (unsigned char)rl[i];
             }
         }
         __syncthreads();
@@ -576,18 +574,18 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         const bool partial = t < k && g.w[t] > 0 && (int64_t)g.w[t] < rem;
         const int lm = block_min_i(g, sl, partial ? g.lastc[t] : INT_MAX);
         const int bound = (int)min((int64_t)lm, (int64_t)c_lo + BG_SPAN - 1);
-        // ---- taken elements: bitmap bits; counters cleared
+        // ---- taken elements: bitmap bits (kept: 0 for the others); counters cleared
         unsigned taken = 0u;
+        int bits[BG_IPT];
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i) {
             const int e = bg_elem(i);
-            if (e < filled) {
-                const int c = g.col[e];
-                if (c <= bound) {
-                    taken |= 1u << i;
-                    const int bit = c - c_lo;
-                    atomicOr(&g.bm[bit >> 6], 1ull << (bit & 63));
-                }
+            const int c = g.col[e];
+            bits[i] = 0;
+            if (e < filled && c <= bound) {
+                taken |= 1u << i;
+                bits[i] = c - c_lo;
+                atomicOr(&g.bm[bits[i] >> 6], 1ull << (bits[i] & 63));
             }
             g.cnt[e] = 0;
         }
@@ -644,18 +642,16 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         __syncthreads();
         mark(3);
         // ---- output position of every taken element; contributions per position
+        // (read for every element, so that the reads are issued together)
         int pos[BG_IPT];
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i) {
-            pos[i] = 0;
-            if ((taken >> i) & 1u) {
-                const int e = bg_elem(i);
-                const int bit = g.col[e] - c_lo;
-                const int wd = bit >> 6;
-                pos[i] = g.wrank[wd] + __popcll(g.bm[wd] & ((1ull << (bit & 63)) - 1ull));
-                atomicAdd(&g.cnt[pos[i]], 1);
-            }
+            const int wd = bits[i] >> 6;
+            pos[i] = g.wrank[wd] + __popcll(g.bm[wd] & ((1ull << (bits[i] & 63)) - 1ull));
         }
+#pragma unroll
+        for (int i = 0; i < BG_IPT; ++i)
+            if ((taken >> i) & 1u) atomicAdd(&g.cnt[pos[i]], 1);
         __syncthreads();
         mark(4);
         // ---- bucket offsets (exclusive scan of the counts) and, in the
@@ -710,14 +706,16 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         mark(5);
         // ---- bucket the contributions (cnt[p] ends as the bucket's end);
         // the next round's windows are fetched into registers
+        int slot[BG_IPT];
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i) {
-            if ((taken >> i) & 1u) {
-                const int slot = atomicAdd(&g.cnt[pos[i]], 1);
-                g.contrib[slot] = (unsigned short)bg_elem(i);
-            }
+            slot[i] = 0;
+            if ((taken >> i) & 1u) slot[i] = atomicAdd(&g.cnt[pos[i]], 1);
         }
-        if (!done) bg_fetch(g, k, Wn, a, ws, rc, rv, rs, cj, vmask);
+#pragma unroll
+        for (int i = 0; i < BG_IPT; ++i)
+            if ((taken >> i) & 1u) g.contrib[slot[i]] = (unsigned short)bg_elem(i);
+        if (!done) bg_fetch(g, Wn, a, ws, rc, rv, rs, rl, vmask);
         __syncthreads();
         mark(6);
         // ---- one output per thread (strided: coalesced stores) for one
