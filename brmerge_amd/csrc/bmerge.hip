@@ -67,7 +67,9 @@ struct BgSmem {
     unsigned long long wstart[BG_W / 64];  // bit woff[j] set for every child with a window
     unsigned char wchild[BG_KMAX];         // the children with windows, in order
     int lastc[BG_KMAX];     // last loaded column of the child
-    int xoff[BG_KMAX + 1];  // top-up offsets (relative to the fill level)
+    unsigned long long xstart[BG_W / 64];  // top-up runs: bit at each run's start (relative to the fill level)
+    unsigned char xchild[BG_KMAX];         // the children with top-up runs, in order
+    int64_t xbase[BG_KMAX];  // top-up element e of the child is at xbase + e
     int hist[32];
     int64_t fbase[BG_KMAX];  // off + cur - woff: element e of the child's window is at fbase + e
     unsigned char src[BG_KMAX];  // SRC_WS0 / SRC_WS1 / SRC_B
@@ -420,7 +422,10 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         g.cur[t] = 0;
     }
     if (t < 32) g.hist[t] = 0;
-    if (t < BG_W / 64) g.wstart[t] = 0ull;
+    if (t < BG_W / 64) {
+        g.wstart[t] = 0ull;
+        g.xstart[t] = 0ull;
+    }
     if (t == 0) {
         g.nbig = 0;
         g.nmid = 0;
@@ -524,47 +529,71 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             const int add = ns > 0 ? (BG_W - filled) / ns : 0;
             if (add == 0) break;
             const int amt = shorty ? (int)min((int64_t)add, rem - g.w[t]) : 0;
+            // one scan gives the run offsets (low 16 bits) and the rank
+            // among children with a run (high bits)
             int tot;
-            const int ex = block_scan_i(g, sl, amt, &tot);
-            if (t < k) g.xoff[t] = ex;
-            if (t == BG_NT - 1) g.xoff[k] = tot;
+            const int ex = block_scan_i(g, sl, amt | ((amt > 0 ? 1 : 0) << 16), &tot);
+            const int xo = ex & 0xFFFF;
+            tot &= 0xFFFF;
+            if (amt > 0) {
+                g.xchild[ex >> 16] = (unsigned char)t;
+                g.xbase[t] = g.off[t] + g.cur[t] + g.w[t] - xo;
+                atomicOr(&g.xstart[xo >> 6], 1ull << (xo & 63));
+            }
             __syncthreads();
             // tot <= BG_W - filled: at most BG_IPT top-up elements per
-            // thread; all loads are issued before the first is used
-            int tc[BG_IPT], tj[BG_IPT];
-            double tv[BG_IPT];
+            // thread (element t + i * BG_NT; the 64 of one wavefront step
+            // share one xstart word).  As in the window fetch, the child of
+            // an element is found by a popcount over the run starts, and
+            // elements at or past tot (>= 1) load element tot - 1 again and
+            // are not stored, so that the LDS reads are issued together;
+            // wavefront steps entirely past tot load nothing.
+            int tc[BG_IPT], tl[BG_IPT];
+            double tv[BG_IPT], ts[BG_IPT];
+            unsigned tvm = 0u;
+            {
+                const int lane = lane_id();
+                const int pc = lane < BG_W / 64 ? __popcll(g.xstart[lane]) : 0;
+                const int before = wave_incl_scan(pc) - pc;
+                const unsigned long long upto = lane == BRM_WAVE - 1 ? ~0ull : (2ull << lane) - 1ull;
+                const int wv = __builtin_amdgcn_readfirstlane((int)(t / BRM_WAVE));
+                int j[BG_IPT];
 #pragma unroll
-            for (int i = 0; i < BG_IPT; ++i) {
-                const int e = t + i * BG_NT;
-                tc[i] = 0;
-                tj[i] = 0;
-                tv[i] = 0.0;
-                if (e < tot) {
-                    const int j = upper_idx(g.xoff, 0, k, e);
-                    const int q = e - g.xoff[j];
-                    const SrcPtr p = src_ptr(g.src[j], a, ws);
-                    const int64_t x = g.off[j] + g.cur[j] + g.w[j] + q;
-                    tc[i] = p.c[x];
-                    tv[i] = p.v[x];
-                    tj[i] = j;
+                for (int i = 0; i < BG_IPT; ++i) {
+                    const int wd = wv + i * BG_NW;  // (t + i * BG_NT) >> 6
+                    j[i] = g.xchild[__builtin_amdgcn_readlane(before, wd) + __popcll(g.xstart[wd] & upto) - 1];
+                }
+#pragma unroll
+                for (int i = 0; i < BG_IPT; ++i) {
+                    const SrcPtr p = src_ptr(g.src[j[i]], a, ws);
+                    const int64_t x = g.xbase[j[i]] + min(t + i * BG_NT, tot - 1);
+                    ts[i] = g.scale[j[i]];
+                    tl[i] = g.lid[j[i]];
+                    if (p.virt) tvm |= 1u << i;
+                    tc[i] = 0;
+                    tv[i] = 0.0;
+                    if (wv * BRM_WAVE + i * BG_NT < tot) {  // the same for the whole wavefront
+                        tc[i] = p.c[x];
+                        tv[i] = p.v[x];
+                    }
                 }
             }
 #pragma unroll
             for (int i = 0; i < BG_IPT; ++i) {
                 const int e = t + i * BG_NT;
                 if (e < tot) {
-                    const int j = tj[i];
                     const int pos = filled + e;
                     g.col[pos] = tc[i];
-                    g.val[pos] = g.src[j] == SRC_B ? fmul(g.scale[j], tv[i]) : tv[i];
-                    g.elid[pos] = g.lid[j];
+                    g.val[pos] = ((tvm >> i) & 1u) ? fmul(ts[i], tv[i]) : tv[i];
+                    g.elid[pos] = (unsigned char)tl[i];
                 }
             }
             __syncthreads();
-            if (t < k && amt > 0) {
-                g.lastc[t] = g.col[filled + ex + amt - 1];
+            if (t < BG_W / 64) g.xstart[t] = 0ull;  // read by every thread before the barrier above
+            if (amt > 0) {
+                g.lastc[t] = g.col[filled + xo + amt - 1];
                 g.w[t] += amt;
-                sgo[1 + it] = filled + ex;
+                sgo[1 + it] = filled + xo;
                 sgn[1 + it] = amt;
             }
             filled += tot;
