@@ -70,7 +70,7 @@ struct BgSmem {
     unsigned long long xstart[BG_W / 64];  // top-up runs: bit at each run's start (relative to the fill level)
     unsigned char xchild[BG_KMAX];         // the children with top-up runs, in order
     int64_t xbase[BG_KMAX];  // top-up element e of the child is at xbase + e
-    int hist[32];
+    int hist[BG_TOPUPS][32];  // per top-up: partial children by log2 extent
     int64_t fbase[BG_KMAX];  // off + cur - woff: element e of the child's window is at fbase + e
     unsigned char src[BG_KMAX];  // SRC_WS0 / SRC_WS1 / SRC_B
     unsigned char lid[BG_KMAX];
@@ -421,7 +421,8 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         g.len[t] = c.src.kind == SRC_VIRTUAL ? c.src.len : slots[c.src.slot];
         g.cur[t] = 0;
     }
-    if (t < 32) g.hist[t] = 0;
+    if (t < 32)
+        for (int it = 0; it < BG_TOPUPS; ++it) g.hist[it][t] = 0;
     if (t < BG_W / 64) {
         g.wstart[t] = 0ull;
         g.xstart[t] = 0ull;
@@ -507,25 +508,25 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
 #pragma unroll
         for (int it = 0; it < BG_TOPUPS; ++it) {
             const bool partial = t < k && g.w[t] > 0 && (int64_t)g.w[t] < rem;
-            if (partial) atomicAdd(&g.hist[31 - __clz(g.lastc[t] - c_lo + 1)], 1);
-            const int np = block_sum_i(g, sl, partial ? 1 : 0);
-            if (np == 0 || filled >= BG_W) {
-                if (t < 32) g.hist[t] = 0;  // read by nobody this round
-                break;
-            }
-            // median bucket of the partial children's extents
-            int mb;
+            // log2 of the extent lastc - c_lo + 1 (>= 1) loaded so far
+            const int bkt = partial ? 31 - __clz(g.lastc[t] - c_lo + 1) : 0;
+            if (partial) atomicAdd(&g.hist[it][bkt], 1);
+            __syncthreads();
+            // from one wave scan of the histogram: the number of partial
+            // children, the median bucket mb, and the number of children
+            // below it (extent < 2^mb: short of the target c_lo + 2^mb - 1)
+            int np, mb, ns;
             {
                 const int lane = lane_id();
-                const int hv = lane < 32 ? g.hist[lane] : 0;
+                const int hv = lane < 32 ? g.hist[it][lane] : 0;
                 const int inc = wave_incl_scan(hv);
+                np = __builtin_amdgcn_readlane(inc, 31);
                 const uint64_t m = __ballot(inc >= (np + 1) / 2);
-                mb = __ffsll((long long)m) - 1;
+                mb = np > 0 ? __ffsll((long long)m) - 1 : 0;
+                ns = __builtin_amdgcn_readlane(inc - hv, mb);
             }
-            const int64_t tgt = (int64_t)c_lo + ((int64_t)1 << mb) - 1;
-            const bool shorty = partial && (int64_t)g.lastc[t] < tgt;
-            const int ns = block_sum_i(g, sl, shorty ? 1 : 0);
-            if (t < 32) g.hist[t] = 0;  // every thread has read it before the barrier above
+            if (np == 0 || filled >= BG_W) break;
+            const bool shorty = partial && bkt < mb;
             const int add = ns > 0 ? (BG_W - filled) / ns : 0;
             if (add == 0) break;
             const int amt = shorty ? (int)min((int64_t)add, rem - g.w[t]) : 0;
@@ -603,6 +604,10 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         const bool partial = t < k && g.w[t] > 0 && (int64_t)g.w[t] < rem;
         const int lm = block_min_i(g, sl, partial ? g.lastc[t] : INT_MAX);
         const int bound = (int)min((int64_t)lm, (int64_t)c_lo + BG_SPAN - 1);
+        // every read of the histograms was before the barrier above
+        if (t < 32)
+#pragma unroll
+            for (int it = 0; it < BG_TOPUPS; ++it) g.hist[it][t] = 0;
         // ---- taken elements: bitmap bits (kept: 0 for the others); counters cleared
         unsigned taken = 0u;
         int bits[BG_IPT];
