@@ -1,7 +1,7 @@
 // Bitmap-rank group merge for long rows (group_cfg 4).
 //
 // One workgroup materialises one tree node above the lists of a long row
-// from up to 256 child lists (B rows read in place, or nodes of the previous
+// from up to BGROUP_KMAX child lists (B rows read in place, or nodes of the previous
 // pass), in rounds over a streaming window, like the merge-path engine, but
 // without merge levels:
 //   1. every child loads a window proportional to what it has left; then,
@@ -19,8 +19,12 @@
 //      aligned binary tree over list positions sums them
 //      (BRMerge-SpGEMM/inc/spgemm.h:639-707: siblings added left + right,
 //      a node with one present child passes it through).
-// The work per element does not depend on the number of children, so one
-// pass merges 2^8 tree levels' worth of lists.
+// The work per element does not depend on the number of children.  The
+// fan-in (BGROUP_KMAX = 2^BPASS_BITS) is set by the take rate: the bound is
+// the smallest end among the partially loaded children, so the longer each
+// child's share of the window, the more of the window is taken per round.
+// With 2^5 children a round takes most of its window (with 2^8, about half),
+// which outweighs the extra pass over the rows with more than 2^10 lists.
 #include <limits.h>
 
 #include <algorithm>
@@ -43,7 +47,7 @@ constexpr int BG_NW = BG_NT / BRM_WAVE;
 constexpr int BG_NSLOT = 8;    // rotating reduction scratch slots
 constexpr int BG_TOPUPS = 2;   // top-up iterations per round
 // Outputs with more contributions than this are summed by a whole wavefront
-// (dense over the 256 list ids) instead of by one thread.
+// (dense over the list ids) instead of by one thread.
 constexpr int BG_THREAD_M = 8;
 constexpr int BG_BIGMAX = BG_W / (BG_THREAD_M + 1) + 1;
 constexpr int BG_MIDMAX = BG_W / 3 + 1;  // outputs with 3 .. BG_THREAD_M contributions
@@ -355,8 +359,8 @@ __device__ __forceinline__ void dpp_level(double& x, bool& px, int lane) {
 
 // Sums of two output buckets by one wavefront, one per half of 32 lanes
 // (m = 0: no bucket).  Each half places its bucket's contributions densely
-// by list id (256 slots of its own in dense/pres, pres all zero on entry and
-// on return), each lane reduces its 8 slots, and the aligned tree over the
+// by list id (BG_KMAX slots of its own in dense/pres, pres all zero on entry
+// and on return), each lane reduces its BG_KMAX / 32 slots, and the aligned tree over the
 // half's lanes is finished with DPP moves (no LDS round trips): lanes
 // 2s-1 mod 2s combine at level s, the roots end in lanes 31 and 63.
 // Returns the first sum in r0 and the second in r1, in every lane.
@@ -807,9 +811,9 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
             unsigned char* pres = reinterpret_cast<unsigned char*>(g.wrank) + wv * 2 * BG_KMAX;
             const int nbig = g.nbig;
             if (nbig > 0) {
-                static_assert(2 * BG_KMAX == 8 * BRM_WAVE, "two flag words per lane");
-                reinterpret_cast<unsigned*>(pres)[lane] = 0u;
-                reinterpret_cast<unsigned*>(pres)[BRM_WAVE + lane] = 0u;
+                static_assert((2 * BG_KMAX) % 4 == 0, "whole flag words");
+#pragma unroll
+                for (int i = lane; i < 2 * BG_KMAX / 4; i += BRM_WAVE) reinterpret_cast<unsigned*>(pres)[i] = 0u;
                 wave_lds_sync();
                 for (;;) {
                     int b = 0;

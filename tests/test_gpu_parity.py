@@ -306,8 +306,9 @@ def test_signed_zero_products(gpu, tier):
 def test_wide_columns_and_hub_columns(gpu, engine, force_long):
     """Columns spread over a million (several bitmap spans per node), a few
     columns shared by most B rows (many contributions per output, deep
-    trees of pass-throughs), and rows with more than 256 lists (two passes
-    of the bitmap engine)."""
+    trees of pass-throughs), and rows on both sides of 2^5, 2^8 and 2^10
+    lists (one, two and three passes of the bitmap engine, whose fan-in is
+    2^5; the merge-path engine's is the same)."""
     old = gpu.get_param("force_long")
     gpu.set_param("force_long", force_long)
     try:
@@ -322,7 +323,7 @@ def test_wide_columns_and_hub_columns(gpu, engine, force_long):
             rows.append(np.sort(c))
         brpt = np.concatenate([[0], np.cumsum([len(b) for b in rows])]).astype(np.int32)
         B = (brpt, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, brpt[-1]), N)
-        acols = [np.sort(rng.choice(K, size=sz, replace=False)) for sz in (3000, 600, 257, 256, 255, 40, 3)]
+        acols = [np.sort(rng.choice(K, size=sz, replace=False)) for sz in (3000, 1025, 1024, 600, 257, 256, 255, 40, 33, 32, 3)]
         arpt = np.concatenate([[0], np.cumsum([len(a) for a in acols])]).astype(np.int32)
         A = (arpt, np.concatenate(acols).astype(np.int32), rng.uniform(-1, 1, arpt[-1]), K)
         R = oracle.brmerge_precise(A, B)
