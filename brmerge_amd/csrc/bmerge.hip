@@ -36,7 +36,7 @@ namespace brm {
 namespace {
 
 constexpr int BG_NT = 256;
-constexpr int BG_W = 2048;                    // window elements per round
+constexpr int BG_W = 2560;                    // window elements per round
 constexpr int BG_IPT = BG_W / BG_NT;          // window elements per thread
 constexpr int BG_SPAN_LOG2 = 17;
 constexpr int64_t BG_SPAN = 1 << BG_SPAN_LOG2;  // columns per round
