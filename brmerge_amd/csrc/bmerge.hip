@@ -26,6 +26,7 @@
 // With 2^5 children a round takes most of its window (with 2^8, about half),
 // which outweighs the extra pass over the rows with more than 2^10 lists.
 #include <limits.h>
+#include <stddef.h>
 
 #include <algorithm>
 
@@ -36,7 +37,7 @@ namespace brm {
 namespace {
 
 constexpr int BG_NT = 256;
-constexpr int BG_W = 2560;                    // window elements per round
+constexpr int BG_W = 3072;                    // window elements per round
 constexpr int BG_IPT = BG_W / BG_NT;          // window elements per thread
 constexpr int BG_SPAN_LOG2 = 17;
 constexpr int64_t BG_SPAN = 1 << BG_SPAN_LOG2;  // columns per round
@@ -58,7 +59,7 @@ struct BgSmem {
     double val[BG_W];
     int col[BG_W];
     unsigned long long bm[BG_WORDS];
-    int cnt[BG_W];
+    unsigned short cnt[BG_W];  // bucket counts, then offsets, then ends (atomics: cnt_add)
     unsigned short wrank[BG_WORDS];
     unsigned short contrib[BG_W];
     unsigned char elid[BG_W];       // list id (node id) of the element's child
@@ -88,6 +89,15 @@ struct BgSmem {
 };
 
 static_assert(sizeof(BgSmem) <= 80 * 1024, "two workgroups per CU");
+static_assert(offsetof(BgSmem, cnt) % 4 == 0 && BG_W % 2 == 0, "cnt in whole 32-bit words");
+
+// Adds one to counter p of cnt and returns its old value.  LDS atomics are
+// 32-bit: the add goes to the word holding counters p & ~1 and p | 1; a
+// counter never exceeds BG_W < 2^16, so no carry reaches its neighbour.
+__device__ __forceinline__ int cnt_add(BgSmem& g, int p) {
+    const int sh = (p & 1) * 16;
+    return (int)((atomicAdd(reinterpret_cast<unsigned*>(g.cnt) + (p >> 1), 1u << sh) >> sh) & 0xFFFFu);
+}
 
 struct WsCols {
     int* col0;
@@ -215,7 +225,7 @@ __device__ __forceinline__ SrcPtr src_ptr(unsigned char sk, const RowArgs& a, co
 }
 
 // Loads this thread's elements of the proportional windows (buffer
-// positions bg_elem(i)) into registers; rl = list id of the element's child.
+// positions bg_elem(i)) into registers; rj = the element's child.
 // The child of element e is the last window starting at or before e: its
 // rank is the number of wstart bits up to e, less one (the 64 elements of
 // one step of a wavefront share one wstart word).  Elements at or past the
@@ -223,8 +233,8 @@ __device__ __forceinline__ SrcPtr src_ptr(unsigned char sk, const RowArgs& a, co
 // that nothing here branches and every batch of LDS reads is issued
 // together.
 __device__ __forceinline__ void bg_fetch(const BgSmem& g, int Wn, const RowArgs& a, const WsCols& ws,
-                                         int (&rc)[BG_IPT], double (&rv)[BG_IPT], double (&rs)[BG_IPT],
-                                         int (&rl)[BG_IPT], unsigned& vmask) {
+                                         int (&rc)[BG_IPT], double (&rv)[BG_IPT], int (&rj)[BG_IPT],
+                                         unsigned& vmask) {
     static_assert(BG_W / 64 <= BRM_WAVE, "one wstart word per lane");
     vmask = 0u;
     const int lane = lane_id();
@@ -244,8 +254,7 @@ __device__ __forceinline__ void bg_fetch(const BgSmem& g, int Wn, const RowArgs&
         const int64_t x = g.fbase[j[i]] + min(bg_elem(i), Wn - 1);
         rc[i] = p.c[x];
         rv[i] = p.v[x];
-        rs[i] = g.scale[j[i]];
-        rl[i] = g.lid[j[i]];
+        rj[i] = j[i];
         if (p.virt) vmask |= 1u << i;
     }
 }
@@ -454,13 +463,13 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
     }
     const int64_t in_total = block_sum_l(g, sl, t < k ? g.len[t] : 0);
 
-    int rc[BG_IPT], rl[BG_IPT];
-    double rv[BG_IPT], rs[BG_IPT];
+    int rc[BG_IPT], rj[BG_IPT];
+    double rv[BG_IPT];
     unsigned vmask = 0u;
     bool done = bg_windows(g, sl, k);
     __syncthreads();
     int Wn = g.woff[k];
-    if (!done) bg_fetch(g, Wn, a, ws, rc, rv, rs, rl, vmask);
+    if (!done) bg_fetch(g, Wn, a, ws, rc, rv, rj, vmask);
     int64_t out_n = 0;
     int rounds = 0;
     // shader cycles per round phase, seen by thread 0 (barrier waits included)
@@ -480,14 +489,25 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         // stated explicitly so that no load is left in flight into the next
         // prefetch, whose destination registers are reused as temporaries
         __builtin_amdgcn_s_waitcnt(0x0F70);
-        // ---- the proportional windows into the buffer (products rounded here)
+        // ---- the proportional windows into the buffer (products rounded
+        // here; the children's scales and list ids read for every element,
+        // so that the reads are issued together)
+        {
+            double sc[BG_IPT];
+            int lq[BG_IPT];
 #pragma unroll
-        for (int i = 0; i < BG_IPT; ++i) {
-            const int e = bg_elem(i);
-            if (e < Wn) {
-                g.col[e] = rc[i];
-                g.val[e] = ((vmask >> i) & 1u) ? fmul(rs[i], rv[i]) : rv[i];
-                g.elid[e] = (unsigned char)rl[i];
+            for (int i = 0; i < BG_IPT; ++i) {
+                sc[i] = g.scale[rj[i]];
+                lq[i] = g.lid[rj[i]];
+            }
+#pragma unroll
+            for (int i = 0; i < BG_IPT; ++i) {
+                const int e = bg_elem(i);
+                if (e < Wn) {
+                    g.col[e] = rc[i];
+                    g.val[e] = ((vmask >> i) & 1u) ? fmul(sc[i], rv[i]) : rv[i];
+                    g.elid[e] = (unsigned char)lq[i];
+                }
             }
         }
         __syncthreads();
@@ -689,7 +709,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
         }
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i)
-            if ((taken >> i) & 1u) atomicAdd(&g.cnt[pos[i]], 1);
+            if ((taken >> i) & 1u) (void)cnt_add(g, pos[i]);
         __syncthreads();
         mark(4);
         // ---- bucket offsets (exclusive scan of the counts) and, in the
@@ -724,7 +744,7 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
 #pragma unroll
             for (int i = 0; i < BG_IPT; ++i) {
                 const int oi = t * BG_IPT + i;
-                if (oi < nout) g.cnt[oi] = bex;
+                if (oi < nout) g.cnt[oi] = (unsigned short)bex;
                 bex += loc[i];
             }
             const int wo = (int)((ex >> 16) & 0xFFFF);
@@ -748,12 +768,12 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i) {
             slot[i] = 0;
-            if ((taken >> i) & 1u) slot[i] = atomicAdd(&g.cnt[pos[i]], 1);
+            if ((taken >> i) & 1u) slot[i] = cnt_add(g, pos[i]);
         }
 #pragma unroll
         for (int i = 0; i < BG_IPT; ++i)
             if ((taken >> i) & 1u) g.contrib[slot[i]] = (unsigned short)bg_elem(i);
-        if (!done) bg_fetch(g, Wn, a, ws, rc, rv, rs, rl, vmask);
+        if (!done) bg_fetch(g, Wn, a, ws, rc, rv, rj, vmask);
         __syncthreads();
         mark(6);
         // ---- one output per thread (strided: coalesced stores) for one
