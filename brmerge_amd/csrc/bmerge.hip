@@ -4,10 +4,10 @@
 // from up to BGROUP_KMAX child lists (B rows read in place, or nodes of the previous
 // pass), in rounds over a streaming window, like the merge-path engine, but
 // without merge levels:
-//   1. every child loads a window proportional to what it has left; then,
-//      twice, the children whose windows stop short of a target column (the
-//      lower edge of the median log2-extent among partially loaded
-//      children) share the free window space as top-ups; all loaded
+//   1. every child loads a window proportional to what it has left; then
+//      (BG_TOPUPS times) the children whose windows stop short of a target
+//      column (the lower edge of the median log2-extent among partially
+//      loaded children) share the free window space as top-ups; all loaded
 //      elements with a column <= bound (the smallest last loaded column of
 //      a partially loaded child, and at most c_lo + 2^17 - 1) are taken --
 //      every contribution to those columns is in the round;
@@ -46,7 +46,7 @@ constexpr int BG_WPT = BG_WORDS / BG_NT;      // bitmap words per thread
 constexpr int BG_KMAX = BGROUP_KMAX;
 constexpr int BG_NW = BG_NT / BRM_WAVE;
 constexpr int BG_NSLOT = 8;    // rotating reduction scratch slots
-constexpr int BG_TOPUPS = 2;   // top-up iterations per round
+constexpr int BG_TOPUPS = 1;   // top-up iterations per round
 // Outputs with more contributions than this are summed by a whole wavefront
 // (dense over the list ids) instead of by one thread.
 constexpr int BG_THREAD_M = 8;
