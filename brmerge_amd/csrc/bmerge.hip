@@ -49,7 +49,7 @@ constexpr int BG_NSLOT = 8;    // rotating reduction scratch slots
 constexpr int BG_TOPUPS = 1;   // top-up iterations per round
 // Outputs with more contributions than this are summed by a whole wavefront
 // (dense over the list ids) instead of by one thread.
-constexpr int BG_THREAD_M = 8;
+constexpr int BG_THREAD_M = 16;
 constexpr int BG_BIGMAX = BG_W / (BG_THREAD_M + 1) + 1;
 constexpr int BG_MIDMAX = BG_W / 3 + 1;  // outputs with 3 .. BG_THREAD_M contributions
 static_assert(BG_KMAX <= BG_NT, "one child per thread in the window logic");
