@@ -259,13 +259,21 @@ __global__ __launch_bounds__(BM_NT) void k_sym_bitmap(const int* __restrict__ ro
                 hint = gallop_idx(loff, hint, nl, e0);
                 int col[BM_U];
                 int h = hint;
+                if (e0 + BM_U <= loff[h + 1]) {
+                    // all BM_U products in list h: consecutive entries of one
+                    // B row, loaded without a list search per product
+                    const int* p = a.bcol + bst[h] + (e0 - loff[h]);
 #pragma unroll
-                for (int u = 0; u < BM_U; ++u) {
-                    const int e = e0 + u;
-                    col[u] = -1;
-                    if (e < Sc) {
-                        while (loff[h + 1] <= e) ++h;  // next non-empty list
-                        col[u] = a.bcol[bst[h] + (e - loff[h])];
+                    for (int u = 0; u < BM_U; ++u) col[u] = p[u];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < BM_U; ++u) {
+                        const int e = e0 + u;
+                        col[u] = -1;
+                        if (e < Sc) {
+                            while (loff[h + 1] <= e) ++h;  // next non-empty list
+                            col[u] = a.bcol[bst[h] + (e - loff[h])];
+                        }
                     }
                 }
 #pragma unroll
