@@ -661,7 +661,24 @@ __global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ 
                     ntk += n;
                     continue;
                 }
+                // the first element above the bound is in [lo, hi]; eight
+                // probes read together narrow it to about a ninth first
                 int lo = 0, hi = n - 1;  // col[o + hi] > bound
+                if (n > 16) {
+                    int pv[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) pv[j] = g.col[o + (j + 1) * (n - 1) / 9];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int pj = (j + 1) * (n - 1) / 9;
+                        if (pv[j] <= bound) lo = pj + 1;
+                    }
+#pragma unroll
+                    for (int j = 7; j >= 0; --j) {
+                        const int pj = (j + 1) * (n - 1) / 9;
+                        if (pv[j] > bound) hi = pj;
+                    }
+                }
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
                     if (g.col[o + mid] <= bound) lo = mid + 1;
