@@ -334,6 +334,25 @@ def test_wide_columns_and_hub_columns(gpu, engine, force_long):
         gpu.set_param("force_long", old)
 
 
+def test_four_pass_rows(gpu, engine):
+    """Rows on both sides of 2^15 lists: three passes at fan-in 2^5 for
+    32,768 lists, four for 32,769, with B rows of 0 to 12 entries (empty
+    lists among them) and columns over 300,000 (three bitmap spans)."""
+    rng = np.random.default_rng(23)
+    K, N = 40_000, 300_000
+    lens = rng.integers(0, 13, K)
+    brpt = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    bcol = np.concatenate([np.sort(rng.choice(N, size=int(d), replace=False)) for d in lens]).astype(np.int32)
+    B = (brpt, bcol, rng.uniform(-1, 1, brpt[-1]), N)
+    acols = [np.sort(rng.choice(K, size=sz, replace=False)) for sz in (32_769, 32_768, 1_025)]
+    arpt = np.concatenate([[0], np.cumsum([len(a) for a in acols])]).astype(np.int32)
+    A = (arpt, np.concatenate(acols).astype(np.int32), rng.uniform(-1, 1, arpt[-1]), K)
+    R = oracle.brmerge_precise(A, B)
+    Cp, Cu = run_both(gpu, A, B)
+    assert_csr_equal(Cp, R)
+    assert_csr_equal(Cu, R)
+
+
 @pytest.mark.parametrize("ncols", [1 << 16, (1 << 16) + 1, 1 << 17, (1 << 17) + 1, 1 << 20], ids=lambda n: f"N{n}")
 @pytest.mark.parametrize("wmed_cap", [128, 512, 1024])
 def test_register_medium_key_widths(gpu, ncols, wmed_cap):
