@@ -50,24 +50,32 @@ def tiers(L, ip, med_cap, wmed_cap, force_long=False):
     return t
 
 
-def kernel_bytes(name, A, L, ip, nnz_row, tier, st):
-    """Algorithmic bytes moved by all launches of kernel `name` in one SpGEMM.
-
-    Per tier: A entries (col 4 + val 8 + product offset 8 = 20 B), one B
-    gather per product (col 4 + val 8 = 12 B; symbolic kernels read columns
-    only, 4 B), C written once (12 B per nnz).  Upper-level merges read and
-    write 12 B per element (k_group: each input element once, each output
-    once; k_bgroup likewise, although elements loaded into a round's window
-    beyond the round's bound are read again in the next round).  Ping-pong
-    traffic inside LDS is not counted."""
+def kernel_bytes(name, A, L, ip, nnz_row, tier):
+    """Algorithmic bytes of all launches of kernel `name` in one SpGEMM, by
+    the per-product model bytes_alg = 4(M+1) + 20 nnzA + 12 ip + 8(M+1) +
+    12 nnz(C) restricted to the rows the kernel finishes: per row, its A
+    entries (col 4 + val 8 + product offset 8 = 20 B each), one B gather per
+    product (col 4 + val 8 = 12 B; the symbolic kernels read columns only,
+    4 B), C written once (12 B per nnz) and its row pointers (4 B in, 8 B
+    out).  Traffic between tree levels (LDS ping-pong, the long-row engine's
+    workspace passes) is never counted: it is overhead, reported apart."""
     sel = {"k_short_numeric": 1, "k_short_count": 1, "k_medium": 2, "k_sym_hash": 2, "k_sym_bitmap": 3,
-           "k_medium_w": 4, "k_rmed": 4, "k_sym_hash_w": 4}
-    if name in sel:
-        m = tier == sel[name]
-        nA, nip, nc = int(L[m].sum()), int(ip[m].sum()), int(nnz_row[m].sum())
-        if name in ("k_short_numeric", "k_medium", "k_medium_w", "k_rmed"):
-            return 20 * nA + 12 * nip + 12 * nc + 16 * int(m.sum())
-        return 12 * nA + 4 * nip + 8 * int(m.sum())
+           "k_medium_w": 4, "k_rmed": 4, "k_sym_hash_w": 4,
+           "k_bgroup": 3, "k_group": 3, "k_leaf": 3}
+    if name not in sel:
+        return None
+    m = tier == sel[name]
+    nA, nip, nc, nr = int(L[m].sum()), int(ip[m].sum()), int(nnz_row[m].sum()), int(m.sum())
+    if name in ("k_short_count", "k_sym_hash", "k_sym_bitmap", "k_sym_hash_w"):
+        return 12 * nA + 4 * nip + 12 * nr
+    return 20 * nA + 12 * nip + 12 * nc + 12 * nr
+
+
+def engine_bytes(name, st):
+    """Bytes the long-row merge kernels move by their own accounting: every
+    element a pass reads (12 B) and writes (12 B), plus 48 B per node
+    descriptor.  Minus the algorithmic bytes of the long rows, this is the
+    workspace traffic of the intermediate tree levels."""
     if name == "k_leaf":
         return 12 * st["leaf_ip"] + 12 * st["leaf_out"] + 20 * st["leaf_tasks"]
     if name in ("k_group", "k_bgroup"):
@@ -89,42 +97,118 @@ def measured_traffic(kernel, workload):
     return int(t["read_bytes_per_launch"] + t["write_bytes_per_launch"])
 
 
-def cpu_baseline(A, target_s=20.0):
-    """Time the oracle's BRMerge-Precise on A*A, or on a row panel of A
-    (every stride-th row) times A when the whole product is too large for
-    the reference's host run: 1 warm-up, then the mean of as many runs as
-    fit in about target_s seconds (at least 1, at most 1000)."""
-    import oracle
-    from brmerge_amd import CSR
+def host_cpu():
+    """The host's CPU model and socket count (lscpu, else /proc/cpuinfo)."""
+    model, sockets = None, None
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name" and model is None:
+                model = v.strip()
+            elif k.strip() == "Socket(s)":
+                try:
+                    sockets = int(v.strip())
+                except ValueError:
+                    pass
+    except (OSError, ValueError):
+        pass
+    if model is None or sockets is None:
+        try:
+            ids = set()
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    k, _, v = line.partition(":")
+                    if k.strip() == "model name" and model is None:
+                        model = v.strip()
+                    elif k.strip() == "physical id":
+                        ids.add(v.strip())
+            if sockets is None and ids:
+                sockets = len(ids)
+        except OSError:
+            pass
+    return model, sockets
 
-    cores = min(16, os.cpu_count() or 1)
+
+PANEL_ROWS = 32768
+PANEL_SEED = 12345
+
+
+def panel(A, ip):
+    """Rows the CPU baseline runs and the GPU result is checked on: all rows
+    when the product has at most 2e8 products, else a seeded uniform random
+    panel of PANEL_ROWS rows."""
+    if int(ip.sum()) <= 200_000_000:
+        return np.arange(A.M), "the whole product"
+    rng = np.random.default_rng(PANEL_SEED)
+    rows = np.sort(rng.choice(A.M, size=min(PANEL_ROWS, A.M), replace=False))
+    return rows, f"a uniform random panel of {len(rows)} rows (numpy default_rng({PANEL_SEED}))"
+
+
+def cpu_baseline(A, rows, what, calls=10):
+    """Time the oracle's BRMerge-Precise (the OpenMP restatement of
+    spgemm.h:468-735) on the host's cores beside the GPU run, with the
+    reference driver's protocol (brmerge_precise.cpp:67-83): 1 warm-up call,
+    then the mean of `calls` calls, each timed around the whole call
+    (output allocation included).  The whole product runs when it has at
+    most 2e8 products; otherwise a seeded uniform random panel of
+    `panel_rows` rows of A times A (the reference's int32 row pointer and
+    host memory cannot hold R-MAT-20's full C).  Threads: every CPU this
+    process may run on, capped by OMP_NUM_THREADS when that is set."""
+    visible = len(os.sched_getaffinity(0))
+    env_thr = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(visible, int(env_thr)) if env_thr.isdigit() and int(env_thr) > 0 else visible
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    import oracle
+
     L, ip = row_stats(A, A)
-    stride = 1 if int(ip.sum()) <= 200_000_000 else 64
-    rows = np.arange(0, A.M, stride)
     rpt = np.asarray(A.rpt, np.int64)
     lens = rpt[rows + 1] - rpt[rows]
     prpt = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
-    idx = np.concatenate([np.arange(rpt[r], rpt[r + 1]) for r in rows]) if len(rows) else np.zeros(0, np.int64)
+    idx = (np.concatenate([np.arange(rpt[r], rpt[r + 1]) for r in rows]) if len(rows)
+           else np.zeros(0, np.int64))
     P = (prpt, A.col[idx], A.val[idx], A.N)
     B = (np.asarray(A.rpt, np.int32), A.col, A.val, A.N)
     ip_panel = int(ip[rows].sum())
-    t0 = time.perf_counter()
-    oracle.brmerge_precise(P, B, nthreads=cores)  # warm-up (reference protocol: 1 + mean of runs)
-    t1 = time.perf_counter() - t0
-    reps = int(max(1, min(1000, round(target_s / max(t1, 1e-4)))))
+    oracle.brmerge_precise(P, B, nthreads=threads)  # warm-up
     ts = []
-    for _ in range(reps):
+    for _ in range(calls):
         t0 = time.perf_counter()
-        oracle.brmerge_precise(P, B, nthreads=cores)
+        C = oracle.brmerge_precise(P, B, nthreads=threads)
         ts.append(time.perf_counter() - t0)
     t = float(np.mean(ts))
-    what = "all rows" if stride == 1 else f"rows i%{stride}==0"
+    model, sockets = host_cpu()
     return {
-        "value": round(2.0 * ip_panel / t / 1e9, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-        "sample": (f"oracle BRMerge-Precise (OpenMP C restatement of spgemm.h:468-735) on {what} "
-                   f"of the same A ({len(rows)} rows, {ip_panel} products) times A; 1 warm-up + mean of {reps}"),
-        "seconds_per_run": round(t, 4),
-    }
+        "value": round(2.0 * ip_panel / t / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+        "sample": (f"oracle BRMerge-Precise (OpenMP C restatement of spgemm.h:468-735) on {what} of the same A "
+                   f"times A: {len(rows)} rows, {ip_panel} products, nnz(C) {C.nnz}; "
+                   f"1 warm-up + mean of {calls} calls"),
+        "seconds_per_call": round(t, 4), "calls": calls, "panel_rows": int(len(rows)), "panel_products": ip_panel,
+        "visible_cpus": visible, "os_cpu_count": os.cpu_count(), "cpu_model": model, "sockets": sockets,
+        "omp": {"OMP_NUM_THREADS": env_thr or None, "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"),
+                "OMP_PLACES": os.environ.get("OMP_PLACES")},
+    }, C
+
+
+def download_panel(C, rows):
+    """(row pointer diffs, columns, values) of `rows` of a DeviceCSR result."""
+    rpt = C.rpt.to_host()
+    lens = rpt[rows + 1] - rpt[rows]
+    cols = [C.col.to_host(int(rpt[r]), int(rpt[r + 1])) for r in rows]
+    vals = [C.val.to_host(int(rpt[r]), int(rpt[r + 1])) for r in rows]
+    return (lens, np.concatenate(cols) if cols else np.zeros(0, np.int32),
+            np.concatenate(vals) if vals else np.zeros(0))
+
+
+def panel_matches(got, R):
+    """Bit-for-bit equality of a downloaded panel with the oracle's C."""
+    lens, col, val = got
+    return (np.array_equal(lens, np.diff(R.rpt)) and np.array_equal(col, R.col)
+            and np.array_equal(val.view(np.uint64), R.val.view(np.uint64)))
+
 
 def _stats_ip(A, B):
     return int(row_stats(A, B)[1].sum())
@@ -346,7 +430,7 @@ def main():
     ap.add_argument("--requests", type=int, default=256, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: offered requests/s (0: half of saturation)")
     ap.add_argument("--max-batch", type=int, default=64, help="serve: requests fused per batch")
-    ap.add_argument("--upper", action="store_true", help="also time BRMerge-Upper")
+    ap.add_argument("--no-upper", action="store_true", help="do not time BRMerge-Upper")
     ap.add_argument("--vendor", action="store_true",
                     help="also time rocSPARSE SpGEMM on the same product (int32 indices: nnz(C) < 2^31)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -395,6 +479,7 @@ def main():
     total_ip = int(ip.sum())
     dA = bm.DeviceCSR.upload(A)
     dB = dA if B is A else bm.DeviceCSR.upload(B)
+    rows_chk, rows_what = panel(A, ip)
 
     print(f"[bench] {args.workload}: M={A.M} nnzA={A.nnz} ip={total_ip} (generated in {t_gen:.1f}s)",
           file=sys.stderr, flush=True)
@@ -422,6 +507,7 @@ def main():
     st = bm.last_stats()
     bm.set_param("profile", 0)
     crpt = last.rpt.to_host()
+    got_precise = download_panel(last, rows_chk) if B is A else None
     last.free()
     nnz_row = np.diff(crpt)
 
@@ -439,15 +525,21 @@ def main():
     roof = None
     if dom is not None:
         n, ms = prof[dom]
-        by = kernel_bytes(dom, A, L, ip, nnz_row, tier, st)
+        by = kernel_bytes(dom, A, L, ip, nnz_row, tier)
         if by is not None:
             by_per_launch = by * args.steps / n
             avg_s = ms / n / 1e3
             ach = by_per_launch / avg_s / 1e9
+            traffic = measured_traffic(dom, args.workload)
             roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic(dom, args.workload),
-                    "avg_launch_ms": round(ms / n, 4), "alg_bytes_per_launch": int(by_per_launch)}
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "avg_launch_ms": round(ms / n, 4), "alg_bytes_per_launch": int(by_per_launch),
+                    "launches_per_step": round(n / args.steps, 2),
+                    "traffic_over_alg": round(traffic / by_per_launch, 3) if traffic else None}
+            eb = engine_bytes(dom, st)
+            if eb is not None:
+                roof["engine_bytes_per_launch"] = int(eb / (n / args.steps))
+                roof["workspace_bytes_per_step"] = int(eb - by)
 
     out = {
         "metric": "SpGEMM GFLOP/s (2*intermediate products/s), BRMerge-Precise, R-MAT A*A",
@@ -473,16 +565,25 @@ def main():
         "kernels": kernels,
         "stats": st,
     }
-    if args.upper:
-        t = bm.Upper_Timing()
-        ts = []
+    got_upper = None
+    if not args.no_upper:
+        C = bm.spgemm_dev(dA, dB, upper=True)  # warm-up
+        C.free()
+        ts, tph = [], []
         for _ in range(max(1, args.steps)):
+            t = bm.Upper_Timing()
             t0 = time.perf_counter()
             C = bm.spgemm_dev(dA, dB, upper=True, timing=t)
             ts.append(time.perf_counter() - t0)
+            tph.append(t.as_dict())
+            print(f"[bench] upper step {_}: {ts[-1]:.3f}s nnz(C)={C.nnz}", file=sys.stderr, flush=True)
+            if _ == max(1, args.steps) - 1 and B is A:
+                got_upper = download_panel(C, rows_chk)
             C.free()
         out["upper"] = {"value": round(2.0 * total_ip / np.mean(ts) / 1e9, 3), "unit": "GFLOP/s",
-                        "ms_per_step": round(1000 * float(np.mean(ts)), 3)}
+                        "ms_per_step": round(1000 * float(np.mean(ts)), 3), "steps": len(ts), "warmup": 1,
+                        "vs_precise": round(float(np.mean(ts)) / (elapsed / args.steps), 3),
+                        "phases_ms": {k: round(1000 * float(np.mean([p[k] for p in tph])), 3) for k in tph[0]}}
     if args.vendor:
         if nnz_c >= 2 ** 31:
             out["rocsparse"] = {"skipped": "nnz(C) exceeds the int32 indices of the rocSPARSE path"}
@@ -501,8 +602,14 @@ def main():
     dA.free()
     if dB is not dA:
         dB.free()
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(A) if B is A else None
+    if not args.no_cpu and B is A:
+        out["cpu_baseline"], R = cpu_baseline(A, rows_chk, rows_what)
+        # The reference's driver checks every run's C (brmerge_precise.cpp:86-93); here the
+        # GPU's C is compared bit for bit with the oracle's on the baseline's rows.
+        out["verified"] = {"rows": rows_what, "products": int(ip[rows_chk].sum()),
+                           "precise_bit_identical": bool(panel_matches(got_precise, R)),
+                           "upper_bit_identical": (bool(panel_matches(got_upper, R))
+                                                   if got_upper is not None else None)}
     print(json.dumps(out), flush=True)
 
 

@@ -17,6 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbrmerge.so")
 DRIVER = os.path.join(HERE, "brm_driver")
 XCHECK = os.path.join(HERE, "libbrm_xcheck.so")
+COMPAT_CALLER = os.path.join(os.path.dirname(HERE), "tests", "compat", "compat_caller")
 
 HIP_SOURCES = ["alloc.hip", "scan.hip", "flop.hip", "short.hip", "lds.hip", "long.hip", "assemble.hip", "api.hip",
                "batch.hip", "serve.hip", "bmerge.hip", "rmed.hip",
@@ -75,6 +76,11 @@ def build(verbose=False):
           os.path.join(CSRC, "driver.cpp"), "-o", DRIVER + ".tmp", "-L", HERE, "-lbrmerge",
           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + "/opt/rocm/lib"])
     os.replace(DRIVER + ".tmp", DRIVER)
+    # a test caller of the reference's C++ interface, built the same way
+    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(os.path.dirname(HERE), "include"),
+          COMPAT_CALLER + ".cpp", "-o", COMPAT_CALLER + ".tmp", "-L", HERE, "-lbrmerge",
+          "-Wl,-rpath,$ORIGIN/../../brmerge_amd", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(COMPAT_CALLER + ".tmp", COMPAT_CALLER)
     # vendor cross-check (rocSPARSE), a separate test/bench library
     _run([hipcc, "-O2", "-fPIC", "-shared", "-std=c++17", os.path.join(HERE, "xcheck", "rocsparse_xcheck.cpp"),
           "-o", XCHECK + ".tmp", "-L/opt/rocm/lib", "-lrocsparse", "-Wl,-rpath,/opt/rocm/lib"])

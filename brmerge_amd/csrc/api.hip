@@ -31,7 +31,7 @@
 namespace brm {
 
 Params& params() {
-    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, -1, 0, 1};
+    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, -1, 0, 1, 0, (int64_t)1 << 30};
     return p;
 }
 
@@ -39,9 +39,12 @@ namespace {
 thread_local std::string t_err;
 thread_local brm_stats t_stats;
 thread_local int t_device = 0;
+thread_local int64_t t_ws_budget = 0;
 }  // namespace
 
 brm_stats& thread_stats() { return t_stats; }
+int64_t thread_ws_budget() { return t_ws_budget; }
+void set_thread_ws_budget(int64_t bytes) { t_ws_budget = bytes; }
 int thread_device() { return t_device; }
 void set_thread_device(int device) { t_device = device; }
 
@@ -73,32 +76,53 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Phase events are recycled through a per-thread pool: creating and
+// destroying some twenty events per call costs more host time than a small
+// product's kernels.
+namespace {
+thread_local std::vector<hipEvent_t> t_event_pool;
+hipEvent_t event_get() {
+    if (!t_event_pool.empty()) {
+        hipEvent_t e = t_event_pool.back();
+        t_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+void event_put(hipEvent_t e) {
+    if (!e) return;
+    if (t_event_pool.size() < 256) t_event_pool.push_back(e);
+    else (void)hipEventDestroy(e);
+}
+}  // namespace
+
 PhaseClock::~PhaseClock() {
     for (auto& m : marks) {
-        (void)hipEventDestroy(m.second.first);
-        (void)hipEventDestroy(m.second.second);
+        event_put(m.second.first);
+        event_put(m.second.second);
     }
-    if (open_ev) (void)hipEventDestroy(open_ev);
+    event_put(open_ev);
 }
 
 void PhaseClock::begin(int p) {
+    if (!on) return;
     end();
-    if (hipEventCreate(&open_ev) != hipSuccess) {
-        open_ev = nullptr;
-        return;
-    }
+    open_ev = event_get();
+    if (!open_ev) return;
     (void)hipEventRecord(open_ev, s);
     open_phase = p;
 }
 
 void PhaseClock::end() {
     if (open_phase < 0 || !open_ev) return;
-    hipEvent_t e1;
-    if (hipEventCreate(&e1) == hipSuccess) {
+    hipEvent_t e1 = event_get();
+    if (e1) {
         (void)hipEventRecord(e1, s);
         marks.push_back({open_phase, {open_ev, e1}});
     } else {
-        (void)hipEventDestroy(open_ev);
+        event_put(open_ev);
     }
     open_ev = nullptr;
     open_phase = -1;
@@ -184,11 +208,12 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     sc.free_now(ip_d);
     sc.free_now(len_d);
 
-    // Wave partition by workspace budget.  The budget is quantised (4 GiB)
-    // and the workspace rounded up (2^27 elements) so that repeated calls of
-    // the same product size their buffers identically and reuse the cached
-    // blocks of the previous call.
+    // Wave partition by workspace budget: the ws_budget_bytes knob, else the
+    // calling thread's share (set by the request server's workers), else 70%
+    // of the memory this call can get, quantised to 4 GiB so that repeated
+    // calls of one product plan identical waves.
     int64_t budget = params().ws_budget_bytes;
+    if (budget <= 0) budget = thread_ws_budget();
     if (budget <= 0) {
         size_t fr = 0, tot = 0;
         HIPC(hipMemGetInfo(&fr, &tot));
@@ -254,13 +279,22 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         k0 = k1;
     }
 
-    auto round_up = [](int64_t n, int64_t g) { return (n + g - 1) / g * g; };
-    mx_ws = round_up(mx_ws, (int64_t)1 << 27);
-    mx_sl = round_up(mx_sl, (int64_t)1 << 24);
-    mx_leaf = round_up(mx_leaf, (int64_t)1 << 20);
-    mx_child = round_up(mx_child, (int64_t)1 << 20);
-    mx_task = round_up(mx_task, (int64_t)1 << 20);
-    mx_rows = (int)round_up(mx_rows, (int64_t)1 << 16);
+    // Buffers are rounded up to one eighth of their next power of two (at
+    // most 12.5% more than needed, with a small floor), so that products of
+    // similar size reuse each other's cached blocks.
+    auto round_geo = [](int64_t n, int64_t floor) {
+        if (n <= floor) return floor;
+        int64_t p2 = 1;
+        while (p2 < n) p2 <<= 1;
+        const int64_t g = p2 / 8;
+        return (n + g - 1) / g * g;
+    };
+    mx_ws = round_geo(mx_ws, (int64_t)1 << 20);
+    mx_sl = round_geo(mx_sl, (int64_t)1 << 16);
+    mx_leaf = round_geo(mx_leaf, (int64_t)1 << 14);
+    mx_child = round_geo(mx_child, (int64_t)1 << 14);
+    mx_task = round_geo(mx_task, (int64_t)1 << 14);
+    mx_rows = (int)round_geo(mx_rows, (int64_t)1 << 10);
     Scratch wsc;
     int *rows_w, *wcol0, *wcol1;
     double *wval0, *wval1;
@@ -509,8 +543,185 @@ int pattern_numeric_long(const Pattern& p, int* ccol, double* cval, hipStream_t 
     return run_long(p.a, p.rows(BIN_LONG), p.bc[BIN_LONG], o, p.crpt, s, sc);
 }
 
+namespace {
+// C-bar bytes one Upper row wave may use: the cbar_budget_bytes knob, or a
+// quarter of the device memory this call can get (free plus the allocator's
+// cached blocks), at most an eighth of the device, in whole GiB.
+int upper_wave_budget(int64_t* out) {
+    int64_t budget = params().cbar_budget_bytes;
+    if (budget <= 0) {
+        size_t fr = 0, tot = 0;
+        HIPC(hipMemGetInfo(&fr, &tot));
+        budget = std::min((int64_t)(fr + dev_cached_bytes()) / 4, (int64_t)tot / 8);
+        budget = std::max(budget / (int64_t(1) << 30), (int64_t)1) * (int64_t(1) << 30);
+    }
+    *out = budget;
+    return BRM_OK;
+}
+
+// BRMerge-Upper numeric phase and assembly (spgemm.h:293-445): every row is
+// merged into C-bar at its product offset ipA[arpt[row]] (the reference's
+// scan_floprC[row], :399-400) with its merged length in rowcnt, then C's row
+// pointer is scanned (:413) and C-bar compacted into C (:427-445).
+//
+// The reference holds C-bar for every row at once (12 bytes per product,
+// 251 GB for R-MAT scale 20).  Here rows run in waves of consecutive rows
+// whose C-bar fits upper_wave_budget(): each wave is re-binned, merged into
+// one reused C-bar buffer (addressed relative to the wave's first product),
+// and compacted into a chunk of exactly its own nnz.  Once every wave is done
+// C is allocated at nnz(C) and the chunks are copied into place.  A product
+// whose C-bar fits one wave is compacted straight into C.
+int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int** ccol_out, double** cval_out,
+              int64_t* nnz_out) {
+    const int M = p.a.M;
+    Scratch& sc = p.sc;
+    clk.begin(P_ALLOC);
+    int64_t budget = 0;
+    BRMC(upper_wave_budget(&budget));
+    const int64_t cap_ip = std::max<int64_t>(1, budget / 12);
+    // Wave boundaries (row indices) and each wave's first product.
+    std::vector<int> wb{0};
+    std::vector<int64_t> wbase{0};
+    int64_t wave_ip_max = p.total_ip;
+    if (p.total_ip > cap_ip) {
+        std::vector<int64_t> iprow(M);
+        BRMC(d2h(iprow.data(), p.iprow, sizeof(int64_t) * M, s));
+        int64_t acc = 0, base = 0;
+        wave_ip_max = 0;
+        for (int r = 0; r < M; ++r) {
+            if (acc > 0 && acc + iprow[r] > cap_ip) {
+                wb.push_back(r);
+                base += acc;
+                wbase.push_back(base);
+                wave_ip_max = std::max(wave_ip_max, acc);
+                acc = 0;
+            }
+            acc += iprow[r];
+        }
+        wave_ip_max = std::max(wave_ip_max, acc);
+    }
+    wb.push_back(M);
+    const int nwaves = (int)wb.size() - 1;
+    int* bcol_bar;
+    double* bval_bar;
+    BRMC(sc.alloc(&bcol_bar, wave_ip_max, "C-bar col"));
+    BRMC(sc.alloc(&bval_bar, wave_ip_max, "C-bar val"));
+    thread_stats().upper_waves = nwaves;
+
+    int64_t* crpt_w = nullptr;
+    if (nwaves > 1) {
+        int mrows = 0;
+        for (int w = 0; w < nwaves; ++w) mrows = std::max(mrows, wb[w + 1] - wb[w]);
+        BRMC(sc.alloc(&crpt_w, (int64_t)mrows + 1, "wave row pointer"));
+    }
+    Scratch chunks;
+    std::vector<int*> ch_col;
+    std::vector<double*> ch_val;
+    std::vector<int64_t> ch_n;
+    const LdsShape shape = lds_shape(p.lds_cfg);
+    for (int w = 0; w < nwaves; ++w) {
+        RowArgs aw = p.a;
+        aw.arpt = p.a.arpt + wb[w];
+        aw.M = wb[w + 1] - wb[w];
+        int bc[BIN_COUNT];
+        const int* bins = p.bins;
+        if (nwaves == 1) {
+            std::copy(p.bc, p.bc + BIN_COUNT, bc);
+        } else {
+            // bins of this wave's rows (row ids relative to its first row)
+            clk.begin(P_LB);
+            HIPC(launch_fill_i32(p.bin_cnt, BIN_COUNT, 0, s));
+            HIPC(launch_rowstats(aw, p.med_cap, shape.lmax, p.wmed_cap, params().force_long, p.iprow + wb[w],
+                                 p.rowcnt + wb[w], p.bins, p.bin_cnt, p.derr, s));
+            BRMC(d2h(bc, p.bin_cnt, sizeof(bc), s));
+        }
+        auto rows = [&](int bin) { return bins + (int64_t)(bin - 1) * aw.M; };
+        clk.begin(P_COMPUTE);
+        // C-bar holds products [wbase[w], wbase[w] + wave ip): kernels index it
+        // by the global ipA, so its base pointers are shifted back by wbase[w].
+        OutArgs o{};
+        o.err = p.derr;
+        o.upper = 1;
+        o.ccol = bcol_bar - wbase[w];
+        o.cval = bval_bar - wbase[w];
+        o.rowlen = p.rowcnt + wb[w];
+        HIPC(launch_short_numeric(rows(BIN_TINY), bc[BIN_TINY], aw, o, s, 32));
+        HIPC(launch_short_numeric(rows(BIN_SHORT), bc[BIN_SHORT], aw, o, s));
+        HIPC(launch_wmed_numeric(rows(BIN_WMED), bc[BIN_WMED], aw, o, s, std::min(p.wmed_cap, WMED_SPLIT0)));
+        HIPC(launch_wmed_numeric(rows(BIN_WMED2), bc[BIN_WMED2], aw, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
+        HIPC(launch_wmed_numeric(rows(BIN_WMED3), bc[BIN_WMED3], aw, o, s, p.wmed_cap));
+        HIPC(launch_medium_numeric(rows(BIN_MEDIUM), bc[BIN_MEDIUM], aw, o, s, p.lds_cfg));
+        BRMC(run_long(aw, rows(BIN_LONG), bc[BIN_LONG], o, nullptr, s, sc));
+        BRMC(check_dev_err(p.derr, s, "numeric"));
+        if (nwaves == 1) {
+            clk.begin(P_PNNZ);
+            HIPC(scan_i64(p.rowcnt, M, p.crpt, s));
+            int64_t nnzC = 0;
+            BRMC(d2h(&nnzC, p.crpt + M, sizeof(int64_t), s));
+            if (nnzC > max_nnz) {
+                set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
+                return BRM_E_OVERFLOW;
+            }
+            clk.begin(P_ALLOC);
+            BRMC(sc.alloc(ccol_out, nnzC, "C col"));
+            BRMC(sc.alloc(cval_out, nnzC, "C val"));
+            clk.begin(P_COPY);
+            HIPC(launch_compact(p.crpt, M, aw, o.ccol, o.cval, *ccol_out, *cval_out, nnzC, s));
+            *nnz_out = nnzC;
+            return BRM_OK;
+        }
+        clk.begin(P_PNNZ);
+        HIPC(scan_i64(p.rowcnt + wb[w], aw.M, crpt_w, s));
+        int64_t nw = 0;
+        BRMC(d2h(&nw, crpt_w + aw.M, sizeof(int64_t), s));
+        clk.begin(P_ALLOC);
+        int* cc;
+        double* cv;
+        BRMC(chunks.alloc(&cc, nw, "C chunk col"));
+        BRMC(chunks.alloc(&cv, nw, "C chunk val"));
+        ch_col.push_back(cc);
+        ch_val.push_back(cv);
+        ch_n.push_back(nw);
+        clk.begin(P_COPY);
+        HIPC(launch_compact(crpt_w, aw.M, aw, o.ccol, o.cval, cc, cv, nw, s));
+        dev_flush_deferred();
+    }
+    // C-bar is no longer needed: return it before C is allocated.
+    HIPC(hipStreamSynchronize(s));
+    sc.free_now(bcol_bar);
+    sc.free_now(bval_bar);
+    clk.begin(P_PNNZ);
+    HIPC(scan_i64(p.rowcnt, M, p.crpt, s));
+    int64_t nnzC = 0;
+    BRMC(d2h(&nnzC, p.crpt + M, sizeof(int64_t), s));
+    if (nnzC > max_nnz) {
+        set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
+        return BRM_E_OVERFLOW;
+    }
+    clk.begin(P_ALLOC);
+    BRMC(sc.alloc(ccol_out, nnzC, "C col"));
+    BRMC(sc.alloc(cval_out, nnzC, "C val"));
+    clk.begin(P_COPY);
+    int64_t off = 0;
+    for (size_t w = 0; w < ch_n.size(); ++w) {
+        if (ch_n[w] > 0) {
+            HIPC(hipMemcpyAsync(*ccol_out + off, ch_col[w], sizeof(int) * ch_n[w], hipMemcpyDeviceToDevice, s));
+            HIPC(hipMemcpyAsync(*cval_out + off, ch_val[w], sizeof(double) * ch_n[w], hipMemcpyDeviceToDevice, s));
+        }
+        off += ch_n[w];
+    }
+    if (off != nnzC) {
+        set_err("Upper row waves: chunk sizes (%lld) disagree with nnz(C) (%lld)", (long long)off, (long long)nnzC);
+        return BRM_E_INTERNAL;
+    }
+    HIPC(hipStreamSynchronize(s));
+    *nnz_out = nnzC;
+    return BRM_OK;
+}
+}  // namespace
+
 int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph) {
-    PhaseClock clk(s);
+    PhaseClock clk(s, ph != nullptr);
     memset(&t_stats, 0, sizeof(t_stats));
     HostTrace tr(s);
     tr.mark("spgemm start", in.M);
@@ -558,40 +769,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
             BRMC(check_dev_err(p.derr, s, "numeric"));
         }
     } else {
-        clk.begin(P_ALLOC);
-        int* bcol_bar;
-        double* bval_bar;
-        BRMC(sc.alloc(&bcol_bar, p.total_ip, "C-bar col"));
-        BRMC(sc.alloc(&bval_bar, p.total_ip, "C-bar val"));
-
-        clk.begin(P_COMPUTE);
-        OutArgs o{};
-        o.err = p.derr;
-        o.upper = 1;
-        o.ccol = bcol_bar;
-        o.cval = bval_bar;
-        o.rowlen = p.rowcnt;
-        HIPC(launch_short_numeric(p.rows(BIN_TINY), p.bc[BIN_TINY], p.a, o, s, 32));
-        HIPC(launch_short_numeric(p.rows(BIN_SHORT), p.bc[BIN_SHORT], p.a, o, s));
-        HIPC(launch_wmed_numeric(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT0)));
-        HIPC(launch_wmed_numeric(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
-        HIPC(launch_wmed_numeric(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, o, s, p.wmed_cap));
-        HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
-        BRMC(run_long(p.a, p.rows(BIN_LONG), p.bc[BIN_LONG], o, nullptr, s, sc));
-        BRMC(check_dev_err(p.derr, s, "numeric"));
-
-        clk.begin(P_PNNZ);
-        HIPC(scan_i64(p.rowcnt, M, p.crpt, s));
-        BRMC(d2h(&nnzC, p.crpt + M, sizeof(int64_t), s));
-        if (nnzC > max_nnz) {
-            set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
-            return BRM_E_OVERFLOW;
-        }
-        clk.begin(P_ALLOC);
-        BRMC(sc.alloc(&ccol, nnzC, "C col"));
-        BRMC(sc.alloc(&cval, nnzC, "C val"));
-        clk.begin(P_COPY);
-        HIPC(launch_compact(p.crpt, M, p.a, bcol_bar, bval_bar, ccol, cval, nnzC, s));
+        BRMC(run_upper(p, max_nnz, s, clk, &ccol, &cval, &nnzC));
     }
     clk.begin(P_CLEAN);
     HIPC(hipStreamSynchronize(s));
@@ -957,6 +1135,8 @@ int brm_set_param(const char* name, int64_t value) {
     else if (!strcmp(name, "wmed_cap")) { if (value < -1 || value > WMED_CAP) return BRM_E_INVALID; p.wmed_cap = (int)value; }
     else if (!strcmp(name, "launch_chunk")) { if (value < 0) return BRM_E_INVALID; p.launch_chunk = value; }
     else if (!strcmp(name, "wmed_engine")) { if (value < 0 || value > 1) return BRM_E_INVALID; p.wmed_engine = (int)value; }
+    else if (!strcmp(name, "cbar_budget_bytes")) { if (value < 0) return BRM_E_INVALID; p.cbar_budget_bytes = value; }
+    else if (!strcmp(name, "sym_chunk_products")) { if (value < 1 || value > ((int64_t)1 << 30)) return BRM_E_INVALID; p.sym_chunk_products = value; }
     else return BRM_E_INVALID;
     return BRM_OK;
 }
@@ -974,6 +1154,8 @@ int64_t brm_get_param(const char* name) {
     if (!strcmp(name, "wmed_cap")) return p.wmed_cap;
     if (!strcmp(name, "launch_chunk")) return p.launch_chunk;
     if (!strcmp(name, "wmed_engine")) return p.wmed_engine;
+    if (!strcmp(name, "cbar_budget_bytes")) return p.cbar_budget_bytes;
+    if (!strcmp(name, "sym_chunk_products")) return p.sym_chunk_products;
     return -1;
 }
 
@@ -985,6 +1167,20 @@ int brm_bench_copy(void* dst, const void* src, uint64_t bytes, void* stream) {
     HIPC(launch_copy(dst, src, bytes, (hipStream_t)stream));
     HIPC(hipStreamSynchronize((hipStream_t)stream));
     return BRM_OK;
+}
+
+int brm_checksum_dev(const void* p, uint64_t bytes, uint64_t* out, void* stream) {
+    if (!out || (bytes & 3) || (bytes > 0 && !p)) {
+        set_err("invalid argument");
+        return BRM_E_INVALID;
+    }
+    BRMC(use_device());
+    hipStream_t s = (hipStream_t)stream;
+    Scratch sc;
+    uint64_t* d;
+    BRMC(sc.alloc(&d, 1, "checksum"));
+    HIPC(launch_checksum(p, (int64_t)(bytes / 4), d, s));
+    return d2h(out, d, sizeof(uint64_t), s);
 }
 
 int brm_last_stats(brm_stats* out) {

@@ -5,6 +5,8 @@
 // into C at crpt[row] (spgemm.h:427-445).  Each workgroup owns a contiguous
 // range of C, locates the row of its first element by binary search on crpt
 // and walks rows forward, so reads and writes are both coalesced.
+#include <algorithm>
+
 #include "brm_device.h"
 #include "kernels.h"
 
@@ -36,6 +38,23 @@ __global__ __launch_bounds__(CP_NT) void k_compact(const int64_t* __restrict__ c
 // Plain streaming copy (16-byte vectors), used to calibrate bandwidth counters.
 __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+// Order-independent 64-bit checksum of 4-byte words: the sum (mod 2^64) of
+// a SplitMix64 finalisation of (word index, word).  Two arrays that differ
+// in any word differ in the sum except with probability about 2^-64.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ w, int64_t n, uint64_t* out) {
+    uint64_t acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        acc += mix64((uint64_t)i * 0x9E3779B97F4A7C15ull + w[i]);
+    acc = wave_sum(acc);
+    if (lane_id() == 0) atomicAdd((unsigned long long*)out, (unsigned long long)acc);
 }
 
 __global__ void k_rpt_to_i32(const int64_t* __restrict__ in, int64_t n, int* __restrict__ out) {
@@ -73,6 +92,17 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
     {
         ProfScope ps_("k_copy", s);
         k_copy<<<2048, 256, 0, s>>>((const uint4*)src, (uint4*)dst, n16);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_checksum(const void* p, int64_t nwords, uint64_t* out, hipStream_t s) {
+    hipError_t e = launch_fill_i64((int64_t*)out, 1, 0, s);
+    if (e != hipSuccess || nwords <= 0) return e;
+    {
+        ProfScope ps_("k_checksum", s);
+        k_checksum<<<(unsigned)std::min<int64_t>(nblk(nwords, 256), 8192), 256, 0, s>>>((const uint32_t*)p, nwords,
+                                                                                      out);
     }
     return hipGetLastError();
 }

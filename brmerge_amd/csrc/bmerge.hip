@@ -908,13 +908,8 @@ hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* childre
                          int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
                          const OutArgs& o, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_bgroup, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)sizeof(BgSmem));
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    static SmemLimit lim;
+    if (hipError_t e = lim.raise((const void*)k_bgroup, (int)sizeof(BgSmem)); e != hipSuccess) return e;
     WsCols w{ws_col[0], ws_col[1], ws_val[0], ws_val[1]};
     {
         ProfScope ps_("k_bgroup", s);

@@ -4,6 +4,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/brmerge.h"
 
 namespace brm {
@@ -167,6 +169,8 @@ struct Params {
     int wmed_cap;              // wavefront tier: products per row (0 = off, <= WMED_CAP, -1 = by N)
     int64_t launch_chunk;      // test hook: rows/tasks per launch (0 = the grid limit)
     int wmed_engine;           // wavefront tier numeric: 0 LDS merge levels, 1 register bitonic sort
+    int64_t cbar_budget_bytes; // Upper: C-bar bytes per row wave (0 = auto)
+    int64_t sym_chunk_products; // long-row symbolic: products per list chunk (1 .. 2^30, default 2^30)
 };
 
 // Shapes of the LDS tree-merge kernels, selectable at run time.
@@ -193,6 +197,19 @@ inline int64_t launch_chunk(int threads_per_item) {
     const int64_t o = params().launch_chunk;
     return (o > 0 && o < lim) ? o : lim;
 }
+
+// ------------------------------------------------------------------ kernel attributes
+// Raises one kernel's dynamic LDS limit the first time any thread launches
+// it (worker threads of the request server may race to the first launch).
+// Declare one `static SmemLimit` per kernel instantiation.
+struct SmemLimit {
+    std::once_flag once;
+    hipError_t err = hipSuccess;
+    hipError_t raise(const void* kern, int bytes) {
+        std::call_once(once, [&] { err = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, bytes); });
+        return err;
+    }
+};
 
 // ------------------------------------------------------------------ profiling
 void prof_begin(const char* name, hipStream_t s);

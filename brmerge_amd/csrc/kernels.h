@@ -80,6 +80,8 @@ hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* childre
 hipError_t launch_compact(const int64_t* crpt, int M, const RowArgs& a, const int* cbar_col,
                           const double* cbar_val, int* ccol, double* cval, int64_t nnzC, hipStream_t s);
 hipError_t launch_rpt_to_i32(const int64_t* in, int64_t n, int* out, hipStream_t s);
+// *out = order-independent 64-bit checksum of nwords 4-byte words at p
+hipError_t launch_checksum(const void* p, int64_t nwords, uint64_t* out, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
 hipError_t launch_fill_i64(int64_t* p, int64_t n, int64_t v, hipStream_t s);
 hipError_t launch_fill_i32(int* p, int64_t n, int v, hipStream_t s);

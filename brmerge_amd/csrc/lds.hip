@@ -229,8 +229,25 @@ constexpr int BM_WORDS = BM_BITS / 32;
 constexpr int BM_CHUNK = 2048;
 constexpr int BM_U = 8;
 
+// A chunk's product offsets are kept in int: chunks are cut at BM_CHUNK lists
+// and at pcap products (at most 2^30, so offsets and the loop counters below
+// stay inside int).  One list is always taken: a B row has at most
+// INT32_MAX entries, and a list longer than pcap is its own chunk.
+__device__ __forceinline__ int sym_chunk_lists(const int64_t* __restrict__ ipA, int cs, int ae, int64_t pcap) {
+    int nl = min(BM_CHUNK, ae - cs);
+    const int64_t cb = ipA[cs];
+    if (ipA[cs + nl] - cb <= pcap) return nl;
+    int lo = 1, hi = nl;  // ipA[cs + hi] - cb > pcap
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ipA[cs + mid] - cb <= pcap) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(BM_NT) void k_sym_bitmap(const int* __restrict__ rows, int nrows, RowArgs a,
-                                                      int64_t* __restrict__ nnzrow) {
+                                                      int64_t* __restrict__ nnzrow, int64_t pcap) {
     extern __shared__ __align__(16) unsigned char smraw[];
     unsigned* bm = reinterpret_cast<unsigned*>(smraw);
     int* loff = reinterpret_cast<int*>(smraw + sizeof(unsigned) * BM_WORDS);
@@ -245,8 +262,8 @@ __global__ __launch_bounds__(BM_NT) void k_sym_bitmap(const int* __restrict__ ro
         const int words = (int)((span + 31) / 32);
         for (int w = t; w < words; w += BM_NT) bm[w] = 0u;
         __syncthreads();
-        for (int cs = ab; cs < ae; cs += BM_CHUNK) {
-            const int nl = min(BM_CHUNK, ae - cs);
+        for (int cs = ab, nl = 0; cs < ae; cs += nl) {
+            nl = sym_chunk_lists(a.ipA, cs, ae, pcap);
             const int64_t cb = a.ipA[cs];
             for (int k = t; k <= nl; k += BM_NT) {
                 loff[k] = (int)(a.ipA[cs + k] - cb);
@@ -302,12 +319,8 @@ hipError_t medium_numeric_t(const int* rows, int n, const RowArgs& a, const OutA
                             const char* name = "k_medium") {
     using SM = MergeSmem<NT, CAP, LMAX>;
     auto kern = k_medium<NT, CAP, LMAX>;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(SM));
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    static SmemLimit lim;
+    if (hipError_t e = lim.raise((const void*)kern, (int)sizeof(SM)); e != hipSuccess) return e;
     {
         ProfScope ps_(name, s);
         const int64_t ch = launch_chunk(NT);
@@ -324,12 +337,8 @@ hipError_t leaf_t(const LeafTask* tasks, int64_t n, const RowArgs& a, const WsPt
                   int64_t* counters, const OutArgs& o, hipStream_t s) {
     using SM = MergeSmem<NT, CAP, LMAX>;
     auto kern = k_leaf<NT, CAP, LMAX>;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(SM));
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    static SmemLimit lim;
+    if (hipError_t e = lim.raise((const void*)kern, (int)sizeof(SM)); e != hipSuccess) return e;
     {
         ProfScope ps_("k_leaf", s);
         const int64_t ch = launch_chunk(NT);
@@ -406,19 +415,14 @@ hipError_t launch_wmed_numeric(const int* rows, int n, const RowArgs& a, const O
 
 hipError_t launch_long_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_sym_bitmap, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)bitmap_smem());
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    static SmemLimit lim;
+    if (hipError_t e = lim.raise((const void*)k_sym_bitmap, (int)bitmap_smem()); e != hipSuccess) return e;
     {
         ProfScope ps_("k_sym_bitmap", s);
         const int64_t ch = launch_chunk(BM_NT);
         for (int64_t b = 0; b < n; b += ch) {
             const int c = (int)std::min<int64_t>(ch, n - b);
-            k_sym_bitmap<<<c, BM_NT, bitmap_smem(), s>>>(rows + b, c, a, nnzrow);
+            k_sym_bitmap<<<c, BM_NT, bitmap_smem(), s>>>(rows + b, c, a, nnzrow, params().sym_chunk_products);
         }
     }
     return hipGetLastError();

@@ -63,12 +63,14 @@ struct Scratch {
 enum Phase { P_PRE, P_FLOP, P_PFLOP, P_LB, P_SYM, P_PNNZ, P_ALLOC, P_COMPUTE, P_COPY, P_CLEAN, P_N };
 
 // hipEvent pairs around phases, summed per phase.
+// A clock constructed with on = false records nothing (no events created).
 struct PhaseClock {
     hipStream_t s;
+    bool on;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
     int open_phase = -1;
     hipEvent_t open_ev = nullptr;
-    explicit PhaseClock(hipStream_t st) : s(st) {}
+    explicit PhaseClock(hipStream_t st, bool on_ = true) : s(st), on(on_) {}
     PhaseClock(const PhaseClock&) = delete;
     PhaseClock& operator=(const PhaseClock&) = delete;
     ~PhaseClock();
@@ -149,11 +151,17 @@ int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_shor
 int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t s);
 int pattern_numeric_long(const Pattern& p, int* ccol, double* cval, hipStream_t s);
 
+// One SpGEMM.  ph (phase seconds, P_N entries) may be null: then no phase
+// events are recorded.
 int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph);
 
 // The calling thread's device for library calls (set by brm_init).
 int thread_device();
 void set_thread_device(int device);
+// Long-row workspace budget of the calling thread's calls (0: read the
+// device's free memory at each call).
+int64_t thread_ws_budget();
+void set_thread_ws_budget(int64_t bytes);
 
 // Fused operands of a batch: A_i stacked by rows with columns shifted to
 // their B_i's rows; B_i stacked by rows, or, when a_col_off is given, not

@@ -12,6 +12,9 @@
 // intermediate product plus its row pointer, 12*ip + 8*(M+1), against the
 // server budget; the reservation shrinks to the result's size when the
 // batch finishes and is returned when the caller releases the result.
+// Each worker's long-row workspace comes out of the same budget: an eighth
+// of it is split evenly among the workers (their per-call workspace budget),
+// and requests are admitted against the remaining seven eighths.
 // Worker threads (one HIP stream each) take every admissible queued request
 // (FIFO, up to max_batch) whenever they are free, so batches formed while
 // others run overlap on the device.
@@ -51,6 +54,7 @@ struct BatchMem {
 struct Request {
     int64_t id = 0;
     int b = 0;
+    int N = 0;  // columns of B (copied at submit: the B table may grow meanwhile)
     const int* arpt = nullptr;
     const int* acol = nullptr;
     const double* aval = nullptr;
@@ -77,7 +81,8 @@ using namespace brm;
 
 struct brm_server {
     int device = 0;
-    int64_t budget = 0;
+    int64_t budget = 0;     // admission budget for results
+    int64_t ws_share = 0;   // long-row workspace budget of each worker
     int max_batch = 64;
     std::vector<ResidentB> bs;
     // the stacked B (device), owned
@@ -197,6 +202,7 @@ int run_batch(brm_server* S, std::vector<std::shared_ptr<Request>>& batch, hipSt
 
 void worker_main(brm_server* S) {
     set_thread_device(S->device);
+    set_thread_ws_budget(S->ws_share);
     if (use_device() != BRM_OK) return;
     hipStream_t s = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return;
@@ -242,6 +248,8 @@ int stack_append(brm_server* S, const int* brpt, const int* bcol, const double* 
     FusedOut fo;
     BRMC(fuse_pairs(parts.data(), (int)parts.size(), nullptr, s, &fo));
     HIPC(hipStreamSynchronize(s));
+    // Readers of bs and stack hold mu (workers are idle while a B is added).
+    std::lock_guard<std::mutex> lk(S->mu);
     ResidentB rb{S->bs.empty() ? 0 : (int64_t)S->stack.K, nnzB, K, N};
     for (void* p : S->stack_bufs) dev_free(p);
     S->stack_bufs = fo.sc.bufs;
@@ -268,7 +276,8 @@ int brm_server_create(int64_t budget_bytes, int max_batch, int streams, brm_serv
         return BRM_E_NOMEM;
     }
     S->device = thread_device();
-    S->budget = budget_bytes;
+    S->ws_share = std::max<int64_t>(budget_bytes / 8 / streams, 1);
+    S->budget = budget_bytes - S->ws_share * streams;
     S->max_batch = max_batch;
     if (hipStreamCreateWithFlags(&S->aux, hipStreamNonBlocking) != hipSuccess) {
         delete S;
@@ -329,6 +338,7 @@ int brm_server_submit(brm_server* S, int b_id, const int* arpt, const int* acol,
     }
     BRMC(use_device());
     auto r = std::make_shared<Request>();
+    int B_N = 0;
     {
         std::lock_guard<std::mutex> ak(S->aux_mu);
         ResidentB B;
@@ -342,6 +352,7 @@ int brm_server_submit(brm_server* S, int b_id, const int* arpt, const int* acol,
             B = S->bs[b_id];
             brpt = S->stack.brpt + B.k0;
         }
+        B_N = B.N;
         // validate A against its B and count its products (the Upper bound)
         Scratch sc;
         int* derr;
@@ -362,6 +373,7 @@ int brm_server_submit(brm_server* S, int b_id, const int* arpt, const int* acol,
         BRMC(d2h(&r->ip, ipA + nnzA, sizeof(int64_t), S->aux));
     }
     r->b = b_id;
+    r->N = B_N;
     r->arpt = arpt;
     r->acol = acol;
     r->aval = aval;
@@ -413,7 +425,7 @@ int brm_server_wait(brm_server* S, int64_t req_id, brm_server_result* out) {
     out->t_done = r->t_done;
     out->batch_size = r->batch_size;
     out->M = r->M;
-    out->N = S->bs[r->b].N;
+    out->N = r->N;
     out->ip = r->ip;
     if (r->status != BRM_OK) {
         set_err("%s", r->err.c_str());

@@ -114,6 +114,12 @@ class DeviceArray:
             _lib.lib().brm_dev_free(self.ptr)
             self.ptr = None
 
+    def checksum(self, stream=None):
+        """Order-independent 64-bit checksum of the array's bytes (device)."""
+        out = ctypes.c_uint64()
+        _lib.check(_lib.lib().brm_checksum_dev(self.ptr, self.n * self.dtype.itemsize, ctypes.byref(out), stream))
+        return out.value
+
 
 class DeviceCSR:
     """CSR resident in HBM (rpt int32 for operands, int64 for results)."""
@@ -129,6 +135,18 @@ class DeviceCSR:
 
     def download(self):
         return CSR(self.rpt.to_host(), self.col.to_host(), self.val.to_host(), self.M, self.N)
+
+    def download_panel(self, rows):
+        """Host CSR of the selected rows (ascending row ids), as an
+        (rpt int64, col, val) triple."""
+        rows = np.asarray(rows, np.int64)
+        rpt = self.rpt.to_host()
+        b, e = rpt[rows], rpt[rows + 1]
+        cols = [self.col.to_host(int(x), int(y)) for x, y in zip(b, e)]
+        vals = [self.val.to_host(int(x), int(y)) for x, y in zip(b, e)]
+        prpt = np.concatenate([[0], np.cumsum(e - b)]).astype(np.int64)
+        return (prpt, np.concatenate(cols) if cols else np.zeros(0, np.int32),
+                np.concatenate(vals) if vals else np.zeros(0))
 
     def download_rows(self, rows):
         """Host copy of selected rows: list of (cols, vals)."""

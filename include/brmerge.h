@@ -156,11 +156,16 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
 
 /* Tuning knobs (names: "ws_budget_bytes", "med_cap", "leaf_cap",
  * "force_long", "profile", "lds_cfg", "group_cfg", "wmed_cap",
- * "wmed_engine", "launch_chunk"); returns BRM_E_INVALID for unknown names or
+ * "wmed_engine", "launch_chunk", "cbar_budget_bytes", "sym_chunk_products");
+ * returns BRM_E_INVALID for unknown names or
  * values.  Knobs never change results, only speed.  group_cfg 0-3:
  * merge-path engine shapes for long rows; 4 (default): the bitmap-rank
  * engine.  wmed_cap: products per row of the wavefront tier (0 off, up to
- * 1024; -1, the default: 1024 when B has at most 65,536 columns, else 512). */
+ * 1024; -1, the default: 1024 when B has at most 65,536 columns, else 512).
+ * cbar_budget_bytes: BRMerge-Upper's C-bar bytes per row wave (0, the
+ * default: a quarter of the memory the call can get, at most an eighth of
+ * the device).  sym_chunk_products: products per list chunk of the
+ * long-row symbolic count (1 to 2^30, the default). */
 int brm_set_param(const char *name, int64_t value);
 int64_t brm_get_param(const char *name);
 
@@ -192,8 +197,16 @@ typedef struct brm_stats {
     int64_t group_rounds;      /* LDS streaming rounds of those merges */
     int64_t group_in;          /* their input elements */
     int64_t group_out;         /* their output elements */
+    int64_t upper_waves;       /* BRMerge-Upper: row waves of C-bar (1 when C-bar fits at once) */
 } brm_stats;
 int brm_last_stats(brm_stats *out);
+
+/* Order-independent 64-bit checksum of `bytes` (a multiple of 4) of device
+ * memory: the sum mod 2^64 of a SplitMix64 mix of (word index, word).  No
+ * counterpart in the reference; it lets callers compare results of
+ * billions of entries (Precise against Upper, one run against another)
+ * without copying them to the host. */
+int brm_checksum_dev(const void *p, uint64_t bytes, uint64_t *out, void *stream);
 
 /*
  * Batched products.  The reference multiplies one pair per call
