@@ -31,7 +31,7 @@
 namespace brm {
 
 Params& params() {
-    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, -1, 0, 1, 0, (int64_t)1 << 30};
+    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, -1, 0, 1, 0, (int64_t)1 << 30, 0};
     return p;
 }
 
@@ -403,8 +403,8 @@ hipError_t bin_launch(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int fo
     p.wmed_cap = std::min(wmed_cap, WMED_CAP);
     hipError_t e = launch_fill_i32(p.bin_cnt, BIN_COUNT, 0, s);
     if (e != hipSuccess) return e;
-    return launch_rowstats(p.a, p.med_cap, shape.lmax, p.wmed_cap, force_long, p.iprow, p.rowcnt, p.bins,
-                           p.bin_cnt, p.derr, s);
+    return launch_rowstats(p.a, p.med_cap, shape.lmax, p.wmed_cap, force_long, params().hash_cap, p.iprow, p.rowcnt,
+                           p.bins, p.bin_cnt, p.derr, s);
 }
 }  // namespace
 
@@ -498,6 +498,7 @@ int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_shor
     HIPC(launch_wmed_count(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, p.rowcnt, s, std::min(p.wmed_cap, WMED_SPLIT)));
     HIPC(launch_wmed_count(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, p.rowcnt, s, p.wmed_cap));
     HIPC(launch_medium_count(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, p.rowcnt, s));
+    HIPC(launch_hash_count(p.rows(BIN_HASH), p.bc[BIN_HASH], p.a, p.rowcnt, s));
     HIPC(launch_long_count(p.rows(BIN_LONG), p.bc[BIN_LONG], p.a, p.rowcnt, s));
     if (clk) clk->begin(P_PNNZ);
     HIPC(scan_i64(p.rowcnt, p.a.M, p.crpt, s));
@@ -533,6 +534,7 @@ int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t
     HIPC(launch_wmed_numeric(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
     HIPC(launch_wmed_numeric(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, o, s, p.wmed_cap));
     HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
+    HIPC(launch_hash_numeric(p.rows(BIN_HASH), p.bc[BIN_HASH], p.a, o, s));
     return BRM_OK;
 }
 
@@ -589,7 +591,7 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
         int64_t acc = 0, base = 0;
         wave_ip_max = 0;
         for (int r = 0; r < M; ++r) {
-            if (acc > 0 && acc + iprow[r] > cap_ip) {
+            if (acc > 0 && iprow[r] > 0 && acc + iprow[r] > cap_ip) {  // empty rows never open a wave
                 wb.push_back(r);
                 base += acc;
                 wbase.push_back(base);
@@ -631,7 +633,8 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
             // bins of this wave's rows (row ids relative to its first row)
             clk.begin(P_LB);
             HIPC(launch_fill_i32(p.bin_cnt, BIN_COUNT, 0, s));
-            HIPC(launch_rowstats(aw, p.med_cap, shape.lmax, p.wmed_cap, params().force_long, p.iprow + wb[w],
+            HIPC(launch_rowstats(aw, p.med_cap, shape.lmax, p.wmed_cap, params().force_long, params().hash_cap,
+                                 p.iprow + wb[w],
                                  p.rowcnt + wb[w], p.bins, p.bin_cnt, p.derr, s));
             BRMC(d2h(bc, p.bin_cnt, sizeof(bc), s));
         }
@@ -651,6 +654,7 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
         HIPC(launch_wmed_numeric(rows(BIN_WMED2), bc[BIN_WMED2], aw, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
         HIPC(launch_wmed_numeric(rows(BIN_WMED3), bc[BIN_WMED3], aw, o, s, p.wmed_cap));
         HIPC(launch_medium_numeric(rows(BIN_MEDIUM), bc[BIN_MEDIUM], aw, o, s, p.lds_cfg));
+        HIPC(launch_hash_numeric(rows(BIN_HASH), bc[BIN_HASH], aw, o, s));
         BRMC(run_long(aw, rows(BIN_LONG), bc[BIN_LONG], o, nullptr, s, sc));
         BRMC(check_dev_err(p.derr, s, "numeric"));
         if (nwaves == 1) {
@@ -731,6 +735,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     for (int b = 0; b < 4; ++b) t_stats.rows_by_tier[b] = p.bc[b];
     t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_WMED] + p.bc[BIN_WMED2] + p.bc[BIN_WMED3];
     t_stats.rows_by_tier[BIN_SHORT] += p.bc[BIN_TINY];
+    t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_HASH];
     t_stats.total_ip = p.total_ip;
     Scratch& sc = p.sc;
     int* ccol = nullptr;
@@ -1136,6 +1141,7 @@ int brm_set_param(const char* name, int64_t value) {
     else if (!strcmp(name, "launch_chunk")) { if (value < 0) return BRM_E_INVALID; p.launch_chunk = value; }
     else if (!strcmp(name, "wmed_engine")) { if (value < 0 || value > 1) return BRM_E_INVALID; p.wmed_engine = (int)value; }
     else if (!strcmp(name, "cbar_budget_bytes")) { if (value < 0) return BRM_E_INVALID; p.cbar_budget_bytes = value; }
+    else if (!strcmp(name, "hash_cap")) { if (value < 0 || value > HASH_CAP) return BRM_E_INVALID; p.hash_cap = (int)value; }
     else if (!strcmp(name, "sym_chunk_products")) { if (value < 1 || value > ((int64_t)1 << 30)) return BRM_E_INVALID; p.sym_chunk_products = value; }
     else return BRM_E_INVALID;
     return BRM_OK;
@@ -1156,6 +1162,7 @@ int64_t brm_get_param(const char* name) {
     if (!strcmp(name, "wmed_engine")) return p.wmed_engine;
     if (!strcmp(name, "cbar_budget_bytes")) return p.cbar_budget_bytes;
     if (!strcmp(name, "sym_chunk_products")) return p.sym_chunk_products;
+    if (!strcmp(name, "hash_cap")) return p.hash_cap;
     return -1;
 }
 

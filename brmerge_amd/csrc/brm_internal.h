@@ -21,10 +21,13 @@ enum RowBin : int {
     BIN_TINY = 5,    // ip <= TINY_MAX: two rows per wavefront, merge in registers
     BIN_WMED2 = 6,   // WMED_SPLIT0 < ip <= min(wmed_cap, WMED_SPLIT), L <= WMED_LMAX
     BIN_WMED3 = 7,   // WMED_SPLIT < ip <= wmed_cap, L <= WMED_LMAX
-    BIN_COUNT = 8
+    BIN_HASH = 8,    // ip <= hash_cap (> 0) and L <= HASH_CAP: Hash-SpGEMM comparator tier (hash.hip)
+    BIN_COUNT = 9
 };
 
 constexpr int SHORT_MAX = 64;
+// Largest product count of a row in the Hash-SpGEMM tier.
+constexpr int HASH_CAP = 1024;
 constexpr int TINY_MAX = 32;
 
 // Wavefront-sized LDS merges for small medium rows (several rows resident
@@ -171,6 +174,7 @@ struct Params {
     int wmed_engine;           // wavefront tier numeric: 0 LDS merge levels, 1 register bitonic sort
     int64_t cbar_budget_bytes; // Upper: C-bar bytes per row wave (0 = auto)
     int64_t sym_chunk_products; // long-row symbolic: products per list chunk (1 .. 2^30, default 2^30)
+    int hash_cap;              // rows of at most hash_cap products take the hash tier (0 = off, <= HASH_CAP)
 };
 
 // Shapes of the LDS tree-merge kernels, selectable at run time.

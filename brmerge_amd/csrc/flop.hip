@@ -66,7 +66,7 @@ __global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __rest
 constexpr int RS_IT = 4;
 
 __global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int med_lmax, int wmed_cap, int force_long,
-                                                 int64_t* __restrict__ iprow,
+                                                 int hash_cap, int64_t* __restrict__ iprow,
                                                  int64_t* __restrict__ rowcnt, int* __restrict__ bins,
                                                  int* __restrict__ bin_cnt, const int* __restrict__ err) {
     __shared__ int lcnt[BIN_COUNT];
@@ -96,6 +96,8 @@ __global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int med
                 rowcnt[i] = 0;
             } else if (force_long) {
                 b_ = BIN_LONG;
+            } else if (ip <= hash_cap && L <= HASH_CAP) {
+                b_ = BIN_HASH;
             } else if (ip <= TINY_MAX) {
                 b_ = BIN_TINY;
             } else if (ip <= SHORT_MAX) {
@@ -168,13 +170,14 @@ hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* er
     return hipGetLastError();
 }
 
-hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int wmed_cap, int force_long, int64_t* iprow,
-                           int64_t* rowcnt, int* bins, int* bin_cnt, const int* err, hipStream_t s) {
+hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int wmed_cap, int force_long, int hash_cap,
+                           int64_t* iprow, int64_t* rowcnt, int* bins, int* bin_cnt, const int* err, hipStream_t s) {
     if (a.M <= 0) return hipSuccess;
     {
         ProfScope ps_("k_rowstats", s);
-        k_rowstats<<<nblk(a.M, NT * RS_IT), NT, 0, s>>>(a, med_cap, med_lmax, wmed_cap, force_long, iprow, rowcnt,
-                                                        bins, bin_cnt, err);
+        k_rowstats<<<nblk(a.M, NT * RS_IT), NT, 0, s>>>(a, med_cap, med_lmax, wmed_cap, force_long,
+                                                        std::min(hash_cap, HASH_CAP), iprow, rowcnt, bins, bin_cnt,
+                                                        err);
     }
     return hipGetLastError();
 }

@@ -33,8 +33,8 @@ hipError_t launch_validate(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* er
 hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* err, hipStream_t s);
 // bins: [BIN_COUNT-1][M] row lists (bin b at (b-1)*M); bin_cnt: [BIN_COUNT]
 // Rows are all treated as empty when *err is set (err may be null).
-hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int wmed_cap, int force_long, int64_t* iprow,
-                           int64_t* nnzrow, int* bins, int* bin_cnt, const int* err, hipStream_t s);
+hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int wmed_cap, int force_long, int hash_cap,
+                           int64_t* iprow, int64_t* nnzrow, int* bins, int* bin_cnt, const int* err, hipStream_t s);
 
 // short.hip
 // group: lanes per row, 64 (ip <= 64) or 32 (ip <= 32, two rows per wavefront)
@@ -70,6 +70,11 @@ hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const O
 // Wavefront-tier capacity chosen when wmed_cap is -1 (register engine): 1024
 // products while 32-bit sort keys hold N columns, else 512.
 int rmed_auto_cap(int N);
+
+// hash.hip (rows of at most HASH_CAP products)
+hipError_t launch_hash_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s);
+// lds.hip: exact row counts of hash-tier rows
+hipError_t launch_hash_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
 
 // bmerge.hip
 hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,

@@ -413,6 +413,19 @@ hipError_t launch_wmed_numeric(const int* rows, int n, const RowArgs& a, const O
     return medium_numeric_t<BRM_WAVE, WMED_CAP, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");
 }
 
+hipError_t launch_hash_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    {
+        ProfScope ps_("k_sym_hash_h", s);
+        const int64_t ch = launch_chunk(BRM_WAVE);
+        for (int64_t b = 0; b < n; b += ch) {
+            const int c = (int)std::min<int64_t>(ch, n - b);
+            k_sym_hash<BRM_WAVE, 2 * HASH_CAP, HASH_CAP><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+        }
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_long_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     static SmemLimit lim;

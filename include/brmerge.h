@@ -156,16 +156,22 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
 
 /* Tuning knobs (names: "ws_budget_bytes", "med_cap", "leaf_cap",
  * "force_long", "profile", "lds_cfg", "group_cfg", "wmed_cap",
- * "wmed_engine", "launch_chunk", "cbar_budget_bytes", "sym_chunk_products");
+ * "wmed_engine", "launch_chunk", "cbar_budget_bytes", "sym_chunk_products",
+ * "hash_cap");
  * returns BRM_E_INVALID for unknown names or
- * values.  Knobs never change results, only speed.  group_cfg 0-3:
+ * values.  Knobs other than hash_cap never change results, only speed.  group_cfg 0-3:
  * merge-path engine shapes for long rows; 4 (default): the bitmap-rank
  * engine.  wmed_cap: products per row of the wavefront tier (0 off, up to
  * 1024; -1, the default: 1024 when B has at most 65,536 columns, else 512).
  * cbar_budget_bytes: BRMerge-Upper's C-bar bytes per row wave (0, the
  * default: a quarter of the memory the call can get, at most an eighth of
  * the device).  sym_chunk_products: products per list chunk of the
- * long-row symbolic count (1 to 2^30, the default). */
+ * long-row symbolic count (1 to 2^30, the default).  hash_cap (0, the
+ * default, to 1024): rows of at most hash_cap products take the Hash-SpGEMM
+ * tier instead of BRMerge's (the comparison baseline of the reference's
+ * Existing-SpGEMM/inc/hash_mult.h:558-608: each column's contributions added
+ * in list order); this knob, alone among them, changes value bits (within
+ * 1e-12 of BRMerge's), never the structure. */
 int brm_set_param(const char *name, int64_t value);
 int64_t brm_get_param(const char *name);
 

@@ -6,5 +6,5 @@ import this package, and only as the checker.  The product library
 """
 from .oracle import (  # noqa: F401
     OracleCSR, load, compute_flop, upper_log, prefix_sum, binary_search_approx,
-    brmerge_precise, brmerge_upper, max_threads,
+    brmerge_precise, brmerge_upper, hash_spgemm, max_threads,
 )

@@ -449,3 +449,123 @@ int brmo_brmerge_upper(const int *arpt, const int *acol, const double *aval,
     if (tm) { tm[0] = tcomp; tm[1] = now_s() - t0; }
     return 0;
 }
+
+/*
+ * Existing-SpGEMM/inc/hash_mult.h:558-608 (hash_numeric, sortOutput) and
+ * :519-552 (sort_and_store_table2mat): Hash SpGEMM's numeric phase, the
+ * paper's comparison baseline.  Per row, a linear-probing table keyed by
+ * column, hash (key * 107) & (size - 1) (HASH_SCAL, :23), of
+ * MIN_HT_N << (bin_id - 1) slots (BIN.h:157-177, MIN_HT_N 8 outside KNL
+ * builds, :34-35; bin_id from the row's products, capped at the column
+ * count); products visited in A-entry order, then B-entry order, each
+ * rounded (multiplies) and added onto its column's value as
+ * t_val + ht_value (:588); the occupied slots sorted by column.  The value
+ * of each column is therefore the sequential left fold of its contributions
+ * in list order.  The table's size and layout do not change any value.
+ *
+ * Rows with at most `cap` products and at most `lmax` A entries take this
+ * order; the others take BRMerge-Precise's tree (merge_row), as the
+ * library's hash tier does.  cap < 0: every row; lmax < 0: no list limit.  Output as brmo_brmerge_precise (int64 row pointer).
+ */
+typedef struct { int c; double v; } brmo_cv;
+static int brmo_cv_less(const void *x, const void *y) {
+    const int a = ((const brmo_cv *)x)->c, b = ((const brmo_cv *)y)->c;
+    return (a > b) - (a < b);
+}
+
+int brmo_hash_numeric(const int *arpt, const int *acol, const double *aval,
+                      const int *brpt, const int *bcol, const double *bval,
+                      int M, int K, int N, int64_t cap, int lmax,
+                      int64_t **crpt_, int **ccol_, double **cval_, int64_t *cnnz_, int nthreads) {
+    (void)K;
+    int *ccol = NULL;
+    double *cval = NULL;
+    int64_t *crpt = NULL;
+    /* the structure (and the values of rows above cap) from Precise */
+    int64_t *pcrpt = NULL;
+    int *pccol = NULL;
+    double *pcval = NULL;
+    int64_t pnnz = 0;
+    if (brmo_brmerge_precise(arpt, acol, aval, brpt, bcol, bval, M, K, N, &pcrpt, &pccol, &pcval, &pnnz, nthreads,
+                             NULL) != 0)
+        return -1;
+    crpt = pcrpt;
+    ccol = pccol;
+    cval = pcval;
+    int fail = 0;
+    const int T = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(T)
+    {
+        int hcap = 0;
+        int *htk = NULL;
+        double *htv = NULL;
+        brmo_cv *pv = NULL;
+        int pvcap = 0;
+#pragma omp for schedule(dynamic, 64)
+        for (int row = 0; row < M; row++) {
+            int64_t ip = 0;
+            for (int j = arpt[row]; j < arpt[row + 1]; j++) ip += brpt[acol[j] + 1] - brpt[acol[j]];
+            const int L = arpt[row + 1] - arpt[row];
+            if (ip == 0 || (cap >= 0 && ip > cap) || (lmax >= 0 && L > lmax)) continue;
+            int64_t nz = ip < N ? ip : N; /* BIN.h:164 */
+            int bid = 0;
+            while (nz > ((int64_t)8 << bid)) bid++;
+            const int hs = 8 << bid; /* MIN_HT_N << (bin_id - 1), bin_id = bid + 1 */
+            if (hs > hcap) {
+                free(htk); free(htv);
+                htk = (int *)malloc(sizeof(int) * (size_t)hs);
+                htv = (double *)malloc(sizeof(double) * (size_t)hs);
+                hcap = hs;
+                if (!htk || !htv) {
+#pragma omp atomic write
+                    fail = 1;
+                    hcap = 0;
+                    continue;
+                }
+            }
+            for (int s = 0; s < hs; s++) htk[s] = -1;
+            for (int j = arpt[row]; j < arpt[row + 1]; j++) {
+                const double av = aval[j];
+                for (int k = brpt[acol[j]]; k < brpt[acol[j] + 1]; k++) {
+                    const double t = av * bval[k];
+                    const int key = bcol[k];
+                    unsigned h = ((unsigned)key * BRMO_HASH_SCALE) & (unsigned)(hs - 1);
+                    for (;;) {
+                        if (htk[h] == key) { htv[h] = t + htv[h]; break; }
+                        if (htk[h] == -1) { htk[h] = key; htv[h] = t; break; }
+                        h = (h + 1) & (unsigned)(hs - 1);
+                    }
+                }
+            }
+            const int64_t n = crpt[row + 1] - crpt[row];
+            if (n > pvcap) {
+                free(pv);
+                pv = (brmo_cv *)malloc(sizeof(brmo_cv) * (size_t)n);
+                pvcap = (int)n;
+                if (!pv) {
+#pragma omp atomic write
+                    fail = 1;
+                    pvcap = 0;
+                    continue;
+                }
+            }
+            int64_t m = 0;
+            for (int s = 0; s < hs; s++)
+                if (htk[s] != -1 && m < n) { pv[m].c = htk[s]; pv[m].v = htv[s]; m++; }
+            if (m != n) {
+#pragma omp atomic write
+                fail = 2;
+                continue;
+            }
+            qsort(pv, (size_t)m, sizeof(brmo_cv), brmo_cv_less);
+            for (int64_t q = 0; q < m; q++) {
+                ccol[crpt[row] + q] = pv[q].c;
+                cval[crpt[row] + q] = pv[q].v;
+            }
+        }
+        free(htk); free(htv); free(pv);
+    }
+    if (fail) { free(crpt); free(ccol); free(cval); return -1; }
+    *crpt_ = crpt; *ccol_ = ccol; *cval_ = cval; *cnnz_ = pnnz;
+    return 0;
+}

@@ -41,6 +41,11 @@ def load():
                       ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int64),
                       ctypes.c_int, _f64p]
         f.restype = ctypes.c_int
+    lib.brmo_hash_numeric.argtypes = [_i32p, _i32p, _f64p, _i32p, _i32p, _f64p, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                      ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_int]
+    lib.brmo_hash_numeric.restype = ctypes.c_int
     _LIB = lib
     return lib
 
@@ -127,6 +132,41 @@ def brmerge_precise(A, B, nthreads=0, timing=False):
     (rpt, col, val[, ncols]).  Returns OracleCSR (and phase times)."""
     C, tm = _run("brmo_brmerge_precise", A, B, nthreads)
     return (C, tm) if timing else C
+
+
+def hash_spgemm(A, B, cap=-1, lmax=-1, nthreads=0):
+    """C = A*B with Hash SpGEMM's accumulation order (Existing-SpGEMM/inc/
+    hash_mult.h:558-608: each column's contributions added in list order,
+    sorted output) for rows with at most `cap` products and `lmax` A entries
+    (no limit when negative); the other rows by BRMerge-Precise.  Returns
+    OracleCSR."""
+    lib = load()
+    arpt, acol, aval = (np.ascontiguousarray(A[0], np.int32), np.ascontiguousarray(A[1], np.int32),
+                        np.ascontiguousarray(A[2], np.float64))
+    brpt, bcol, bval = (np.ascontiguousarray(B[0], np.int32), np.ascontiguousarray(B[1], np.int32),
+                        np.ascontiguousarray(B[2], np.float64))
+    M, K = len(arpt) - 1, len(brpt) - 1
+    N = int(B[3]) if len(B) > 3 else (int(bcol.max()) + 1 if len(bcol) else 0)
+    if len(acol) == 0:
+        acol, aval = np.zeros(1, np.int32), np.zeros(1)
+    if len(bcol) == 0:
+        bcol, bval = np.zeros(1, np.int32), np.zeros(1)
+    pr, pc, pv = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    nnz = ctypes.c_int64()
+    rc = lib.brmo_hash_numeric(arpt, acol, aval, brpt, bcol, bval, M, K, N, int(cap), int(lmax), ctypes.byref(pr),
+                               ctypes.byref(pc), ctypes.byref(pv), ctypes.byref(nnz), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"brmo_hash_numeric failed with {rc}")
+    n = nnz.value
+    rpt = np.ctypeslib.as_array(ctypes.cast(pr, ctypes.POINTER(ctypes.c_int64)), (M + 1,)).copy()
+    if n > 0:
+        col = np.ctypeslib.as_array(ctypes.cast(pc, ctypes.POINTER(ctypes.c_int32)), (n,)).copy()
+        val = np.ctypeslib.as_array(ctypes.cast(pv, ctypes.POINTER(ctypes.c_double)), (n,)).copy()
+    else:
+        col, val = np.zeros(0, np.int32), np.zeros(0)
+    for p in (pr, pc, pv):
+        lib.brmo_free(p)
+    return OracleCSR(M, N, rpt, col, val)
 
 
 def brmerge_upper(A, B, nthreads=0, timing=False):

@@ -129,7 +129,7 @@ def test_rmat15_many_waves(knobs, engine):
     st = gpu.last_stats()
     assert st["upper_waves"] >= 3 and st["long_waves"] >= 3
     assert_csr_equal(Cu, R)
-    # a C-bar budget below one row still makes progress: one row per wave
+    # a C-bar budget below one row still makes progress: one non-empty row per wave
     gpu.set_param("cbar_budget_bytes", 12)
     gpu.set_param("ws_budget_bytes", 0)
     A2 = gpu.gen.rmat(9, seed=4)

@@ -255,3 +255,67 @@ def test_precise_equals_upper_bitwise_on_skewed_input():
     C1 = oracle.brmerge_precise(A, A, nthreads=5)
     C2 = oracle.brmerge_upper(A, A, nthreads=3)
     assert_same(C1, C2.rpt, C2.col, C2.val)
+
+
+# ---- Hash SpGEMM's accumulation order (Existing-SpGEMM/inc/hash_mult.h:558-608)
+def _one_column(vals):
+    """A 1 x L times L x 1 product whose single output sums `vals` over the
+    lists in order (A all ones, B row p holds vals[p] in column 0)."""
+    L = len(vals)
+    A = (np.array([0, L], np.int32), np.arange(L, dtype=np.int32), np.ones(L), L)
+    B = (np.arange(L + 1, dtype=np.int32), np.zeros(L, np.int32), np.array(vals, np.float64), 1)
+    return A, B
+
+
+def test_hash_order_known_answers():
+    """The hash order adds each contribution onto the running value in list
+    order: ((v0 + v1) + v2) + v3.  With b = 2^53 (where the spacing of
+    doubles becomes 2, so b + 1 rounds to b, ties to even, while 1 - b is
+    exact):
+      v = (b, 1, 1, -b): left fold ((b + 1) + 1) - b = (b + 1) - b = 0;
+                         tree (b + 1) + (1 - b) = b - (b - 1) = 1;
+                         right fold b + (1 + (1 - b)) = b + (2 - b) = 2.
+      v = (1, 1e16, -1e16, 1) (spacing 2 at 1e16, so 1e16 +- 1 rounds back):
+                         left fold ((1 + 1e16) - 1e16) + 1 = 1; tree
+                         (1 + 1e16) + (1 - 1e16) = 1e16 - 1e16 = 0."""
+    b = 2.0 ** 53
+    A, B = _one_column([b, 1.0, 1.0, -b])
+    assert oracle.hash_spgemm(A, B).val.tolist() == [0.0]
+    assert oracle.brmerge_precise(A, B).val.tolist() == [1.0]
+    A, B = _one_column([1.0, 1e16, -1e16, 1.0])
+    assert oracle.hash_spgemm(A, B).val.tolist() == [1.0]
+    assert oracle.brmerge_precise(A, B).val.tolist() == [0.0]
+    # cap: rows above it keep BRMerge's tree
+    assert oracle.hash_spgemm(A, B, cap=3).val.tolist() == [0.0]
+    assert oracle.hash_spgemm(A, B, cap=4).val.tolist() == [1.0]
+    assert oracle.hash_spgemm(A, B, cap=4, lmax=3).val.tolist() == [0.0]  # more lists than lmax: the tree
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_hash_order_structure_and_two_contributions(seed):
+    """Same structure as Precise; values within 1e-12 of scipy; and bit-equal
+    to Precise wherever a column has at most two contributions (one add: no
+    order to choose)."""
+    rng = np.random.default_rng(500 + seed)
+    from tests._util import rand_csr_tuple
+    M, K, N = [int(x) for x in rng.integers(1, 200, 3)]
+    A = rand_csr_tuple(rng, M, K, int(rng.integers(1, 40)), empty_frac=0.2, adversarial=False)
+    B = rand_csr_tuple(rng, K, N, int(rng.integers(1, 30)), empty_frac=0.2, adversarial=False)
+    H = oracle.hash_spgemm(A, B, nthreads=2)
+    P = oracle.brmerge_precise(A, B, nthreads=2)
+    np.testing.assert_array_equal(H.rpt, P.rpt)
+    np.testing.assert_array_equal(H.col, P.col)
+    Asp = sp.csr_matrix((A[2], A[1], A[0]), shape=(M, K))
+    Bsp = sp.csr_matrix((B[2], B[1], B[0]), shape=(K, N))
+    ones = sp.csr_matrix((np.ones(len(A[1])), A[1], A[0]), shape=(M, K)) @ \
+        sp.csr_matrix((np.ones(len(B[1])), B[1], B[0]), shape=(K, N))
+    ones.sort_indices()
+    np.testing.assert_array_equal(ones.indices, H.col)
+    ref = (Asp @ Bsp).toarray()
+    dense = np.zeros((M, N))
+    for i in range(M):
+        dense[i, H.col[H.rpt[i]:H.rpt[i + 1]]] = H.val[H.rpt[i]:H.rpt[i + 1]]
+    scale = max(np.abs(ref).max(), 1e-300)
+    assert np.abs(dense - ref).max() / scale <= 1e-12
+    few = np.asarray(ones.data) <= 2  # contributions per output entry
+    assert np.array_equal(bits(H.val[few]), bits(P.val[few]))
