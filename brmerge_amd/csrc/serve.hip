@@ -12,9 +12,10 @@
 // intermediate product plus its row pointer, 12*ip + 8*(M+1), against the
 // server budget; the reservation shrinks to the result's size when the
 // batch finishes and is returned when the caller releases the result.
-// Each worker's long-row workspace comes out of the same budget: an eighth
-// of it is split evenly among the workers (their per-call workspace budget),
-// and requests are admitted against the remaining seven eighths.
+// Each worker's long-row workspace is bounded too: an eighth of the budget,
+// split evenly among the workers, is their per-call workspace budget (on top
+// of the admitted results), so concurrent batches never size workspace by
+// reading the device's free memory at the same time.
 // Worker threads (one HIP stream each) take every admissible queued request
 // (FIFO, up to max_batch) whenever they are free, so batches formed while
 // others run overlap on the device.
@@ -81,7 +82,7 @@ using namespace brm;
 
 struct brm_server {
     int device = 0;
-    int64_t budget = 0;     // admission budget for results
+    int64_t budget = 0;     // admission budget (results)
     int64_t ws_share = 0;   // long-row workspace budget of each worker
     int max_batch = 64;
     std::vector<ResidentB> bs;
@@ -277,7 +278,7 @@ int brm_server_create(int64_t budget_bytes, int max_batch, int streams, brm_serv
     }
     S->device = thread_device();
     S->ws_share = std::max<int64_t>(budget_bytes / 8 / streams, 1);
-    S->budget = budget_bytes - S->ws_share * streams;
+    S->budget = budget_bytes;
     S->max_batch = max_batch;
     if (hipStreamCreateWithFlags(&S->aux, hipStreamNonBlocking) != hipSuccess) {
         delete S;

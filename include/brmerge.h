@@ -274,8 +274,9 @@ void brm_batch_plan_destroy(brm_batch_plan *plan);
  * freed afterwards); it waits until no request is queued or running.  A of
  * a submitted request is read when its batch runs: keep it alive until the
  * request is done.  Results live in HBM until brm_server_release; their
- * bytes count against the budget until then.  Times are seconds on the
- * host's steady clock.
+ * bytes count against the budget until then.  Each worker's long-row
+ * workspace is bounded by an eighth of the budget divided among the workers.
+ * Times are seconds on the host's steady clock.
  */
 typedef struct brm_server brm_server;
 typedef struct brm_server_result {
