@@ -68,7 +68,7 @@ EXPORTED = [
     "brm_flop_count", "brm_flop_count_dev",
     "brm_set_param", "brm_get_param", "brm_debug_fail_alloc_after",
     "brm_memcpy_h2d", "brm_memcpy_d2h", "brm_prof_reset", "brm_prof_report", "brm_last_stats", "brm_bench_copy",
-    "brm_checksum_dev",
+    "brm_checksum_dev", "brm_spgemm_precise_dev_noalloc", "brm_noalloc_scratch_bytes",
     "brm_spgemm_batched_dev", "brm_batch_plan_create", "brm_batch_plan_execute", "brm_batch_plan_result",
     "brm_batch_plan_info", "brm_batch_plan_destroy",
     "brm_server_create", "brm_server_add_b", "brm_server_submit", "brm_server_poll", "brm_server_wait",
@@ -124,6 +124,9 @@ def lib():
         "brm_last_stats": ([P], I),
         "brm_bench_copy": ([P, P, U64, P], I),
         "brm_checksum_dev": ([P, U64, ctypes.POINTER(U64), P], I),
+        "brm_spgemm_precise_dev_noalloc": ([P, P, P, I64, P, P, P, I64, I, I, I, P, P, P, I64, ctypes.POINTER(I64),
+                                            P], I),
+        "brm_noalloc_scratch_bytes": ([], U64),
         "brm_spgemm_batched_dev": ([P, I, PP, PP, PP, ctypes.POINTER(I64), P, P, P], I),
         "brm_batch_plan_create": ([P, I, I, PP, P], I),
         "brm_batch_plan_execute": ([P, P], I),

@@ -39,7 +39,31 @@ void release_cached_locked() {
 
 }  // namespace
 
+namespace {
+// Arena of the calling thread (brm_spgemm_precise_dev_noalloc): while set,
+// every allocation of the thread's library calls is carved from it, and
+// nothing is released until the arena is cleared.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+};
+thread_local Arena t_arena;
+}  // namespace
+
+void arena_begin(void* base, size_t bytes) { t_arena = Arena{(char*)base, bytes, 0}; }
+void arena_end() { t_arena = Arena{}; }
+int64_t arena_remaining() { return t_arena.base ? (int64_t)(t_arena.size - t_arena.used) : -1; }
+size_t arena_used() { return t_arena.used; }
+
 void* dev_alloc(size_t bytes) {
+    if (t_arena.base) {
+        const size_t off = (t_arena.used + 255) & ~size_t(255);
+        const size_t sz = bytes > 0 ? bytes : 1;
+        if (off > t_arena.size || sz > t_arena.size - off) return nullptr;
+        t_arena.used = off + sz;
+        return t_arena.base + off;
+    }
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_fail_after >= 0) {
         if (g_fail_after == 0) { g_fail_after = -1; return nullptr; }
@@ -71,6 +95,7 @@ void* dev_alloc(size_t bytes) {
 
 void dev_free(void* p) {
     if (!p) return;
+    if (t_arena.base && (char*)p >= t_arena.base && (char*)p < t_arena.base + t_arena.size) return;
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_used.find(p);
     if (it == g_used.end()) return;  // not ours
@@ -130,7 +155,7 @@ void prof_drain_locked() {
 }  // namespace
 
 void prof_begin(const char* name, hipStream_t s) {
-    if (!params().profile) return;
+    if (!params().profile || t_arena.base) return;  // no events inside the no-allocation entry
     hipEvent_t a;
     if (hipEventCreate(&a) != hipSuccess) return;
     (void)hipEventRecord(a, s);

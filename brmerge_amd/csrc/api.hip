@@ -214,7 +214,16 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     // calls of one product plan identical waves.
     int64_t budget = params().ws_budget_bytes;
     if (budget <= 0) budget = thread_ws_budget();
-    if (budget <= 0) {
+    const int64_t arena = arena_remaining();
+    if (arena >= 0) {
+        // the arena bounds everything this function allocates: per long row,
+        // 20 bytes of wave tables beside row_bytes(), and alignment slack
+        budget = arena - (int64_t)20 * nlong - (int64_t)64 * 1024;
+        if (budget < 0) {
+            set_err("long rows need more scratch memory than the no-allocation entry's fixed buffer holds");
+            return BRM_E_NOMEM;
+        }
+    } else if (budget <= 0) {
         size_t fr = 0, tot = 0;
         HIPC(hipMemGetInfo(&fr, &tot));
         budget = std::min((int64_t)(fr + dev_cached_bytes()) / 10 * 7, (int64_t)tot / 2);
@@ -240,6 +249,10 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     std::vector<Wave> waves;
     int64_t mx_ws = 0, mx_sl = 0, mx_leaf = 0, mx_child = 0, mx_task = 0;
     int mx_rows = 0;
+  for (;;) {  // (the arena case re-plans with a smaller budget until the buffers fit)
+    waves.clear();
+    mx_ws = mx_sl = mx_leaf = mx_child = mx_task = 0;
+    mx_rows = 0;
     for (int k0 = 0; k0 < nlong;) {
         int k1 = k0;
         int64_t used = 0;
@@ -278,6 +291,18 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         waves.push_back(std::move(w));
         k0 = k1;
     }
+    if (arena < 0) break;
+    // bytes of the buffers allocated below (each sized for its largest wave)
+    const int64_t need = 24 * mx_ws + 8 * mx_sl + (int64_t)sizeof(LeafTask) * mx_leaf +
+                         (int64_t)sizeof(Child) * mx_child + (int64_t)sizeof(GroupTask) * mx_task +
+                         20 * (int64_t)mx_rows + 8 * (CNT_N + MAX_PASSES + 2) + 16 * 256;
+    if (need <= arena) break;
+    if ((int)waves.size() == nlong) {
+        set_err("a long row needs more scratch memory than the no-allocation entry's fixed buffer holds");
+        return BRM_E_NOMEM;
+    }
+    budget = budget / 4 * 3;
+  }
 
     // Buffers are rounded up to one eighth of their next power of two (at
     // most 12.5% more than needed, with a small floor), so that products of
@@ -289,12 +314,14 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         const int64_t g = p2 / 8;
         return (n + g - 1) / g * g;
     };
-    mx_ws = round_geo(mx_ws, (int64_t)1 << 20);
-    mx_sl = round_geo(mx_sl, (int64_t)1 << 16);
-    mx_leaf = round_geo(mx_leaf, (int64_t)1 << 14);
-    mx_child = round_geo(mx_child, (int64_t)1 << 14);
-    mx_task = round_geo(mx_task, (int64_t)1 << 14);
-    mx_rows = (int)round_geo(mx_rows, (int64_t)1 << 10);
+    if (arena < 0) {
+        mx_ws = round_geo(mx_ws, (int64_t)1 << 20);
+        mx_sl = round_geo(mx_sl, (int64_t)1 << 16);
+        mx_leaf = round_geo(mx_leaf, (int64_t)1 << 14);
+        mx_child = round_geo(mx_child, (int64_t)1 << 14);
+        mx_task = round_geo(mx_task, (int64_t)1 << 14);
+        mx_rows = (int)round_geo(mx_rows, (int64_t)1 << 10);
+    }
     Scratch wsc;
     int *rows_w, *wcol0, *wcol1;
     double *wval0, *wval1;
@@ -425,7 +452,8 @@ int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p)
     BRMC(sc.alloc(&p.ipA, in.nnzA + 1, "ipA"));
     BRMC(sc.alloc(&p.iprow, M, "row ip"));
     BRMC(sc.alloc(&p.rowcnt, M, "row nnz"));
-    BRMC(sc.alloc(&p.crpt, (int64_t)M + 1, "crpt"));
+    if (p.crpt_ext) p.crpt = p.crpt_ext;
+    else BRMC(sc.alloc(&p.crpt, (int64_t)M + 1, "crpt"));
     BRMC(sc.alloc(&p.bins, (BIN_COUNT - 1) * (int64_t)M, "row bins"));
     HIPC(launch_fill_i32(p.derr, 1, 0, s));
     p.nnzA = in.nnzA;
@@ -724,12 +752,18 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
 }
 }  // namespace
 
-int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph) {
+int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph,
+               const ExtOut* ext) {
     PhaseClock clk(s, ph != nullptr);
     memset(&t_stats, 0, sizeof(t_stats));
     HostTrace tr(s);
     tr.mark("spgemm start", in.M);
     Pattern p;
+    if (ext && upper) {
+        set_err("caller-owned outputs are for BRMerge-Precise");
+        return BRM_E_INVALID;
+    }
+    if (ext) p.crpt_ext = ext->crpt;
     BRMC(pattern_prepare(in, s, &clk, p));
     const int M = in.M;
     for (int b = 0; b < 4; ++b) t_stats.rows_by_tier[b] = p.bc[b];
@@ -746,9 +780,12 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         // Small products size C by the product count (an upper bound of
         // nnz(C), at most SMALL_IP elements) and read nnz(C) with the
         // numeric phase's error flag: one host round trip less.
+        // Caller-owned C: the short-row stash is not used (it would take
+        // scratch memory), and the product count bounds nnz(C) only when the
+        // caller's capacity covers it.
         constexpr int64_t SMALL_IP = (int64_t)1 << 24;
-        const bool small = p.total_ip <= SMALL_IP && p.total_ip <= max_nnz;
-        BRMC(pattern_symbolic(p, s, &clk, true, !small));
+        const bool small = p.total_ip <= SMALL_IP && p.total_ip <= max_nnz && (!ext || p.total_ip <= ext->cap);
+        BRMC(pattern_symbolic(p, s, &clk, ext == nullptr, !small));
         nnzC = p.nnzC;
         if (!small && nnzC > max_nnz) {
             set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
@@ -756,8 +793,19 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         }
         const int64_t cap = small ? p.total_ip : nnzC;
         clk.begin(P_ALLOC);
-        BRMC(sc.alloc(&ccol, cap, "C col"));
-        BRMC(sc.alloc(&cval, cap, "C val"));
+        if (ext) {
+            if (!small && nnzC > ext->cap) {
+                if (ext->nnz_needed) *ext->nnz_needed = nnzC;
+                set_err("nnz(C) = %lld exceeds the caller's capacity of %lld entries", (long long)nnzC,
+                        (long long)ext->cap);
+                return BRM_E_OVERFLOW;
+            }
+            ccol = ext->ccol;
+            cval = ext->cval;
+        } else {
+            BRMC(sc.alloc(&ccol, cap, "C col"));
+            BRMC(sc.alloc(&cval, cap, "C val"));
+        }
         tr.mark("C allocated", cap);
         clk.begin(P_COMPUTE);
         BRMC(pattern_numeric_tiers(p, ccol, cval, s));
@@ -972,7 +1020,8 @@ int brm_init(int device) {
         return BRM_E_DEVICE;
     }
     t_device = device;
-    return hip_status(hipSetDevice(device), "hipSetDevice");
+    BRMC(hip_status(hipSetDevice(device), "hipSetDevice"));
+    return noalloc_setup(device);  // the fixed scratch of brm_spgemm_precise_dev_noalloc on this device
 }
 
 const char* brm_last_error(void) { return t_err.c_str(); }

@@ -159,6 +159,13 @@ void dev_flush_deferred();
 void dev_trim();                 // release cached blocks to the driver
 size_t dev_cached_bytes();
 void dev_set_fail_after(int64_t n);  // test hook: fail the n-th next allocation (-1 = off)
+// While an arena is set, the calling thread's allocations are carved from
+// [base, base + bytes) (256-byte aligned, never released, nullptr when it is
+// exhausted) instead of the caching allocator.
+void arena_begin(void* base, size_t bytes);
+void arena_end();
+int64_t arena_remaining();  // -1: no arena
+size_t arena_used();
 
 // ------------------------------------------------------------------ tunables
 struct Params {

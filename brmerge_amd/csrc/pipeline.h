@@ -115,6 +115,7 @@ struct Pattern {
     int64_t* rowcnt = nullptr;
     int64_t* crpt = nullptr;
     int* bins = nullptr;  // [BIN_COUNT-1][M]: row lists of the non-empty bins
+    int64_t* crpt_ext = nullptr;  // caller's row pointer of C (set before pattern_prepare), else allocated
     int bc[BIN_COUNT] = {};
     int64_t total_ip = 0;
     int64_t nnzC = 0;
@@ -151,9 +152,21 @@ int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_shor
 int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t s);
 int pattern_numeric_long(const Pattern& p, int* ccol, double* cval, hipStream_t s);
 
+// Caller-owned C of a Precise product: row pointer (M+1) and cap entries of
+// columns and values.  When nnz(C) > cap the call fails with BRM_E_OVERFLOW
+// and reports nnz(C) in *nnz_needed.
+struct ExtOut {
+    int64_t* crpt;
+    int* ccol;
+    double* cval;
+    int64_t cap;
+    int64_t* nnz_needed;
+};
+
 // One SpGEMM.  ph (phase seconds, P_N entries) may be null: then no phase
-// events are recorded.
-int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph);
+// events are recorded.  ext (Precise only) may be null: then C is allocated.
+int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph,
+               const ExtOut* ext = nullptr);
 
 // The calling thread's device for library calls (set by brm_init).
 int thread_device();
@@ -162,6 +175,9 @@ void set_thread_device(int device);
 // device's free memory at each call).
 int64_t thread_ws_budget();
 void set_thread_ws_budget(int64_t bytes);
+// Device address of the no-allocation entry's fixed scratch on `device`
+// (the current device), resolved once per device.
+int noalloc_setup(int device);
 
 // Fused operands of a batch: A_i stacked by rows with columns shifted to
 // their B_i's rows; B_i stacked by rows, or, when a_col_off is given, not

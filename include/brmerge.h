@@ -207,6 +207,25 @@ typedef struct brm_stats {
 } brm_stats;
 int brm_last_stats(brm_stats *out);
 
+/*
+ * BRMerge-Precise with no device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * caller passes A and B (device) and C's arrays: crpt (M+1 entries) and
+ * `cap` entries of ccol / cval.  Every scratch buffer comes from one fixed
+ * device array of brm_noalloc_scratch_bytes() bytes (256 MiB), held by each
+ * GPU the library runs on until the process ends.  Products whose scratch
+ * would exceed it fail with BRM_E_NOMEM; when nnz(C) > cap the call fails
+ * with BRM_E_OVERFLOW and *cnnz is nnz(C).  Calls on one GPU share the
+ * array: the library serialises them, and each returns only once its
+ * stream is idle.  Results are bit-identical to brm_spgemm_precise_dev.
+ */
+int brm_spgemm_precise_dev_noalloc(const int *arpt, const int *acol, const double *aval, int64_t nnzA,
+                                   const int *brpt, const int *bcol, const double *bval, int64_t nnzB, int M, int K,
+                                   int N, int64_t *crpt, int *ccol, double *cval, int64_t cap, int64_t *cnnz,
+                                   void *stream);
+uint64_t brm_noalloc_scratch_bytes(void);
+
 /* Order-independent 64-bit checksum of `bytes` (a multiple of 4) of device
  * memory: the sum mod 2^64 of a SplitMix64 mix of (word index, word).  No
  * counterpart in the reference; it lets callers compare results of
