@@ -267,8 +267,21 @@ def run_batched(args, bm):
         pel = (time.perf_counter() - t0) / args.steps
         info = plan.info()
         plan.close()
+        # like for like: a loop of replays of one-product plans (the same
+        # sample of products as the one-call loop), each plan built once
+        singles = [bm.BatchPlan([dev[i]], autotune=False) for i in range(ns)]
+        for sp in singles:
+            sp.execute()
+        t0 = time.perf_counter()
+        for sp in singles:
+            sp.execute()
+        single_rate = ns / (time.perf_counter() - t0)
+        for sp in singles:
+            sp.close()
         plan_out = {"products_per_s": round(n / pel, 2), "ms_per_execute": round(pel * 1e3, 3),
                     "create_s": round(tcreate, 3), "gflops": round(2.0 * ip / pel / 1e9, 3),
+                    "single_product_plans": {"products_per_s": round(single_rate, 2), "sample": ns},
+                    "speedup_vs_single_plans": round(n / pel / single_rate, 2),
                     "tuned": {"lds_cfg": info["lds_cfg"], "med_cap": info["med_cap"], "wmed_cap": info["wmed_cap"],
                               "tuning_ms": info["tuning_ms"]}}
     for a, b in dev:
@@ -285,7 +298,7 @@ def run_batched(args, bm):
         "gflops": round(2.0 * ip / el / 1e9, 3),
         "one_call_per_product": {"products_per_s": round(loop_rate, 2), "sample": ns},
         "speedup_vs_loop": round(rate / loop_rate, 2),
-        "plan_speedup_vs_loop": round(plan_out["products_per_s"] / loop_rate, 2) if plan_out else None,
+        "plan_speedup_vs_single_plans": plan_out["speedup_vs_single_plans"] if plan_out else None,
         "plan": plan_out, "kernels_per_step": kernels, "stats": st,
     }
 
@@ -321,7 +334,7 @@ def run_serve(args, bm):
     loop_s = time.perf_counter() - t0
     loop_rate = n / loop_s
     out = {}
-    with bm.Server(budget_bytes=64 << 30, max_batch=args.max_batch, streams=2) as srv:
+    with bm.Server(budget_bytes=64 << 30, max_batch=args.max_batch, streams=args.streams) as srv:
         ids = [srv.add_b(d) for d in dBs]
 
         def drive(rate):
@@ -361,7 +374,7 @@ def run_serve(args, bm):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "serve", "requests": n, "resident_b": len(Bs), "b": f"rmat{scale}",
                    "median_rows": int(np.median(Ms)), "intermediate_products": ip_tot,
-                   "budget_bytes": 64 << 30, "max_batch": args.max_batch, "streams": 2},
+                   "budget_bytes": 64 << 30, "max_batch": args.max_batch, "streams": args.streams},
         "gflops_saturated": round(2.0 * ip_tot / span / 1e9, 3),
         "saturated": {"requests_per_s": round(sat_rate, 2), "p50_ms": round(1e3 * float(np.median(lat)), 3),
                       "p99_ms": round(1e3 * float(np.quantile(lat, 0.99)), 3),
@@ -425,14 +438,16 @@ def main():
     ap.add_argument("--workload", default="rmat20",
                     help="rmatS | er16k | amgN (stencil*aggregation) | galerkinN (A*P then P^T*(AP)) | batched | serve")
     ap.add_argument("--pairs", type=int, default=4096, help="batched: number of products")
-    ap.add_argument("--loop-sample", type=int, default=256, help="batched: products timed one call each")
+    ap.add_argument("--loop-sample", type=int, default=1024, help="batched: products timed one call each")
     ap.add_argument("--no-plan", action="store_true", help="batched: skip the graph-captured plan")
     ap.add_argument("--requests", type=int, default=256, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve: offered requests/s (0: half of saturation)")
     ap.add_argument("--max-batch", type=int, default=64, help="serve: requests fused per batch")
+    ap.add_argument("--streams", type=int, default=2, help="serve: worker streams")
     ap.add_argument("--no-upper", action="store_true", help="do not time BRMerge-Upper")
     ap.add_argument("--vendor", action="store_true",
                     help="also time rocSPARSE SpGEMM on the same product (int32 indices: nnz(C) < 2^31)")
+    ap.add_argument("--hash", action="store_true", help="also time the Hash-SpGEMM comparator tier (hash_cap 1024)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-profile", action="store_true", help="do not bracket launches with hipEvents")
     ap.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
@@ -584,6 +599,23 @@ def main():
                         "ms_per_step": round(1000 * float(np.mean(ts)), 3), "steps": len(ts), "warmup": 1,
                         "vs_precise": round(float(np.mean(ts)) / (elapsed / args.steps), 3),
                         "phases_ms": {k: round(1000 * float(np.mean([p[k] for p in tph])), 3) for k in tph[0]}}
+    if args.hash:
+        # the Hash-SpGEMM comparator tier on the rows it takes (at most 1,024
+        # products and lists), BRMerge's tiers on the others
+        bm.set_param("hash_cap", 1024)
+        C = bm.spgemm_dev(dA, dB)
+        C.free()
+        ts = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            C = bm.spgemm_dev(dA, dB)
+            ts.append(time.perf_counter() - t0)
+            C.free()
+        bm.set_param("hash_cap", 0)
+        out["hash_tier"] = {"value": round(2.0 * total_ip / np.mean(ts) / 1e9, 3), "unit": "GFLOP/s",
+                            "ms_per_step": round(1000 * float(np.mean(ts)), 3),
+                            "rows_in_tier": int(((ip > 0) & (ip <= 1024) & (L <= 1024)).sum()),
+                            "vs_precise": round(float(np.mean(ts)) / (elapsed / args.steps), 3)}
     if args.vendor:
         if nnz_c >= 2 ** 31:
             out["rocsparse"] = {"skipped": "nnz(C) exceeds the int32 indices of the rocSPARSE path"}

@@ -7,7 +7,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbrmerge.so")
+# BRM_LIB selects another build of the library (e.g. a kernel-shape variant)
+LIB_PATH = os.environ.get("BRM_LIB") or os.path.join(HERE, "libbrmerge.so")
 
 BRM_OK = 0
 BRM_E_INVALID = 1

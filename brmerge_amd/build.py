@@ -21,7 +21,7 @@ COMPAT_CALLER = os.path.join(os.path.dirname(HERE), "tests", "compat", "compat_c
 
 HIP_SOURCES = ["alloc.hip", "scan.hip", "flop.hip", "short.hip", "lds.hip", "long.hip", "assemble.hip", "api.hip",
                "batch.hip", "serve.hip", "bmerge.hip", "rmed.hip",
-               "transpose.hip", "hash.hip", "noalloc.hip"]
+               "transpose.hip", "hash.hip", "noalloc.hip", "small.hip"]
 CXX_SOURCES = ["gen.cpp"]
 
 ARCH = os.environ.get("BRM_OFFLOAD_ARCH", "gfx950")

@@ -1,12 +1,13 @@
 // Caching device allocator.  hipMalloc/hipFree of the multi-GB outputs and
 // workspaces of a large SpGEMM cost milliseconds each; blocks are therefore
 // kept after release and reused by later calls of the same or smaller size.
-// On allocation failure every cached block is returned to the driver and the
-// request retried once.  All library calls synchronise their stream before
+// On allocation failure cached blocks are returned to the driver, largest
+// first, until the request fits.  All library calls synchronise their stream before
 // returning, so a released block is never still in use by the device.
 #include <stdio.h>
 #include <string.h>
 
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <string>
@@ -80,22 +81,29 @@ void* dev_alloc(size_t bytes) {
         g_used[p] = bsz;
         return p;
     }
+    // On failure, return cached blocks to the driver, largest first, only
+    // until the request fits: blocks that later calls of the same sizes will
+    // reuse stay cached.
     void* p = nullptr;
-    if (hipMalloc(&p, sz) != hipSuccess) {
+    while (hipMalloc(&p, sz) != hipSuccess) {
         (void)hipGetLastError();
-        release_cached_locked();
-        if (hipMalloc(&p, sz) != hipSuccess) {
-            (void)hipGetLastError();
-            return nullptr;
-        }
+        p = nullptr;
+        if (g_free.empty()) return nullptr;
+        auto big = std::prev(g_free.end());
+        (void)hipFree(big->second);
+        g_cached -= big->first;
+        g_free.erase(big);
     }
     g_used[p] = sz;
     return p;
 }
 
+static bool vm_free(void* p);
+
 void dev_free(void* p) {
     if (!p) return;
     if (t_arena.base && (char*)p >= t_arena.base && (char*)p < t_arena.base + t_arena.size) return;
+    if (vm_free(p)) return;
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_used.find(p);
     if (it == g_used.end()) return;  // not ours
@@ -125,6 +133,118 @@ void dev_trim() {
 size_t dev_cached_bytes() {
     std::lock_guard<std::mutex> lk(g_mu);
     return g_cached;
+}
+
+// ---------------------------------------------------------------- growable arrays
+// A device array whose final size is known only after its first elements are
+// written: a virtual address range reserved for the largest size it may
+// reach, backed by physical memory (hipMemCreate) only as it grows, in pieces
+// of exactly VM_STEP (the reservation is a whole number of pieces; on MI355X
+// access could not be granted to a larger piece).  BRMerge-Upper's row waves
+// compact into C this way.
+namespace {
+constexpr size_t VM_STEP = size_t(2) << 30;
+struct VmBlock {
+    size_t reserved = 0;
+    size_t mapped = 0;
+    size_t gran = 0;  // piece size
+    int dev = 0;
+    std::vector<std::pair<hipMemGenericAllocationHandle_t, size_t>> handles;
+};
+std::mutex v_mu;
+std::unordered_map<void*, VmBlock> v_blocks;  // by base address
+
+hipMemAllocationProp vm_prop(int dev) {
+    hipMemAllocationProp prop{};
+    prop.type = hipMemAllocationTypePinned;
+    prop.location.type = hipMemLocationTypeDevice;
+    prop.location.id = dev;
+    return prop;
+}
+
+void vm_release_block(void* va, VmBlock& b) {
+    size_t off = 0;
+    for (auto& h : b.handles) {
+        (void)hipMemUnmap((char*)va + off, h.second);
+        (void)hipMemRelease(h.first);
+        off += h.second;
+    }
+    (void)hipMemAddressFree(va, b.reserved);
+    (void)hipGetLastError();
+}
+}  // namespace
+
+void* vm_reserve(size_t bytes, int dev) {
+    hipMemAllocationProp prop = vm_prop(dev);
+    size_t gran = 0;
+    if (hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityMinimum) != hipSuccess || gran == 0) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    const size_t piece = (VM_STEP + gran - 1) / gran * gran;
+    const size_t res = (std::max<size_t>(bytes, 1) + piece - 1) / piece * piece;
+    void* va = nullptr;
+    if (hipMemAddressReserve(&va, res, 0, nullptr, 0) != hipSuccess || !va) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    VmBlock b;
+    b.reserved = res;
+    b.gran = piece;
+    b.dev = dev;
+    std::lock_guard<std::mutex> lk(v_mu);
+    v_blocks[va] = std::move(b);
+    return va;
+}
+
+bool vm_grow(void* va, size_t bytes) {
+    std::lock_guard<std::mutex> lk(v_mu);
+    auto it = v_blocks.find(va);
+    if (it == v_blocks.end()) return false;
+    VmBlock& b = it->second;
+    if (bytes > b.reserved) return false;
+    const hipMemAllocationProp prop = vm_prop(b.dev);
+    while (b.mapped < bytes) {
+        const size_t sz = b.gran;  // one piece (the reservation is a whole number of them)
+        hipMemGenericAllocationHandle_t h;
+        // physical memory beside the caching allocator: when it runs out,
+        // cached blocks go back to the driver, largest first
+        while (hipMemCreate(&h, sz, &prop, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            std::lock_guard<std::mutex> gk(g_mu);
+            if (g_free.empty()) return false;
+            auto big = std::prev(g_free.end());
+            (void)hipFree(big->second);
+            g_cached -= big->first;
+            g_free.erase(big);
+        }
+        if (hipMemMap((char*)va + b.mapped, sz, 0, h, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipMemRelease(h);
+            return false;
+        }
+        hipMemAccessDesc d{};
+        d.location = prop.location;
+        d.flags = hipMemAccessFlagsProtReadWrite;
+        if (hipMemSetAccess((char*)va + b.mapped, sz, &d, 1) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipMemUnmap((char*)va + b.mapped, sz);
+            (void)hipMemRelease(h);
+            return false;
+        }
+        b.handles.push_back({h, sz});
+        b.mapped += sz;
+    }
+    return true;
+}
+
+static bool vm_free(void* p) {
+    std::lock_guard<std::mutex> lk(v_mu);
+    auto it = v_blocks.find(p);
+    if (it == v_blocks.end()) return false;
+    vm_release_block(p, it->second);
+    v_blocks.erase(it);
+    return true;
 }
 
 // ---------------------------------------------------------------- profiling

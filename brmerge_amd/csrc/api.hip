@@ -31,7 +31,7 @@
 namespace brm {
 
 Params& params() {
-    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, -1, 0, 1, 0, (int64_t)1 << 30, 0};
+    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, -1, 0, 1, 0, (int64_t)1 << 30, 0, 1, 1};
     return p;
 }
 
@@ -217,8 +217,8 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     const int64_t arena = arena_remaining();
     if (arena >= 0) {
         // the arena bounds everything this function allocates: per long row,
-        // 20 bytes of wave tables beside row_bytes(), and alignment slack
-        budget = arena - (int64_t)20 * nlong - (int64_t)64 * 1024;
+        // 28 bytes of wave tables beside row_bytes(), and alignment slack
+        budget = arena - (int64_t)28 * nlong - (int64_t)64 * 1024;
         if (budget < 0) {
             set_err("long rows need more scratch memory than the no-allocation entry's fixed buffer holds");
             return BRM_E_NOMEM;
@@ -226,32 +226,51 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     } else if (budget <= 0) {
         size_t fr = 0, tot = 0;
         HIPC(hipMemGetInfo(&fr, &tot));
-        budget = std::min((int64_t)(fr + dev_cached_bytes()) / 10 * 7, (int64_t)tot / 2);
+        const int64_t avail = (int64_t)(fr + dev_cached_bytes());
+        budget = std::min(avail / 10 * 7, (int64_t)tot / 2);
         budget = budget / (int64_t(4) << 30) * (int64_t(4) << 30);
         if (budget < (int64_t)(1) << 30) budget = (int64_t)1 << 30;
+        // Keep the previous call's budget while it still fits: the free
+        // memory seen here moves a little from call to call, and a budget one
+        // quantum off would re-partition the waves, resize every buffer and
+        // defeat the allocator's cache.
+        static thread_local int64_t last = 0;
+        if (last > 0 && last <= avail / 10 * 8 && last <= (int64_t)tot / 2) budget = last;
+        last = budget;
     }
+    const int group_cfg = params().group_cfg;
+    const bool bitmap = group_cfg == GROUP_CFG_BITMAP;
+    const int pass_bits = bitmap ? BPASS_BITS : PASS_BITS;
+    // Workspace elements row k stores in buffer b.  Pass j writes its nodes
+    // below the root into buffer j & 1 and the root writes C, so with the
+    // bitmap engine a row of one pass stores nothing and a row of two passes
+    // only pass-1 nodes (buffer 1).  The merge-path engine's leaves may use
+    // either buffer.
+    auto ws_need = [&](int k, int b) -> int64_t {
+        if (!bitmap) return ip[k];
+        const int H = ceil_log2_host(len[k]);
+        const int J = H == 0 ? 1 : (H + pass_bits - 1) / pass_bits;
+        return (b == 1 ? J >= 2 : J >= 3) ? ip[k] : 0;
+    };
     auto row_bytes = [&](int k) {
         const int H = ceil_log2_host(len[k]);
-        return 24 * ip[k] + 8 * ((int64_t)2 << H) +
+        return 12 * (ws_need(k, 0) + ws_need(k, 1)) + 8 * ((int64_t)2 << H) +
                (int64_t)(sizeof(GroupTask) + sizeof(LeafTask) + 2 * sizeof(Child)) * (len[k] + 16);
     };
 
     tr.mark("long info read back");
-    const int group_cfg = params().group_cfg;
-    const bool bitmap = group_cfg == GROUP_CFG_BITMAP;
-    const int pass_bits = bitmap ? BPASS_BITS : PASS_BITS;
     // ---- partition into waves and size every buffer for the largest wave
     struct Wave {
         int k0, k1;
-        int64_t ws_tot, sl_tot, leaf_cap, child_cap;
-        std::vector<int64_t> wsb, slb, seg;
+        int64_t ws_tot[2], sl_tot, leaf_cap, child_cap;
+        std::vector<int64_t> wsb, slb, seg;  // wsb: two per row (buffers 0 and 1)
     };
     std::vector<Wave> waves;
-    int64_t mx_ws = 0, mx_sl = 0, mx_leaf = 0, mx_child = 0, mx_task = 0;
+    int64_t mx_ws[2] = {0, 0}, mx_sl = 0, mx_leaf = 0, mx_child = 0, mx_task = 0;
     int mx_rows = 0;
   for (;;) {  // (the arena case re-plans with a smaller budget until the buffers fit)
     waves.clear();
-    mx_ws = mx_sl = mx_leaf = mx_child = mx_task = 0;
+    mx_ws[0] = mx_ws[1] = mx_sl = mx_leaf = mx_child = mx_task = 0;
     mx_rows = 0;
     for (int k0 = 0; k0 < nlong;) {
         int k1 = k0;
@@ -260,8 +279,8 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         Wave w;
         w.k0 = k0;
         w.k1 = k1;
-        w.ws_tot = w.sl_tot = w.leaf_cap = w.child_cap = 0;
-        w.wsb.resize(k1 - k0);
+        w.ws_tot[0] = w.ws_tot[1] = w.sl_tot = w.leaf_cap = w.child_cap = 0;
+        w.wsb.resize(2 * (size_t)(k1 - k0));
         w.slb.resize(k1 - k0);
         std::vector<int64_t> capP(MAX_PASSES + 1, 0);
         for (int k = 0; k < k1 - k0; ++k) {
@@ -272,9 +291,11 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
                 set_err("long row with 2^%d lists exceeds the supported tree height", H);
                 return BRM_E_INVALID;
             }
-            w.wsb[k] = w.ws_tot;
+            w.wsb[2 * k] = w.ws_tot[0];
+            w.wsb[2 * k + 1] = w.ws_tot[1];
             w.slb[k] = w.sl_tot;
-            w.ws_tot += ip[k0 + k];
+            w.ws_tot[0] += ws_need(k0 + k, 0);
+            w.ws_tot[1] += ws_need(k0 + k, 1);
             w.sl_tot += (int64_t)2 << H;
             w.leaf_cap += Lr;
             w.child_cap += 2 * (int64_t)Lr + 2;
@@ -282,7 +303,8 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         }
         w.seg.assign(MAX_PASSES + 2, 0);
         for (int j = 0; j <= MAX_PASSES; ++j) w.seg[j + 1] = w.seg[j] + capP[j];
-        mx_ws = std::max(mx_ws, w.ws_tot);
+        mx_ws[0] = std::max(mx_ws[0], w.ws_tot[0]);
+        mx_ws[1] = std::max(mx_ws[1], w.ws_tot[1]);
         mx_sl = std::max(mx_sl, w.sl_tot);
         mx_leaf = std::max(mx_leaf, w.leaf_cap);
         mx_child = std::max(mx_child, w.child_cap);
@@ -293,9 +315,9 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     }
     if (arena < 0) break;
     // bytes of the buffers allocated below (each sized for its largest wave)
-    const int64_t need = 24 * mx_ws + 8 * mx_sl + (int64_t)sizeof(LeafTask) * mx_leaf +
+    const int64_t need = 12 * (mx_ws[0] + mx_ws[1]) + 8 * mx_sl + (int64_t)sizeof(LeafTask) * mx_leaf +
                          (int64_t)sizeof(Child) * mx_child + (int64_t)sizeof(GroupTask) * mx_task +
-                         20 * (int64_t)mx_rows + 8 * (CNT_N + MAX_PASSES + 2) + 16 * 256;
+                         28 * (int64_t)mx_rows + 8 * (CNT_N + MAX_PASSES + 2) + 16 * 256;
     if (need <= arena) break;
     if ((int)waves.size() == nlong) {
         set_err("a long row needs more scratch memory than the no-allocation entry's fixed buffer holds");
@@ -315,7 +337,8 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         return (n + g - 1) / g * g;
     };
     if (arena < 0) {
-        mx_ws = round_geo(mx_ws, (int64_t)1 << 20);
+        mx_ws[0] = round_geo(mx_ws[0], (int64_t)1 << 20);
+        mx_ws[1] = round_geo(mx_ws[1], (int64_t)1 << 20);
         mx_sl = round_geo(mx_sl, (int64_t)1 << 16);
         mx_leaf = round_geo(mx_leaf, (int64_t)1 << 14);
         mx_child = round_geo(mx_child, (int64_t)1 << 14);
@@ -330,19 +353,19 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     GroupTask* tasks;
     Child* children;
     BRMC(wsc.alloc(&rows_w, mx_rows, "wave rows"));
-    BRMC(wsc.alloc(&wsb_d, mx_rows, "wave ws_base"));
+    BRMC(wsc.alloc(&wsb_d, 2 * (int64_t)mx_rows, "wave ws_base"));
     BRMC(wsc.alloc(&slb_d, mx_rows, "wave slot_base"));
     BRMC(wsc.alloc(&seg_d, MAX_PASSES + 2, "task segments"));
-    BRMC(wsc.alloc(&wcol0, mx_ws, "workspace"));
-    BRMC(wsc.alloc(&wcol1, mx_ws, "workspace"));
-    BRMC(wsc.alloc(&wval0, mx_ws, "workspace"));
-    BRMC(wsc.alloc(&wval1, mx_ws, "workspace"));
+    BRMC(wsc.alloc(&wcol0, mx_ws[0], "workspace"));
+    BRMC(wsc.alloc(&wcol1, mx_ws[1], "workspace"));
+    BRMC(wsc.alloc(&wval0, mx_ws[0], "workspace"));
+    BRMC(wsc.alloc(&wval1, mx_ws[1], "workspace"));
     BRMC(wsc.alloc(&slots, mx_sl, "length slots"));
     BRMC(wsc.alloc(&counters, CNT_N, "planner counters"));
     BRMC(wsc.alloc(&leaves, mx_leaf, "leaf tasks"));
     BRMC(wsc.alloc(&tasks, mx_task, "group tasks"));
     BRMC(wsc.alloc(&children, mx_child, "group children"));
-    tr.mark("workspace allocated", mx_ws);
+    tr.mark("workspace allocated", mx_ws[0] + mx_ws[1]);
     int* const wcol[2] = {wcol0, wcol1};
     double* const wval[2] = {wval0, wval1};
     int* derr = o.err;
@@ -351,7 +374,7 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     for (const Wave& w : waves) {
         const int nw = w.k1 - w.k0;
         HIPC(hipMemcpyAsync(rows_w, rows.data() + w.k0, sizeof(int) * nw, hipMemcpyHostToDevice, s));
-        HIPC(hipMemcpyAsync(wsb_d, w.wsb.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(wsb_d, w.wsb.data(), sizeof(int64_t) * 2 * nw, hipMemcpyHostToDevice, s));
         HIPC(hipMemcpyAsync(slb_d, w.slb.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice, s));
         HIPC(hipMemcpyAsync(seg_d, w.seg.data(), sizeof(int64_t) * (MAX_PASSES + 2), hipMemcpyHostToDevice, s));
         HIPC(launch_fill_i64(counters, CNT_N, 0, s));
@@ -526,7 +549,8 @@ int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_shor
     HIPC(launch_wmed_count(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, p.rowcnt, s, std::min(p.wmed_cap, WMED_SPLIT)));
     HIPC(launch_wmed_count(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, p.rowcnt, s, p.wmed_cap));
     HIPC(launch_medium_count(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, p.rowcnt, s));
-    HIPC(launch_hash_count(p.rows(BIN_HASH), p.bc[BIN_HASH], p.a, p.rowcnt, s));
+    HIPC(launch_hash_count(p.rows(BIN_HASH), p.bc[BIN_HASH], p.a, p.rowcnt, s, HASH_SMALL));
+    HIPC(launch_hash_count(p.rows(BIN_HASH2), p.bc[BIN_HASH2], p.a, p.rowcnt, s, HASH_CAP));
     HIPC(launch_long_count(p.rows(BIN_LONG), p.bc[BIN_LONG], p.a, p.rowcnt, s));
     if (clk) clk->begin(P_PNNZ);
     HIPC(scan_i64(p.rowcnt, p.a.M, p.crpt, s));
@@ -562,7 +586,8 @@ int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t
     HIPC(launch_wmed_numeric(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
     HIPC(launch_wmed_numeric(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, o, s, p.wmed_cap));
     HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
-    HIPC(launch_hash_numeric(p.rows(BIN_HASH), p.bc[BIN_HASH], p.a, o, s));
+    HIPC(launch_hash_numeric(p.rows(BIN_HASH), p.bc[BIN_HASH], p.a, o, s, HASH_SMALL));
+    HIPC(launch_hash_numeric(p.rows(BIN_HASH2), p.bc[BIN_HASH2], p.a, o, s, HASH_CAP));
     return BRM_OK;
 }
 
@@ -644,6 +669,36 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
         for (int w = 0; w < nwaves; ++w) mrows = std::max(mrows, wb[w + 1] - wb[w]);
         BRMC(sc.alloc(&crpt_w, (int64_t)mrows + 1, "wave row pointer"));
     }
+    // Several waves: C grows wave by wave in reserved address ranges (physical
+    // memory mapped as it grows, nothing moved); where the driver cannot
+    // reserve them, each wave is compacted into an exact-size chunk and the
+    // chunks are copied into C at the end.
+    int* vcol = nullptr;
+    double* vval = nullptr;
+    int64_t voff = 0;
+    if (nwaves > 1 && params().upper_vm) {
+        vcol = (int*)vm_reserve(sizeof(int) * (size_t)p.total_ip, thread_device());
+        if (vcol) sc.bufs.push_back(vcol);
+        vval = vcol ? (double*)vm_reserve(sizeof(double) * (size_t)p.total_ip, thread_device()) : nullptr;
+        if (vval) sc.bufs.push_back(vval);
+        // map a first step of each now, so that a driver without physical
+        // allocations for reserved ranges falls back before any wave runs
+        if (vcol && (!vval || !vm_grow(vcol, 1) || !vm_grow(vval, 1))) {
+            sc.free_now(vcol);
+            if (vval) sc.free_now(vval);
+            vcol = nullptr;
+            vval = nullptr;
+        }
+    }
+    // The long-row workspace of every wave gets the same budget (C-bar's), so
+    // that the waves' buffers are sized alike and reused from the cache.
+    struct WsBudgetScope {
+        int64_t saved;
+        explicit WsBudgetScope(int64_t b) : saved(thread_ws_budget()) {
+            if (b > 0 && saved <= 0 && params().ws_budget_bytes <= 0) set_thread_ws_budget(b);
+        }
+        ~WsBudgetScope() { set_thread_ws_budget(saved); }
+    } ws_scope(nwaves > 1 ? budget : 0);
     Scratch chunks;
     std::vector<int*> ch_col;
     std::vector<double*> ch_val;
@@ -682,7 +737,8 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
         HIPC(launch_wmed_numeric(rows(BIN_WMED2), bc[BIN_WMED2], aw, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
         HIPC(launch_wmed_numeric(rows(BIN_WMED3), bc[BIN_WMED3], aw, o, s, p.wmed_cap));
         HIPC(launch_medium_numeric(rows(BIN_MEDIUM), bc[BIN_MEDIUM], aw, o, s, p.lds_cfg));
-        HIPC(launch_hash_numeric(rows(BIN_HASH), bc[BIN_HASH], aw, o, s));
+        HIPC(launch_hash_numeric(rows(BIN_HASH), bc[BIN_HASH], aw, o, s, HASH_SMALL));
+        HIPC(launch_hash_numeric(rows(BIN_HASH2), bc[BIN_HASH2], aw, o, s, HASH_CAP));
         BRMC(run_long(aw, rows(BIN_LONG), bc[BIN_LONG], o, nullptr, s, sc));
         BRMC(check_dev_err(p.derr, s, "numeric"));
         if (nwaves == 1) {
@@ -709,11 +765,22 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
         clk.begin(P_ALLOC);
         int* cc;
         double* cv;
-        BRMC(chunks.alloc(&cc, nw, "C chunk col"));
-        BRMC(chunks.alloc(&cv, nw, "C chunk val"));
-        ch_col.push_back(cc);
-        ch_val.push_back(cv);
-        ch_n.push_back(nw);
+        if (vcol) {
+            if (!vm_grow(vcol, sizeof(int) * (size_t)(voff + nw)) ||
+                !vm_grow(vval, sizeof(double) * (size_t)(voff + nw))) {
+                set_err("mapping %lld more entries of C failed", (long long)nw);
+                return BRM_E_NOMEM;
+            }
+            cc = vcol + voff;
+            cv = vval + voff;
+            voff += nw;
+        } else {
+            BRMC(chunks.alloc(&cc, nw, "C chunk col"));
+            BRMC(chunks.alloc(&cv, nw, "C chunk val"));
+            ch_col.push_back(cc);
+            ch_val.push_back(cv);
+            ch_n.push_back(nw);
+        }
         clk.begin(P_COPY);
         HIPC(launch_compact(crpt_w, aw.M, aw, o.ccol, o.cval, cc, cv, nw, s));
         dev_flush_deferred();
@@ -729,6 +796,17 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
     if (nnzC > max_nnz) {
         set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnzC);
         return BRM_E_OVERFLOW;
+    }
+    if (vcol) {
+        if (voff != nnzC) {
+            set_err("Upper row waves: %lld entries compacted, nnz(C) = %lld", (long long)voff, (long long)nnzC);
+            return BRM_E_INTERNAL;
+        }
+        HIPC(hipStreamSynchronize(s));
+        *ccol_out = vcol;
+        *cval_out = vval;
+        *nnz_out = nnzC;
+        return BRM_OK;
     }
     clk.begin(P_ALLOC);
     BRMC(sc.alloc(ccol_out, nnzC, "C col"));
@@ -752,6 +830,77 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
 }
 }  // namespace
 
+namespace {
+// Products of at most SMALL_MAX_ROWS rows whose rows all have at most 64
+// lists and products: three launches and one host round trip (small.hip).
+// C is allocated for 64 entries per row (a bound of nnz(C) when every row is
+// short), so nothing waits on the device before the end.  *handled = false
+// when a row is longer (nothing is kept; the general pipeline runs).
+int run_small(const DevIn& in, int64_t max_nnz, hipStream_t s, PhaseClock& clk, DevOut* out, bool* handled) {
+    *handled = false;
+    if (in.nnzA > INT32_MAX || in.nnzB > INT32_MAX) return BRM_OK;
+    const int M = in.M;
+    const int64_t cap = (int64_t)M * 64;
+    Scratch sc;
+    SmallState* st;
+    int64_t *rowcnt, *crpt;
+    int *scol, *ccol;
+    double *sval, *cval;
+    clk.begin(P_PRE);
+    BRMC(sc.alloc(reinterpret_cast<int64_t**>(&st), small_state_words(M), "small-product state"));
+    BRMC(sc.alloc(&rowcnt, M, "row nnz"));
+    BRMC(sc.alloc(&scol, cap, "short-row stash"));
+    BRMC(sc.alloc(&sval, cap, "short-row stash"));
+    BRMC(sc.alloc(&crpt, (int64_t)M + 1, "crpt"));
+    BRMC(sc.alloc(&ccol, std::min(cap, max_nnz), "C col"));
+    BRMC(sc.alloc(&cval, std::min(cap, max_nnz), "C val"));
+    RowArgs a{};
+    a.arpt = in.arpt;
+    a.acol = in.acol;
+    a.aval = in.aval;
+    a.brpt = in.brpt;
+    a.bcol = in.bcol;
+    a.bval = in.bval;
+    a.M = M;
+    a.K = in.K;
+    a.N = in.N;
+    clk.begin(P_SYM);
+    HIPC(launch_small(a, in.nnzA, in.nnzB, scol, sval, rowcnt, st, s));
+    clk.begin(P_COMPUTE);
+    HIPC(launch_small_emit(M, rowcnt, scol, sval, crpt, ccol, cval, st, s));
+    SmallState h{};
+    BRMC(d2h(&h, st, sizeof(h), s));
+    if (h.err & DERR_INVALID) {
+        set_err("validation: malformed CSR input (row pointers, column range or unsorted/duplicate B columns)");
+        return BRM_E_INVALID;
+    }
+    if (h.err) return dev_err_status(h.err, "small product");
+    if (h.miss) return BRM_OK;
+    int64_t nnz = 0, ipt = 0;
+    for (int i = 0; i < SMALL_SLOTS; ++i) {
+        nnz += h.nnz[i];
+        ipt += h.ip[i];
+    }
+    if (nnz > max_nnz) {
+        set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnz);
+        return BRM_E_OVERFLOW;
+    }
+    sc.release(crpt);
+    sc.release(ccol);
+    sc.release(cval);
+    out->crpt = crpt;
+    out->ccol = ccol;
+    out->cval = cval;
+    out->nnz = nnz;
+    brm_stats& ts = thread_stats();
+    ts.total_ip = ipt;
+    ts.nnz_c = nnz;
+    ts.rows_by_tier[BIN_SHORT] = M;
+    *handled = true;
+    return BRM_OK;
+}
+}  // namespace
+
 int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph,
                const ExtOut* ext) {
     PhaseClock clk(s, ph != nullptr);
@@ -764,12 +913,28 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         return BRM_E_INVALID;
     }
     if (ext) p.crpt_ext = ext->crpt;
+    // the fused path when every row is likely short: lists per row times
+    // entries per B row at most 64 on average
+    const double avg_l = in.M > 0 ? (double)in.nnzA / in.M : 0.0, avg_b = in.K > 0 ? (double)in.nnzB / in.K : 0.0;
+    if (!upper && !ext && params().small_fused && in.M > 0 && in.M <= SMALL_MAX_ROWS && !params().force_long &&
+        params().hash_cap == 0 && avg_l * avg_b <= 64.0) {
+        bool handled = false;
+        BRMC(run_small(in, max_nnz, s, clk, out, &handled));
+        if (handled) {
+            clk.begin(P_CLEAN);
+            HIPC(hipStreamSynchronize(s));
+            clk.end();
+            if (ph) clk.collect(ph);
+            dev_flush_deferred();
+            return BRM_OK;
+        }
+    }
     BRMC(pattern_prepare(in, s, &clk, p));
     const int M = in.M;
     for (int b = 0; b < 4; ++b) t_stats.rows_by_tier[b] = p.bc[b];
     t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_WMED] + p.bc[BIN_WMED2] + p.bc[BIN_WMED3];
     t_stats.rows_by_tier[BIN_SHORT] += p.bc[BIN_TINY];
-    t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_HASH];
+    t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_HASH] + p.bc[BIN_HASH2];
     t_stats.total_ip = p.total_ip;
     Scratch& sc = p.sc;
     int* ccol = nullptr;
@@ -1190,6 +1355,8 @@ int brm_set_param(const char* name, int64_t value) {
     else if (!strcmp(name, "launch_chunk")) { if (value < 0) return BRM_E_INVALID; p.launch_chunk = value; }
     else if (!strcmp(name, "wmed_engine")) { if (value < 0 || value > 1) return BRM_E_INVALID; p.wmed_engine = (int)value; }
     else if (!strcmp(name, "cbar_budget_bytes")) { if (value < 0) return BRM_E_INVALID; p.cbar_budget_bytes = value; }
+    else if (!strcmp(name, "upper_vm")) { p.upper_vm = value ? 1 : 0; }
+    else if (!strcmp(name, "small_fused")) { p.small_fused = value ? 1 : 0; }
     else if (!strcmp(name, "hash_cap")) { if (value < 0 || value > HASH_CAP) return BRM_E_INVALID; p.hash_cap = (int)value; }
     else if (!strcmp(name, "sym_chunk_products")) { if (value < 1 || value > ((int64_t)1 << 30)) return BRM_E_INVALID; p.sym_chunk_products = value; }
     else return BRM_E_INVALID;
@@ -1212,6 +1379,8 @@ int64_t brm_get_param(const char* name) {
     if (!strcmp(name, "cbar_budget_bytes")) return p.cbar_budget_bytes;
     if (!strcmp(name, "sym_chunk_products")) return p.sym_chunk_products;
     if (!strcmp(name, "hash_cap")) return p.hash_cap;
+    if (!strcmp(name, "upper_vm")) return p.upper_vm;
+    if (!strcmp(name, "small_fused")) return p.small_fused;
     return -1;
 }
 

@@ -9,9 +9,9 @@
 //      column (the lower edge of the median log2-extent among partially
 //      loaded children) share the free window space as top-ups; all loaded
 //      elements with a column <= bound (the smallest last loaded column of
-//      a partially loaded child, and at most c_lo + 2^17 - 1) are taken --
+//      a partially loaded child, and at most c_lo + 2^16 - 1) are taken --
 //      every contribution to those columns is in the round;
-//   2. the taken columns set bits of an LDS bitmap over [c_lo, c_lo + 2^17);
+//   2. the taken columns set bits of an LDS bitmap over [c_lo, c_lo + 2^16);
 //      a popcount prefix over the bitmap words ranks every distinct column,
 //      which is its output position in the round;
 //   3. contributions are bucketed by output position (counting sort), each
@@ -37,9 +37,9 @@ namespace brm {
 namespace {
 
 constexpr int BG_NT = 256;
-constexpr int BG_W = 3072;                    // window elements per round
+constexpr int BG_W = 2048;                    // window elements per round
 constexpr int BG_IPT = BG_W / BG_NT;          // window elements per thread
-constexpr int BG_SPAN_LOG2 = 17;
+constexpr int BG_SPAN_LOG2 = 16;
 constexpr int64_t BG_SPAN = 1 << BG_SPAN_LOG2;  // columns per round
 constexpr int BG_WORDS = (int)(BG_SPAN / 64);
 constexpr int BG_WPT = BG_WORDS / BG_NT;      // bitmap words per thread
@@ -49,7 +49,7 @@ constexpr int BG_NSLOT = 8;    // rotating reduction scratch slots
 constexpr int BG_TOPUPS = 1;   // top-up iterations per round
 // Outputs with more contributions than this are summed by a whole wavefront
 // (dense over the list ids) instead of by one thread.
-constexpr int BG_THREAD_M = 16;
+constexpr int BG_THREAD_M = 8;
 constexpr int BG_BIGMAX = BG_W / (BG_THREAD_M + 1) + 1;
 constexpr int BG_MIDMAX = BG_W / 3 + 1;  // outputs with 3 .. BG_THREAD_M contributions
 static_assert(BG_KMAX <= BG_NT, "one child per thread in the window logic");
@@ -88,7 +88,7 @@ struct BgSmem {
     int red[BG_NSLOT][BG_NW];
 };
 
-static_assert(sizeof(BgSmem) <= 80 * 1024, "two workgroups per CU");
+static_assert(sizeof(BgSmem) <= 160 * 1024 / 3, "three workgroups per CU");
 static_assert(offsetof(BgSmem, cnt) % 4 == 0 && BG_W % 2 == 0, "cnt in whole 32-bit words");
 
 // Adds one to counter p of cnt and returns its old value.  LDS atomics are
@@ -416,7 +416,10 @@ __device__ void bg_wave_sum2(const BgSmem& g, int s0, int m0, int s1, int m1, do
                           __builtin_amdgcn_readlane(__double2loint(x), 63));
 }
 
-__global__ __launch_bounds__(BG_NT) void k_bgroup(const GroupTask* __restrict__ tasks,
+// At most 168 VGPRs (three waves per SIMD, launch bounds) and a third of the
+// CU's LDS: three workgroups per CU, so that one workgroup's barrier and
+// memory waits are covered by the other two.
+__global__ __launch_bounds__(BG_NT, 3) void k_bgroup(const GroupTask* __restrict__ tasks,
                                                   const Child* __restrict__ children, RowArgs a, WsCols ws,
                                                   int64_t* __restrict__ slots, int64_t* counters, OutArgs o) {
     extern __shared__ __align__(16) unsigned char smraw[];

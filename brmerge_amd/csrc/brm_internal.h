@@ -21,13 +21,16 @@ enum RowBin : int {
     BIN_TINY = 5,    // ip <= TINY_MAX: two rows per wavefront, merge in registers
     BIN_WMED2 = 6,   // WMED_SPLIT0 < ip <= min(wmed_cap, WMED_SPLIT), L <= WMED_LMAX
     BIN_WMED3 = 7,   // WMED_SPLIT < ip <= wmed_cap, L <= WMED_LMAX
-    BIN_HASH = 8,    // ip <= hash_cap (> 0) and L <= HASH_CAP: Hash-SpGEMM comparator tier (hash.hip)
-    BIN_COUNT = 9
+    BIN_HASH = 8,    // ip <= hash_cap (> 0), ip and L <= HASH_SMALL: Hash-SpGEMM comparator tier (hash.hip)
+    BIN_HASH2 = 9,   // other rows of ip <= hash_cap and L <= HASH_CAP: the same tier with a larger table
+    BIN_COUNT = 10
 };
 
 constexpr int SHORT_MAX = 64;
-// Largest product count of a row in the Hash-SpGEMM tier.
+// Largest product count of a row in the Hash-SpGEMM tier, and of its
+// small-table launch.
 constexpr int HASH_CAP = 1024;
+constexpr int HASH_SMALL = 128;
 constexpr int TINY_MAX = 32;
 
 // Wavefront-sized LDS merges for small medium rows (several rows resident
@@ -120,7 +123,7 @@ constexpr int CNT_N = CNT_PH0 + N_PHASES;
 struct LongArgs {
     const int* rows;        // long rows of this wave
     int nrows;
-    const int64_t* ws_base;  // per row: element offset into workspace buffers
+    const int64_t* ws_base;  // per row: [2] element offsets into workspace buffers 0 and 1
     const int64_t* slot_base;
     const int* arpt;
     const int* acol;
@@ -156,6 +159,11 @@ void dev_free(void* p);
 // its stream (blocks still referenced by queued kernels).
 void dev_defer_free(void* p);
 void dev_flush_deferred();
+// Growable device arrays (virtual address reservation, physical memory mapped
+// as they grow); released by dev_free(base).  vm_reserve returns nullptr when
+// the device or driver does not support it.
+void* vm_reserve(size_t bytes, int dev);
+bool vm_grow(void* base, size_t bytes);  // maps [0, bytes); false on failure
 void dev_trim();                 // release cached blocks to the driver
 size_t dev_cached_bytes();
 void dev_set_fail_after(int64_t n);  // test hook: fail the n-th next allocation (-1 = off)
@@ -182,6 +190,8 @@ struct Params {
     int64_t cbar_budget_bytes; // Upper: C-bar bytes per row wave (0 = auto)
     int64_t sym_chunk_products; // long-row symbolic: products per list chunk (1 .. 2^30, default 2^30)
     int hash_cap;              // rows of at most hash_cap products take the hash tier (0 = off, <= HASH_CAP)
+    int upper_vm;              // Upper waves grow C in reserved address ranges (1) or copy chunks (0)
+    int small_fused;           // products of few short rows in three launches (1) or the general pipeline (0)
 };
 
 // Shapes of the LDS tree-merge kernels, selectable at run time.

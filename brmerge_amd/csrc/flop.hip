@@ -97,7 +97,7 @@ __global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int med
             } else if (force_long) {
                 b_ = BIN_LONG;
             } else if (ip <= hash_cap && L <= HASH_CAP) {
-                b_ = BIN_HASH;
+                b_ = (ip <= HASH_SMALL && L <= HASH_SMALL) ? BIN_HASH : BIN_HASH2;
             } else if (ip <= TINY_MAX) {
                 b_ = BIN_TINY;
             } else if (ip <= SHORT_MAX) {

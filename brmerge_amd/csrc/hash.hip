@@ -15,9 +15,9 @@
 // then sorted by column (sortOutput, :519-552) with a wavefront bitonic sort
 // in LDS and written out.
 //
-// The table has twice the slots of the largest row's product count, so its
-// size and layout (which never change a value) need not follow the
-// reference's MIN_HT_N << (bin - 1).
+// Each row's table has the smallest power of two of slots that is at least
+// twice its product count (at least 64).  Size and layout never change a
+// value, so they need not follow the reference's MIN_HT_N << (bin - 1).
 #include <limits.h>
 
 #include <algorithm>
@@ -28,34 +28,38 @@
 namespace brm {
 namespace {
 
-constexpr int HT_SLOTS = 2 * HASH_CAP;  // table slots (power of two)
 constexpr unsigned HT_NONE = 0xFFFFu;    // no owner / no lane
 constexpr unsigned HASH_SCAL = 107u;     // hash_mult.h:23
 
+// LDS of one row of at most CAP products and CAP lists.
+template <int CAP>
 struct HashSmem {
-    double val[HT_SLOTS];
-    int key[HT_SLOTS];
-    unsigned own[HT_SLOTS];  // bits 0-15: electing lane (HT_NONE: none); bit 16: has a value
-    int loff[HASH_CAP + 1];  // product offset of each list in the row; reused as sort keys
-    int bst[HASH_CAP];       // B row start of each list; reused as sort payloads (slots)
+    static constexpr int SLOTS = 2 * CAP;  // largest table (power of two)
+    double val[SLOTS];
+    int key[SLOTS];
+    unsigned own[SLOTS];  // bits 0-15: electing lane (HT_NONE: none); bit 16: has a value
+    int loff[CAP + 1];    // product offset of each list in the row; reused as sort keys
+    int bst[CAP];         // B row start of each list; reused as sort payloads (slots)
 };
-static_assert(HASH_CAP + 1 >= 1024 + 1, "sort buffers hold up to HASH_CAP entries");
 
 __device__ __forceinline__ void wsync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
 
+template <int CAP>
 __global__ __launch_bounds__(BRM_WAVE) void k_hash_row(const int* __restrict__ rows, int nrows, RowArgs a, OutArgs o) {
     extern __shared__ __align__(16) unsigned char smraw[];
-    HashSmem& g = *reinterpret_cast<HashSmem*>(smraw);
+    HashSmem<CAP>& g = *reinterpret_cast<HashSmem<CAP>*>(smraw);
     const int lane = lane_id();
     const int row = rows[blockIdx.x];
     const int ab = a.arpt[row];
     const int L = a.arpt[row + 1] - ab;
     const int64_t ib = a.ipA[ab];
-    const int S = (int)(a.ipA[ab + L] - ib);  // products, <= HASH_CAP
-    for (int k = lane; k < HT_SLOTS; k += BRM_WAVE) {
+    const int S = (int)(a.ipA[ab + L] - ib);  // products, <= CAP
+    int hs = BRM_WAVE;
+    while (hs < 2 * S) hs <<= 1;  // <= 2 * CAP
+    for (int k = lane; k < hs; k += BRM_WAVE) {
         g.key[k] = -1;
         g.own[k] = HT_NONE;
     }
@@ -75,11 +79,11 @@ __global__ __launch_bounds__(BRM_WAVE) void k_hash_row(const int* __restrict__ r
             const int64_t k = (int64_t)g.bst[hint] + (e - g.loff[hint]);
             col = a.bcol[k];
             v = fmul(a.aval[ab + hint], a.bval[k]);
-            unsigned h = ((unsigned)col * HASH_SCAL) & (unsigned)(HT_SLOTS - 1);
+            unsigned h = ((unsigned)col * HASH_SCAL) & (unsigned)(hs - 1);
             for (;;) {
                 const int old = atomicCAS(&g.key[h], -1, col);
                 if (old == -1 || old == col) break;
-                h = (h + 1) & (unsigned)(HT_SLOTS - 1);
+                h = (h + 1) & (unsigned)(hs - 1);
             }
             slot = (int)h;
         }
@@ -98,7 +102,7 @@ __global__ __launch_bounds__(BRM_WAVE) void k_hash_row(const int* __restrict__ r
     }
     // occupied slots (key, slot), compacted in slot order
     int n = 0;
-    for (int k0 = 0; k0 < HT_SLOTS; k0 += BRM_WAVE) {
+    for (int k0 = 0; k0 < hs; k0 += BRM_WAVE) {
         const int k = k0 + lane;
         const int kk = g.key[k];
         const uint64_t m = __ballot(kk != -1);
@@ -110,7 +114,7 @@ __global__ __launch_bounds__(BRM_WAVE) void k_hash_row(const int* __restrict__ r
         n += __popcll(m);
     }
     wsync();
-    // bitonic sort of n (<= HASH_CAP) keys, padded to a power of two
+    // bitonic sort of n (<= CAP) keys, padded to a power of two
     int np2 = 1;
     while (np2 < n) np2 <<= 1;
     for (int k = n + lane; k < np2; k += BRM_WAVE) {
@@ -158,19 +162,27 @@ __global__ __launch_bounds__(BRM_WAVE) void k_hash_row(const int* __restrict__ r
 
 }  // namespace
 
-hipError_t launch_hash_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
+namespace {
+template <int CAP>
+hipError_t hash_numeric_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
     static SmemLimit lim;
-    if (hipError_t e = lim.raise((const void*)k_hash_row, (int)sizeof(HashSmem)); e != hipSuccess) return e;
+    if (hipError_t e = lim.raise((const void*)k_hash_row<CAP>, (int)sizeof(HashSmem<CAP>)); e != hipSuccess) return e;
     {
         ProfScope ps_("k_hash_row", s);
         const int64_t ch = launch_chunk(BRM_WAVE);
         for (int64_t b = 0; b < n; b += ch) {
             const int c = (int)std::min<int64_t>(ch, n - b);
-            k_hash_row<<<c, BRM_WAVE, sizeof(HashSmem), s>>>(rows + b, c, a, o);
+            k_hash_row<CAP><<<c, BRM_WAVE, sizeof(HashSmem<CAP>), s>>>(rows + b, c, a, o);
         }
     }
     return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_hash_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s, int cap) {
+    if (n <= 0) return hipSuccess;
+    // a small table keeps many rows resident per CU
+    return cap <= HASH_SMALL ? hash_numeric_t<HASH_SMALL>(rows, n, a, o, s) : hash_numeric_t<HASH_CAP>(rows, n, a, o, s);
 }
 
 }  // namespace brm

@@ -71,10 +71,31 @@ hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const O
 // products while 32-bit sort keys hold N columns, else 512.
 int rmed_auto_cap(int N);
 
+// small.hip: products of at most SMALL_MAX_ROWS rows, all in the short tier.
+constexpr int SMALL_MAX_ROWS = 1 << 16;
+constexpr int SMALL_SLOTS = 32;  // partial-sum counters (summed by the host)
+struct SmallState {
+    int err;     // DevErr bits
+    int miss;    // a row has more than 64 lists or products: the general pipeline runs instead
+    int ticket;  // k_small_emit workgroup order
+    int pad;
+    int64_t nnz[SMALL_SLOTS];  // nnz(C), in partial sums
+    int64_t ip[SMALL_SLOTS];   // intermediate products, in partial sums
+};
+// int64 words of the state block (SmallState, then the look-back states)
+int64_t small_state_words(int M);
+// clears the state block, then merges every row into 64-slot stash rows
+hipError_t launch_small(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* scol, double* sval, int64_t* rowcnt,
+                        SmallState* st, hipStream_t s);
+// C's row pointer (single-pass scan) and C from the stash
+hipError_t launch_small_emit(int M, const int64_t* rowcnt, const int* scol, const double* sval, int64_t* crpt,
+                             int* ccol, double* cval, SmallState* st, hipStream_t s);
+
 // hash.hip (rows of at most HASH_CAP products)
-hipError_t launch_hash_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s);
+// cap: the largest product count of the listed rows (<= HASH_SMALL picks the small table)
+hipError_t launch_hash_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s, int cap);
 // lds.hip: exact row counts of hash-tier rows
-hipError_t launch_hash_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s);
+hipError_t launch_hash_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s, int cap);
 
 // bmerge.hip
 hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,

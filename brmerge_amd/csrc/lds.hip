@@ -413,14 +413,17 @@ hipError_t launch_wmed_numeric(const int* rows, int n, const RowArgs& a, const O
     return medium_numeric_t<BRM_WAVE, WMED_CAP, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");
 }
 
-hipError_t launch_hash_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {
+hipError_t launch_hash_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s, int cap) {
     if (n <= 0) return hipSuccess;
     {
         ProfScope ps_("k_sym_hash_h", s);
         const int64_t ch = launch_chunk(BRM_WAVE);
         for (int64_t b = 0; b < n; b += ch) {
             const int c = (int)std::min<int64_t>(ch, n - b);
-            k_sym_hash<BRM_WAVE, 2 * HASH_CAP, HASH_CAP><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+            if (cap <= HASH_SMALL)
+                k_sym_hash<BRM_WAVE, 2 * HASH_SMALL, HASH_SMALL><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+            else
+                k_sym_hash<BRM_WAVE, 2 * HASH_CAP, HASH_CAP><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
         }
     }
     return hipGetLastError();

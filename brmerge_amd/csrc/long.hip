@@ -43,6 +43,13 @@ __device__ __forceinline__ int64_t heap_idx(int H, int s, int h) {
 }
 __device__ __forceinline__ int pass_height(int j, int H, int pb) { return min(j * pb, H); }
 
+// A row's element offsets into workspace buffers 0 and 1 (each buffer holds
+// only the rows that store nodes in it).
+struct WsBase {
+    int64_t b0, b1;
+    __device__ int64_t at(int buf) const { return buf ? b1 : b0; }
+};
+
 struct NodeK {
     int kind;
     int q;
@@ -80,7 +87,7 @@ __device__ int frontier_height(const int64_t* ipr, int L, int H, int p, const Fi
 }
 
 __device__ Src make_src(const LongArgs& la, int a0, const int64_t* ipr, int L, int H, int s, int h, int buf,
-                        int64_t wsb, int64_t slb) {
+                        WsBase wsb, int64_t slb) {
     const NodeK nk = node_kind(ipr, L, s, h);
     Src r;
     r.kind = nk.kind;
@@ -96,7 +103,7 @@ __device__ Src make_src(const LongArgs& la, int a0, const int64_t* ipr, int L, i
         r.scale = la.aval[ai];
     } else if (nk.kind == SRC_STORED) {
         r.buf = buf;
-        r.off = wsb + (ipr[s] - ipr[0]);
+        r.off = wsb.at(buf) + (ipr[s] - ipr[0]);
         r.slot = slb + heap_idx(H, s, h);
     }
     return r;
@@ -104,7 +111,7 @@ __device__ Src make_src(const LongArgs& la, int a0, const int64_t* ipr, int L, i
 
 // Emits the group task of node (s, hj), the pass-j node (or the root).
 __device__ void emit_group(const LongArgs& la, int row, int a0, const int64_t* ipr, int L, int H, int j, int s,
-                           int hj, int hprev, const Fit& cap, int64_t wsb, int64_t slb, int64_t final_off, int* err) {
+                           int hj, int hprev, const Fit& cap, WsBase wsb, int64_t slb, int64_t final_off, int* err) {
     const bool root = (hj == H);
     const NodeK u = node_kind(ipr, L, s, hj);
     if (u.kind == SRC_EMPTY || (u.kind == SRC_VIRTUAL && !root)) return;
@@ -151,7 +158,7 @@ __device__ void emit_group(const LongArgs& la, int row, int a0, const int64_t* i
     } else {
         T.final_dst = 0;
         T.dst_buf = j & 1;
-        T.dst_off = wsb + (ipr[s] - ipr[0]);
+        T.dst_off = wsb.at(j & 1) + (ipr[s] - ipr[0]);
         T.slot = slb + heap_idx(H, s, hj);
     }
     const unsigned long long idx = atomicAdd((unsigned long long*)&la.counters[CNT_TASK0 + j], 1ull);
@@ -168,7 +175,7 @@ __device__ void emit_group(const LongArgs& la, int row, int a0, const int64_t* i
 // emits each node's children together, thread t taking child slot t, with
 // a block scan for their order.
 __device__ void plan_no_frontier(const LongArgs& la, int row, int a0, const int64_t* ipr, int L, int H, int J,
-                                 int64_t wsb, int64_t slb, int64_t final_off, int* err) {
+                                 WsBase wsb, int64_t slb, int64_t final_off, int* err) {
     __shared__ int64_t sm[PL_NT / BRM_WAVE + 1];
     __shared__ int64_t cbase;
     const int t = threadIdx.x;
@@ -220,7 +227,7 @@ __device__ void plan_no_frontier(const LongArgs& la, int row, int a0, const int6
                 } else {
                     T.final_dst = 0;
                     T.dst_buf = j & 1;
-                    T.dst_off = wsb + (ipr[s] - ipr[0]);
+                    T.dst_off = wsb.at(j & 1) + (ipr[s] - ipr[0]);
                     T.slot = slb + heap_idx(H, s, hj);
                 }
                 const unsigned long long idx =
@@ -242,7 +249,8 @@ __global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int r0, int* err) {
     const int H = ceil_log2i(L);
     const int pb = la.pass_bits;
     const int J = H == 0 ? 1 : (H + pb - 1) / pb;
-    const int64_t wsb = la.ws_base[r], slb = la.slot_base[r];
+    const WsBase wsb{la.ws_base[2 * (int64_t)r], la.ws_base[2 * (int64_t)r + 1]};
+    const int64_t slb = la.slot_base[r];
     const int64_t final_off = la.final_off ? la.final_off[row] : ipr[0];
     if (la.frontier_cap == 0 && (1 << pb) <= PL_NT) {
         plan_no_frontier(la, row, a0, ipr, L, H, J, wsb, slb, final_off, err);
@@ -287,7 +295,7 @@ __global__ __launch_bounds__(PL_NT) void k_plan(LongArgs la, int r0, int* err) {
             lt.buf = (jf - 1) & 1;
             lt.final_dst = 0;
             lt.pad = 0;
-            lt.dst_off = wsb + (ipr[s] - ipr[0]);
+            lt.dst_off = wsb.at(lt.buf) + (ipr[s] - ipr[0]);
             lt.slot = slb + heap_idx(H, s, hf);
             la.leaves[li] = lt;
         }

@@ -1,41 +1,22 @@
 // Short-row tier: rows with at most 64 intermediate products, one
 // wavefront per row (two per wavefront, 32 lanes each, for rows of at most
-// 32 products), everything in registers.
-//
-// Each lane holds one product (column, list position, value).  A rank sort
-// by (column, position) over the row's lanes (v_readlane broadcasts, or
-// shuffles inside a half-wavefront) places the
-// products in column order with equal columns adjacent and ordered by
-// position; ds_permute moves every lane's product to its rank.  The merge tree
-// of BRMerge (BRMerge-SpGEMM/inc/spgemm.h:639-707: lists paired (2j, 2j+1)
-// level by level, odd list carried) is then evaluated column-wise: at level l
-// two live entries of the same column combine iff their positions share the
-// parent node (pos >> (l+1)); the left one keeps the sum, the right one dies.
-// That is exactly the reference's value rule -- an aligned binary tree over
-// list positions in which a node with one present child passes it through --
-// so the sums are bit-identical.  The symbolic variant counts distinct
-// columns (first occurrence of each column in the rank order).
+// 32 products), everything in registers.  Each lane holds one product
+// (column, list position, value); short_merge.h ranks them by (column,
+// position) and evaluates BRMerge's tree column-wise.  The symbolic variant
+// counts distinct columns (first occurrence of each column in the rank order).
 #include <limits.h>
 
 #include <algorithm>
 
 #include "brm_device.h"
 #include "kernels.h"
+#include "short_merge.h"
 
 namespace brm {
 namespace {
 
 constexpr int SH_NT = 256;
 constexpr int SH_WPB = SH_NT / BRM_WAVE;
-
-__device__ __forceinline__ int permute_i32(int dst_lane, int x) {
-    return __builtin_amdgcn_ds_permute(dst_lane << 2, x);
-}
-__device__ __forceinline__ double permute_f64(int dst_lane, double x) {
-    int lo = __builtin_amdgcn_ds_permute(dst_lane << 2, __double2loint(x));
-    int hi = __builtin_amdgcn_ds_permute(dst_lane << 2, __double2hiint(x));
-    return __hiloint2double(hi, lo);
-}
 
 // G = 64: one row per wavefront; G = 32: two rows per wavefront, each in its
 // own half (every shuffle, ballot mask and rank stays inside the half).
@@ -99,70 +80,15 @@ __global__ __launch_bounds__(SH_NT) void k_short(const int* __restrict__ rows, i
         col = a.bcol[bi];
         if (NUMERIC) val = fmul(av, a.bval[bi]);
     }
-    const uint64_t key = act ? (((uint64_t)(uint32_t)col << 32) | (uint32_t)p) : ~0ull;
-    const uint32_t klo_me = (uint32_t)key, khi_me = (uint32_t)(key >> 32);
-
-    // rank by (column, position) inside the group; inactive lanes hold the
-    // largest key, so they are never counted
-    int rank = 0;
-    bool dup = false;
-    if (G == BRM_WAVE) {
-        const int n = __builtin_amdgcn_readfirstlane(ip);
-        for (int k = 0; k < n; ++k) {
-            const uint32_t klo = (uint32_t)__builtin_amdgcn_readlane((int)klo_me, k);
-            const uint32_t khi = (uint32_t)__builtin_amdgcn_readlane((int)khi_me, k);
-            const uint64_t kk = ((uint64_t)khi << 32) | klo;
-            const bool lt = kk < key;
-            rank += lt;
-            dup |= lt && (khi == khi_me);
-        }
-    } else {
-#pragma unroll 8
-        for (int k = 0; k < G; ++k) {
-            const uint32_t klo = (uint32_t)__shfl((int)klo_me, gbase + k, BRM_WAVE);
-            const uint32_t khi = (uint32_t)__shfl((int)khi_me, gbase + k, BRM_WAVE);
-            const uint64_t kk = ((uint64_t)khi << 32) | klo;
-            const bool lt = kk < key;
-            rank += lt;
-            dup |= lt && (khi == khi_me);
-        }
-    }
-
+    const ShortMerged m = short_merge<NUMERIC, G>(col, p, val, act, ip, gl, gbase, gmask);
     if (!NUMERIC) {
-        const uint64_t first = __ballot(act && !dup) & gmask;
-        if (has && gl == 0) nnzrow[row] = (int64_t)__popcll(first);
+        if (has && gl == 0) nnzrow[row] = (int64_t)m.nout;
         return;
     }
-
-    const int dst = act ? gbase + rank : lane;
-    col = permute_i32(dst, col);
-    int pos = permute_i32(dst, p);
-    val = permute_f64(dst, val);
-    bool alive = gl < ip;
-
-    const uint64_t below_me = (1ull << lane) - 1ull;
-    const uint64_t above_me = lane == BRM_WAVE - 1 ? 0ull : ~0ull << (lane + 1);
-    for (int l = 0; l < 31; ++l) {
-        const uint64_t am = __ballot(alive) & gmask;
-        const uint64_t above = am & above_me;
-        const int nxt = above ? (__ffsll((long long)above) - 1) : lane;
-        const int ncol = __shfl(col, nxt, BRM_WAVE);
-        const int npos = __shfl(pos, nxt, BRM_WAVE);
-        const double nval = __shfl(val, nxt, BRM_WAVE);
-        const bool same = alive && above != 0ull && ncol == col;
-        if (__ballot(same) == 0ull) break;  // no column left to combine in any group
-        const bool comb = same && ((npos >> (l + 1)) == (pos >> (l + 1)));
-        const uint64_t cm = __ballot(comb);
-        const uint64_t below = am & below_me;
-        const int prv = below ? (63 - __clzll((long long)below)) : -1;
-        const bool dies = prv >= 0 && ((cm >> prv) & 1ull);
-        if (comb) val = fadd(val, nval);
-        alive = alive && !dies;
-    }
-
-    const uint64_t am = __ballot(alive) & gmask;
-    const int idx = __popcll(am & below_me);
-    const int nout = __popcll(am);
+    const bool alive = m.alive;
+    const int idx = m.idx, nout = m.nout;
+    col = m.col;
+    val = m.val;
     if (!has) return;
     if (!o.upper) {
         const int64_t cb = o.crpt[row];

@@ -157,7 +157,7 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
 /* Tuning knobs (names: "ws_budget_bytes", "med_cap", "leaf_cap",
  * "force_long", "profile", "lds_cfg", "group_cfg", "wmed_cap",
  * "wmed_engine", "launch_chunk", "cbar_budget_bytes", "sym_chunk_products",
- * "hash_cap");
+ * "hash_cap", "upper_vm");
  * returns BRM_E_INVALID for unknown names or
  * values.  Knobs other than hash_cap never change results, only speed.  group_cfg 0-3:
  * merge-path engine shapes for long rows; 4 (default): the bitmap-rank
@@ -171,7 +171,9 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
  * tier instead of BRMerge's (the comparison baseline of the reference's
  * Existing-SpGEMM/inc/hash_mult.h:558-608: each column's contributions added
  * in list order); this knob, alone among them, changes value bits (within
- * 1e-12 of BRMerge's), never the structure. */
+ * 1e-12 of BRMerge's), never the structure.  upper_vm (1, the default):
+ * BRMerge-Upper's row waves compact into C grown in a reserved address range;
+ * 0 (or a driver without address reservation): into chunks copied into C. */
 int brm_set_param(const char *name, int64_t value);
 int64_t brm_get_param(const char *name);
 

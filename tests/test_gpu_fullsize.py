@@ -55,7 +55,7 @@ def assert_panel_equal(got, R):
 
 @pytest.fixture
 def knobs(gpu):
-    names = ("ws_budget_bytes", "cbar_budget_bytes", "group_cfg", "force_long", "sym_chunk_products")
+    names = ("ws_budget_bytes", "cbar_budget_bytes", "group_cfg", "force_long", "sym_chunk_products", "upper_vm")
     saved = {k: gpu.get_param(k) for k in names}
     yield gpu
     for k, v in saved.items():
@@ -110,12 +110,14 @@ def test_amg96_galerkin_whole(gpu):
     assert_csr_equal(gpu.brmerge_upper(PT, AP), R2)
 
 
-@pytest.mark.parametrize("engine", [0, 4], ids=["mergepath", "bitmap"])
-def test_rmat15_many_waves(knobs, engine):
+@pytest.mark.parametrize("engine,vm", [(0, 1), (4, 1), (4, 0)], ids=["mergepath", "bitmap", "bitmap-chunks"])
+def test_rmat15_many_waves(knobs, engine, vm):
     """Long rows over at least three workspace waves and Upper over at least
-    three C-bar waves, on both long-row engines."""
+    three C-bar waves, on both long-row engines; Upper's C grown in a
+    reserved address range, or assembled from copied chunks."""
     gpu = knobs
     gpu.set_param("group_cfg", engine)
+    gpu.set_param("upper_vm", vm)
     A = gpu.gen.rmat(15)
     ip = row_ip(A, A)
     R = oracle.brmerge_precise(as_tuple(A), as_tuple(A))

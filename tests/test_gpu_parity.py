@@ -30,8 +30,9 @@ TIER_SETTINGS = {
     "register_medium_512": {"wmed_cap": 512, "wmed_engine": 1, "med_cap": 512},
     "register_medium_1024": {"wmed_cap": 1024, "wmed_engine": 1},
     "lds_wave_medium": {"wmed_cap": 512, "wmed_engine": 0},
+    "general_pipeline": {"small_fused": 0},
 }
-KNOBS = ("force_long", "leaf_cap", "med_cap", "wmed_cap", "group_cfg", "launch_chunk", "wmed_engine")
+KNOBS = ("force_long", "leaf_cap", "med_cap", "wmed_cap", "group_cfg", "launch_chunk", "wmed_engine", "small_fused")
 
 
 @pytest.fixture
