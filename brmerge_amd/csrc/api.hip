@@ -831,29 +831,27 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
 }  // namespace
 
 namespace {
-// Products of at most SMALL_MAX_ROWS rows whose rows all have at most 64
-// lists and products: three launches and one host round trip (small.hip).
-// C is allocated for 64 entries per row (a bound of nnz(C) when every row is
-// short), so nothing waits on the device before the end.  *handled = false
-// when a row is longer (nothing is kept; the general pipeline runs).
-int run_small(const DevIn& in, int64_t max_nnz, hipStream_t s, PhaseClock& clk, DevOut* out, bool* handled) {
+// Products of at most SMALL_MAX_ROWS rows whose rows all have at most `group`
+// (32 or 64) lists and products: one fill, one kernel and one host round trip
+// (small.hip).  C is allocated for `group` entries per row (a bound of nnz(C)
+// when every row is short), so nothing waits on the device before the end.
+// *handled = false when a row is longer (nothing is kept).
+int run_small_g(const DevIn& in, int group, int64_t max_nnz, hipStream_t s, PhaseClock& clk, DevOut* out,
+                bool* handled) {
     *handled = false;
-    if (in.nnzA > INT32_MAX || in.nnzB > INT32_MAX) return BRM_OK;
     const int M = in.M;
-    const int64_t cap = (int64_t)M * 64;
+    const int64_t cap = (int64_t)M * group;
+    if (cap > max_nnz) return BRM_OK;
     Scratch sc;
     SmallState* st;
-    int64_t *rowcnt, *crpt;
-    int *scol, *ccol;
-    double *sval, *cval;
+    int64_t* crpt;
+    int* ccol;
+    double* cval;
     clk.begin(P_PRE);
-    BRMC(sc.alloc(reinterpret_cast<int64_t**>(&st), small_state_words(M), "small-product state"));
-    BRMC(sc.alloc(&rowcnt, M, "row nnz"));
-    BRMC(sc.alloc(&scol, cap, "short-row stash"));
-    BRMC(sc.alloc(&sval, cap, "short-row stash"));
+    BRMC(sc.alloc(reinterpret_cast<int64_t**>(&st), small_state_words(M, group), "small-product state"));
     BRMC(sc.alloc(&crpt, (int64_t)M + 1, "crpt"));
-    BRMC(sc.alloc(&ccol, std::min(cap, max_nnz), "C col"));
-    BRMC(sc.alloc(&cval, std::min(cap, max_nnz), "C val"));
+    BRMC(sc.alloc(&ccol, cap, "C col"));
+    BRMC(sc.alloc(&cval, cap, "C val"));
     RowArgs a{};
     a.arpt = in.arpt;
     a.acol = in.acol;
@@ -864,10 +862,8 @@ int run_small(const DevIn& in, int64_t max_nnz, hipStream_t s, PhaseClock& clk, 
     a.M = M;
     a.K = in.K;
     a.N = in.N;
-    clk.begin(P_SYM);
-    HIPC(launch_small(a, in.nnzA, in.nnzB, scol, sval, rowcnt, st, s));
     clk.begin(P_COMPUTE);
-    HIPC(launch_small_emit(M, rowcnt, scol, sval, crpt, ccol, cval, st, s));
+    HIPC(launch_small(a, in.nnzA, in.nnzB, crpt, ccol, cval, st, group, s));
     SmallState h{};
     BRMC(d2h(&h, st, sizeof(h), s));
     if (h.err & DERR_INVALID) {
@@ -880,10 +876,6 @@ int run_small(const DevIn& in, int64_t max_nnz, hipStream_t s, PhaseClock& clk, 
     for (int i = 0; i < SMALL_SLOTS; ++i) {
         nnz += h.nnz[i];
         ipt += h.ip[i];
-    }
-    if (nnz > max_nnz) {
-        set_err("nnz(C) = %lld exceeds the int32 row pointer", (long long)nnz);
-        return BRM_E_OVERFLOW;
     }
     sc.release(crpt);
     sc.release(ccol);
@@ -898,6 +890,19 @@ int run_small(const DevIn& in, int64_t max_nnz, hipStream_t s, PhaseClock& clk, 
     ts.rows_by_tier[BIN_SHORT] = M;
     *handled = true;
     return BRM_OK;
+}
+
+// Groups of 32 lanes (two rows per wavefront) when the average row has at
+// most 32 products, else 64; a row too long for 32 lanes retries at 64.
+int run_small(const DevIn& in, double avg_ip, int64_t max_nnz, hipStream_t s, PhaseClock& clk, DevOut* out,
+              bool* handled) {
+    *handled = false;
+    if (in.nnzA > INT32_MAX || in.nnzB > INT32_MAX) return BRM_OK;
+    if (avg_ip <= 32.0) {
+        BRMC(run_small_g(in, 32, max_nnz, s, clk, out, handled));
+        if (*handled) return BRM_OK;
+    }
+    return run_small_g(in, 64, max_nnz, s, clk, out, handled);
 }
 }  // namespace
 
@@ -919,7 +924,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     if (!upper && !ext && params().small_fused && in.M > 0 && in.M <= SMALL_MAX_ROWS && !params().force_long &&
         params().hash_cap == 0 && avg_l * avg_b <= 64.0) {
         bool handled = false;
-        BRMC(run_small(in, max_nnz, s, clk, out, &handled));
+        BRMC(run_small(in, avg_l * avg_b, max_nnz, s, clk, out, &handled));
         if (handled) {
             clk.begin(P_CLEAN);
             HIPC(hipStreamSynchronize(s));

@@ -72,24 +72,23 @@ hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const O
 int rmed_auto_cap(int N);
 
 // small.hip: products of at most SMALL_MAX_ROWS rows, all in the short tier.
-constexpr int SMALL_MAX_ROWS = 1 << 16;
+constexpr int SMALL_MAX_ROWS = 1 << 20;
 constexpr int SMALL_SLOTS = 32;  // partial-sum counters (summed by the host)
 struct SmallState {
     int err;     // DevErr bits
-    int miss;    // a row has more than 64 lists or products: the general pipeline runs instead
-    int ticket;  // k_small_emit workgroup order
+    int miss;    // a row has more than G lists or products: nothing of C is usable
+    int ticket;  // workgroup order of k_small
     int pad;
     int64_t nnz[SMALL_SLOTS];  // nnz(C), in partial sums
     int64_t ip[SMALL_SLOTS];   // intermediate products, in partial sums
 };
 // int64 words of the state block (SmallState, then the look-back states)
-int64_t small_state_words(int M);
-// clears the state block, then merges every row into 64-slot stash rows
-hipError_t launch_small(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* scol, double* sval, int64_t* rowcnt,
-                        SmallState* st, hipStream_t s);
-// C's row pointer (single-pass scan) and C from the stash
-hipError_t launch_small_emit(int M, const int64_t* rowcnt, const int* scol, const double* sval, int64_t* crpt,
-                             int* ccol, double* cval, SmallState* st, hipStream_t s);
+// for M rows in groups of `group` (32 or 64) lanes
+int64_t small_state_words(int M, int group);
+// Clears the state block, then merges every row and writes C's row pointer
+// and C (capacity group * M entries).
+hipError_t launch_small(const RowArgs& a, int64_t nnzA, int64_t nnzB, int64_t* crpt, int* ccol, double* cval,
+                        SmallState* st, int group, hipStream_t s);
 
 // hash.hip (rows of at most HASH_CAP products)
 // cap: the largest product count of the listed rows (<= HASH_SMALL picks the small table)
