@@ -898,6 +898,7 @@ int run_small(const DevIn& in, double avg_ip, int64_t max_nnz, hipStream_t s, Ph
               bool* handled) {
     *handled = false;
     if (in.nnzA > INT32_MAX || in.nnzB > INT32_MAX) return BRM_OK;
+    if (in.M < 1 || in.K < 1 || in.nnzA < 1 || in.nnzB < 1) return BRM_OK;  // C is empty: the general pipeline
     if (avg_ip <= 32.0) {
         BRMC(run_small_g(in, 32, max_nnz, s, clk, out, handled));
         if (*handled) return BRM_OK;

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(SH_NT) void k_short(const int* __restrict__ rows, i
         col = a.bcol[bi];
         if (NUMERIC) val = fmul(av, a.bval[bi]);
     }
-    const ShortMerged m = short_merge<NUMERIC, G>(col, p, val, act, ip, gl, gbase, gmask);
+    const ShortMerged m = short_merge<NUMERIC, G>(col, p, val, act, ip, gl, gbase, gmask, short_keys_fit<G>(a.N));
     if (!NUMERIC) {
         if (has && gl == 0) nnzrow[row] = (int64_t)m.nout;
         return;

@@ -103,3 +103,46 @@ def test_fused_invalid_b_row(gpu):
         bm.brmerge_precise(csr_from_tuple(A), csr_from_tuple(Bbad))
     assert e.value.status == 1
     _check(bm, A, B)
+
+
+@pytest.mark.parametrize("used", [False, True], ids=["unused", "used"])
+def test_long_b_row(gpu, used):
+    """A B row of 100 entries (longer than the fused kernel checks) sends
+    the product to the general pipeline, whether an A row uses it or not."""
+    rng = np.random.default_rng(11)
+    K, N = 500, 800
+    lens = np.ones(K, np.int64)
+    lens[K - 1] = 100
+    brpt = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    bcol = np.concatenate([np.sort(rng.choice(N, size=n, replace=False)) for n in lens]).astype(np.int32)
+    B = (brpt, bcol, rng.uniform(-1, 1, len(bcol)), N)
+    A, _ = _rows_with(rng, K, N, rng.integers(1, 8, size=2000), kuse=K - 1)
+    if used:  # the last entry of A's row 5 becomes column K - 1 (no other row uses it)
+        rpt, col = A[0], A[1]
+        col[rpt[6] - 1] = K - 1
+    _check(gpu, A, B)
+
+
+@pytest.mark.parametrize("N", [(1 << 26) - 1, 1 << 26, (1 << 27) - 1, 1 << 27, (1 << 31) - 1])
+@pytest.mark.parametrize("shape", ["g32", "g64"])
+def test_wide_columns(gpu, N, shape):
+    """Columns up to N - 1 around the widths where the register sort
+    switches from 32-bit (column, lane) keys to 64-bit (column, position)
+    keys: 2^27 for 32-lane groups, 2^26 for 64-lane groups.  Both the fused
+    path and the general pipeline's short tier are checked."""
+    rng = np.random.default_rng(N % 1000 + (shape == "g64"))
+    K = 3000
+    if shape == "g32":
+        lens_a, per_b = rng.integers(4, 21, size=2000), 1
+    else:
+        lens_a, per_b = np.full(2000, 8), 8
+    rpt = np.concatenate([[0], np.cumsum(lens_a)]).astype(np.int32)
+    col = np.concatenate([np.sort(rng.choice(K, size=n, replace=False)) for n in lens_a]).astype(np.int32)
+    A = (rpt, col, rng.uniform(-1, 1, len(col)), K)
+    # B columns from a small pool near N - 1 (so that columns repeat across
+    # lists and merge), N - 1 itself included
+    pool = np.unique(np.concatenate([[N - 1], N - 1 - rng.integers(0, 200, size=60)]))
+    brows = [np.sort(rng.choice(pool, size=per_b, replace=False)) for _ in range(K)]
+    brpt = np.concatenate([[0], np.cumsum([len(r) for r in brows])]).astype(np.int32)
+    B = (brpt, np.concatenate(brows).astype(np.int32), rng.uniform(-1, 1, K * per_b), N)
+    _check(gpu, A, B)
