@@ -137,11 +137,41 @@ void PhaseClock::collect(double* out) {
     }
 }
 
-int d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
-    HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+// Small read-backs go through a pinned buffer of the calling thread (a copy
+// into pageable memory is staged by the runtime and takes longer), several
+// of them behind one synchronisation.  The buffer is kept until the process
+// ends.
+constexpr size_t D2H_PINNED = 4096;
+
+int d2h_parts(std::initializer_list<D2HPart> parts, hipStream_t s) {
+    static thread_local unsigned char* pinned = nullptr;
+    size_t tot = 0;
+    for (const D2HPart& q : parts) tot += (q.bytes + 15) & ~(size_t)15;
+    if (tot <= D2H_PINNED && !pinned) {
+        void* ptr = nullptr;
+        if (hipHostMalloc(&ptr, D2H_PINNED, hipHostMallocDefault) == hipSuccess) pinned = (unsigned char*)ptr;
+        else (void)hipGetLastError();
+    }
+    if (tot > D2H_PINNED || !pinned) {
+        for (const D2HPart& q : parts) HIPC(hipMemcpyAsync(q.dst, q.src, q.bytes, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        return BRM_OK;
+    }
+    size_t o = 0;
+    for (const D2HPart& q : parts) {
+        HIPC(hipMemcpyAsync(pinned + o, q.src, q.bytes, hipMemcpyDeviceToHost, s));
+        o += (q.bytes + 15) & ~(size_t)15;
+    }
     HIPC(hipStreamSynchronize(s));
+    o = 0;
+    for (const D2HPart& q : parts) {
+        memcpy(q.dst, pinned + o, q.bytes);
+        o += (q.bytes + 15) & ~(size_t)15;
+    }
     return BRM_OK;
 }
+
+int d2h(void* dst, const void* src, size_t bytes, hipStream_t s) { return d2h_parts({{dst, src, bytes}}, s); }
 
 int check_dev_err(const int* derr, hipStream_t s, const char* stage) {
     int e = 0;
@@ -507,10 +537,9 @@ int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p)
     HIPC(bin_launch(p, params().lds_cfg, std::min(params().med_cap, shape.cap), params().wmed_cap,
                     params().force_long, s));
     int e = 0;
-    HIPC(hipMemcpyAsync(&e, p.derr, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPC(hipMemcpyAsync(&p.total_ip, p.ipA + in.nnzA, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    HIPC(hipMemcpyAsync(p.bc, p.bin_cnt, sizeof(p.bc), hipMemcpyDeviceToHost, s));
-    HIPC(hipStreamSynchronize(s));
+    BRMC(d2h_parts({{&e, p.derr, sizeof(int)}, {&p.total_ip, p.ipA + in.nnzA, sizeof(int64_t)},
+                    {p.bc, p.bin_cnt, sizeof(p.bc)}},
+                   s));
     return dev_err_status(e, "validation");
 }
 
@@ -985,9 +1014,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         tr.mark("long numeric");
         if (small) {
             int e = 0;
-            HIPC(hipMemcpyAsync(&e, p.derr, sizeof(int), hipMemcpyDeviceToHost, s));
-            HIPC(hipMemcpyAsync(&nnzC, p.crpt + M, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-            HIPC(hipStreamSynchronize(s));
+            BRMC(d2h_parts({{&e, p.derr, sizeof(int)}, {&nnzC, p.crpt + M, sizeof(int64_t)}}, s));
             BRMC(dev_err_status(e, "numeric"));
         } else {
             BRMC(check_dev_err(p.derr, s, "numeric"));

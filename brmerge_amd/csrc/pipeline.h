@@ -2,6 +2,7 @@
 // the batched and planned entries (batch.hip) and the request server
 // (serve.hip).
 #pragma once
+#include <initializer_list>
 #include <algorithm>
 #include <string>
 #include <vector>
@@ -14,6 +15,13 @@ void set_err(const char* fmt, ...);
 int hip_status(hipError_t e, const char* what);
 int use_device();
 int d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
+// several device-to-host copies behind one synchronisation
+struct D2HPart {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+int d2h_parts(std::initializer_list<D2HPart> parts, hipStream_t s);
 int check_dev_err(const int* derr, hipStream_t s, const char* stage);
 int dev_err_status(int err_flags, const char* stage);
 double now_s();

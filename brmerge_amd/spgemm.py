@@ -187,11 +187,12 @@ def spgemm_dev(dA, dB, upper=False, stream=None, timing=None):
     L = _lib.lib()
     r, c, v = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
     nnz = ctypes.c_int64()
-    tc = _lib.UpperTimingC() if upper else _lib.PreciseTimingC()
+    # phase timing (hipEvents around each phase) only when asked for
+    tc = (_lib.UpperTimingC() if upper else _lib.PreciseTimingC()) if timing is not None else None
     fn = L.brm_spgemm_upper_dev if upper else L.brm_spgemm_precise_dev
     st = fn(dA.rpt.ptr, dA.col.ptr, dA.val.ptr, dA.nnz, dB.rpt.ptr, dB.col.ptr, dB.val.ptr, dB.nnz,
             dA.M, dA.N, dB.N, ctypes.byref(r), ctypes.byref(c), ctypes.byref(v), ctypes.byref(nnz),
-            ctypes.cast(ctypes.pointer(tc), ctypes.c_void_p), stream)
+            ctypes.cast(ctypes.pointer(tc), ctypes.c_void_p) if tc is not None else None, stream)
     _lib.check(st)
     if timing is not None:
         timing._load(tc)
