@@ -10,7 +10,7 @@ from .spgemm import (  # noqa: F401
     brmerge_precise, brmerge_upper, compute_flop, DeviceArray, DeviceCSR, spgemm_dev,
     set_param, get_param, prof_reset, prof_report, last_stats,
 )
-from .batch import spgemm_batched, split_products, BatchPlan  # noqa: F401
+from .batch import spgemm_batched, split_products, BatchPlan, set_tune_cache  # noqa: F401
 from .serve import Server  # noqa: F401
 from . import gen  # noqa: F401
 

@@ -71,7 +71,7 @@ EXPORTED = [
     "brm_memcpy_h2d", "brm_memcpy_d2h", "brm_prof_reset", "brm_prof_report", "brm_last_stats", "brm_bench_copy",
     "brm_checksum_dev", "brm_spgemm_precise_dev_noalloc", "brm_noalloc_scratch_bytes",
     "brm_spgemm_batched_dev", "brm_batch_plan_create", "brm_batch_plan_execute", "brm_batch_plan_result",
-    "brm_batch_plan_info", "brm_batch_plan_destroy",
+    "brm_batch_plan_info", "brm_batch_plan_destroy", "brm_set_tune_cache",
     "brm_server_create", "brm_server_add_b", "brm_server_submit", "brm_server_poll", "brm_server_wait",
     "brm_server_release", "brm_server_stats", "brm_server_destroy",
     "brm_gen_er", "brm_gen_er_var", "brm_gen_rmat", "brm_gen_stencil27", "brm_gen_aggregation",
@@ -134,6 +134,7 @@ def lib():
         "brm_batch_plan_result": ([P, PP, PP, PP, ctypes.POINTER(I64), PP], I),
         "brm_batch_plan_info": ([P, ctypes.c_char_p, U64], I),
         "brm_batch_plan_destroy": ([P], None),
+        "brm_set_tune_cache": ([ctypes.c_char_p], I),
         "brm_server_create": ([I64, I, I, PP], I),
         "brm_server_add_b": ([P, P, P, P, I64, I, I, ctypes.POINTER(I)], I),
         "brm_server_submit": ([P, I, P, P, P, I64, I, ctypes.POINTER(I64)], I),

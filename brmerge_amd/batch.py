@@ -27,6 +27,13 @@ def _pairs_array(pairs):
     return arr
 
 
+def set_tune_cache(path):
+    """Name the JSON file of tuned tier shapes that BatchPlan(autotune=True)
+    reads (a hit skips the tuner) and records its choices in; None turns
+    it off.  See brm_set_tune_cache in include/brmerge.h."""
+    _lib.check(_lib.lib().brm_set_tune_cache(path.encode() if path else None))
+
+
 def spgemm_batched(pairs, stream=None, timing=None):
     """C_i = A_i * B_i for a list of (DeviceCSR A_i, DeviceCSR B_i).
 

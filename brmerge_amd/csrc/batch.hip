@@ -16,6 +16,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cstdlib>
+#include <fstream>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -49,6 +53,8 @@ struct Fused {
     const int64_t* kbase;  // fused B rows
     const int64_t* abase;  // fused A entries
     const int64_t* bbase;  // fused B entries
+    const int64_t* achunk;  // entry chunks of A: chunks of pair p are achunk[p] .. achunk[p+1]-1
+    const int64_t* bchunk;  // entry chunks of B, numbered after all of A's
     int* arpt;
     int* acol;
     double* aval;
@@ -120,40 +126,61 @@ __global__ __launch_bounds__(CAT_NT) void k_cat_rows(Fused f, int* err) {
 
 // Fused entries: A columns shifted to the pair's B rows (checked against
 // the pair's K), B columns kept (checked against the pair's N); values
-// copied when `vals`, indices written when `cols`.
+// copied when `vals`, indices written when `cols`.  Entries go in chunks of
+// CAT_CHUNK, each within one pair: workgroup b finds its pair by a search
+// over the chunk table (the same for the whole workgroup) and copies its
+// chunk with coalesced loads and stores, CAT_EPT entries per thread in
+// flight.
+constexpr int CAT_EPT = 16;
+constexpr int64_t CAT_CHUNK = (int64_t)CAT_NT * CAT_EPT;
+
 __global__ __launch_bounds__(CAT_NT) void k_cat_entries(Fused f, int cols, int vals, int* err) {
-    const int64_t chunk = ((int64_t)blockIdx.x * CAT_NT + (threadIdx.x & ~(BRM_WAVE - 1))) * CAT_ITEMS;
-    const int lane = lane_id();
-    const int64_t total = f.nnzA + f.nnzB;
-    for (int it = 0; it < CAT_ITEMS; ++it) {
-        const int64_t x0 = chunk + (int64_t)it * BRM_WAVE;
-        if (x0 >= total) break;
-        const int64_t x = x0 + lane;
-        if (x < f.nnzA) {
-            const int p = x0 + BRM_WAVE <= f.nnzA ? wave_seg(f.abase, f.n, x0, x) : seg_search(f.abase, 0, f.n - 1, x);
-            const PairDev& P = f.pairs[p];
-            const int64_t j = x - f.abase[p];
-            if (cols) {
-                const int c = P.acol[j];
-                if (c < 0 || c >= P.K) atomicOr(err, (int)DERR_INVALID);
-                f.acol[x] = (int)(min(max(c, 0), max(P.K - 1, 0)) + f.kbase[p]);
+    const int64_t b = blockIdx.x;
+    const int64_t na = f.achunk[f.n];
+    const bool isA = b < na;
+    const int64_t* chunk = isA ? f.achunk : f.bchunk;
+    const int64_t cb = isA ? b : b - na;  // chunk number within A's or B's
+    const int p = seg_search(chunk, 0, f.n - 1, cb);
+    const PairDev& P = f.pairs[p];
+    const int64_t j0 = (cb - chunk[p]) * CAT_CHUNK;
+    const int64_t jn = min((int64_t)CAT_CHUNK, (isA ? P.nnzA : P.nnzB) - j0);
+    const int* sc = isA ? P.acol : P.bcol;
+    const double* sv = isA ? P.aval : P.bval;
+    int* dc = (isA ? f.acol + f.abase[p] : f.bcol + f.bbase[p]) + j0;
+    double* dv = (isA ? f.aval + f.abase[p] : f.bval + f.bbase[p]) + j0;
+    const int lim = isA ? P.K : P.N;
+    const int shift = isA ? (int)f.kbase[p] : 0;
+    bool bad = false;
+    if (cols) {
+        int c[CAT_EPT];
+#pragma unroll
+        for (int i = 0; i < CAT_EPT; ++i) {
+            const int64_t j = (int64_t)i * CAT_NT + threadIdx.x;
+            c[i] = j < jn ? sc[j0 + j] : 0;
+        }
+#pragma unroll
+        for (int i = 0; i < CAT_EPT; ++i) {
+            const int64_t j = (int64_t)i * CAT_NT + threadIdx.x;
+            if (j < jn) {
+                bad |= c[i] < 0 || c[i] >= lim;
+                dc[j] = min(max(c[i], 0), max(lim - 1, 0)) + shift;
             }
-            if (vals) f.aval[x] = P.aval[j];
-        } else if (x < total) {
-            const int64_t y = x - f.nnzA;
-            const int64_t y0 = x0 - f.nnzA;
-            const int p = (y0 >= 0 && x0 + BRM_WAVE <= total) ? wave_seg(f.bbase, f.n, y0, y)
-                                                              : seg_search(f.bbase, 0, f.n - 1, y);
-            const PairDev& P = f.pairs[p];
-            const int64_t j = y - f.bbase[p];
-            if (cols) {
-                const int c = P.bcol[j];
-                if (c < 0 || c >= P.N) atomicOr(err, (int)DERR_INVALID);
-                f.bcol[y] = c;
-            }
-            if (vals) f.bval[y] = P.bval[j];
         }
     }
+    if (vals) {
+        double v[CAT_EPT];
+#pragma unroll
+        for (int i = 0; i < CAT_EPT; ++i) {
+            const int64_t j = (int64_t)i * CAT_NT + threadIdx.x;
+            v[i] = j < jn ? sv[j0 + j] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < CAT_EPT; ++i) {
+            const int64_t j = (int64_t)i * CAT_NT + threadIdx.x;
+            if (j < jn) dv[j] = v[i];
+        }
+    }
+    if (__ballot(bad) != 0ull && lane_id() == 0) atomicOr(err, (int)DERR_INVALID);
 }
 
 // Per-product row pointers of a fused result: product p owns fused rows
@@ -184,6 +211,7 @@ struct FusedBatch {
     Scratch sc;
     Fused f{};
     int Nmax = 0;
+    int64_t nchunks = 0;  // workgroups of k_cat_entries
     std::vector<int64_t> mbase, kbase, abase, bbase;
 };
 
@@ -249,16 +277,24 @@ int build_fused(const brm_pair* pairs, int n, hipStream_t s, FusedBatch& fb, int
     int64_t* tab_d;
     Scratch& sc = fb.sc;
     BRMC(sc.alloc(&pairs_d, n, "batch pairs"));
-    BRMC(sc.alloc(&tab_d, 4 * ((int64_t)n + 1), "batch tables"));
+    BRMC(sc.alloc(&tab_d, 6 * ((int64_t)n + 1), "batch tables"));
     BRMC(sc.alloc(&f.arpt, f.M + 1, "fused A rpt"));
     BRMC(sc.alloc(&f.acol, f.nnzA, "fused A col"));
     BRMC(sc.alloc(&f.aval, f.nnzA, "fused A val"));
     BRMC(sc.alloc(&f.brpt, f.K + 1, "fused B rpt"));
     BRMC(sc.alloc(&f.bcol, f.nnzB, "fused B col"));
     BRMC(sc.alloc(&f.bval, f.nnzB, "fused B val"));
+    // entry chunks: those of A first, then those of B
+    std::vector<int64_t> achunk(n + 1, 0), bchunk(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        achunk[i + 1] = achunk[i] + (pairs[i].nnzA + CAT_CHUNK - 1) / CAT_CHUNK;
+        bchunk[i + 1] = bchunk[i] + (a_col_off ? 0 : (pairs[i].nnzB + CAT_CHUNK - 1) / CAT_CHUNK);
+    }
+    fb.nchunks = achunk[n] + bchunk[n];
     std::vector<int64_t> tab;
-    tab.reserve(4 * ((size_t)n + 1));
-    for (auto* v : {&fb.mbase, &fb.kbase, &fb.abase, &fb.bbase}) tab.insert(tab.end(), v->begin(), v->end());
+    tab.reserve(6 * ((size_t)n + 1));
+    for (auto* v : {&fb.mbase, &fb.kbase, &fb.abase, &fb.bbase, &achunk, &bchunk})
+        tab.insert(tab.end(), v->begin(), v->end());
     if (n > 0) HIPC(hipMemcpyAsync(pairs_d, pd.data(), sizeof(PairDev) * n, hipMemcpyHostToDevice, s));
     HIPC(hipMemcpyAsync(tab_d, tab.data(), sizeof(int64_t) * tab.size(), hipMemcpyHostToDevice, s));
     f.pairs = pairs_d;
@@ -266,6 +302,8 @@ int build_fused(const brm_pair* pairs, int n, hipStream_t s, FusedBatch& fb, int
     f.kbase = tab_d + (n + 1);
     f.abase = tab_d + 2 * ((int64_t)n + 1);
     f.bbase = tab_d + 3 * ((int64_t)n + 1);
+    f.achunk = tab_d + 4 * ((int64_t)n + 1);
+    f.bchunk = tab_d + 5 * ((int64_t)n + 1);
     HIPC(launch_fill_i32(derr, 1, 0, s));
     if (n > 0) {
         {
@@ -273,9 +311,9 @@ int build_fused(const brm_pair* pairs, int n, hipStream_t s, FusedBatch& fb, int
             k_cat_rows<<<cat_blocks(f.M + f.K), CAT_NT, 0, s>>>(f, derr);
         }
         HIPC(hipGetLastError());
-        if (f.nnzA + f.nnzB > 0) {
+        if (fb.nchunks > 0) {
             ProfScope ps_("k_cat_entries", s);
-            k_cat_entries<<<cat_blocks(f.nnzA + f.nnzB), CAT_NT, 0, s>>>(f, 1, 1, derr);
+            k_cat_entries<<<(unsigned)fb.nchunks, CAT_NT, 0, s>>>(f, 1, 1, derr);
         }
         HIPC(hipGetLastError());
     } else {
@@ -290,11 +328,13 @@ DevIn fused_in(const FusedBatch& fb) {
     return DevIn{f.arpt, f.acol, f.aval, f.nnzA, f.brpt, f.bcol, f.bval, f.nnzB, (int)f.M, (int)f.K, fb.Nmax};
 }
 
-hipError_t refresh_values(const Fused& f, int* derr, hipStream_t s) {
-    if (f.n == 0 || f.nnzA + f.nnzB == 0) return hipSuccess;
+hipError_t refresh_values(const FusedBatch& fb, int* derr, hipStream_t s) {
+    const Fused& f = fb.f;
+    const int64_t nchunks = fb.nchunks;
+    if (f.n == 0 || nchunks == 0) return hipSuccess;
     {
         ProfScope ps_("k_cat_entries", s);
-        k_cat_entries<<<cat_blocks(f.nnzA + f.nnzB), CAT_NT, 0, s>>>(f, 0, 1, derr);
+        k_cat_entries<<<(unsigned)nchunks, CAT_NT, 0, s>>>(f, 0, 1, derr);
     }
     return hipGetLastError();
 }
@@ -308,6 +348,147 @@ struct TuneCand {
 };
 const TuneCand kCands[] = {{0, 4096, 0},   {2, 2048, 0},   {0, 4096, 256}, {0, 4096, 512},
                            {0, 4096, 1024}, {2, 2048, 512}, {2, 2048, 1024}};
+
+// ---- tuned-shape cache: a JSON file of tier shapes keyed by a batch's
+// shape statistics (log2 buckets of rows, products, columns and the row
+// count of every bin), so that a plan over a batch of the same shape skips
+// the tuner.  Written one entry per line (read in any JSON layout):
+//   {"version": 1, "entries": [
+//     {"key": "...", "lds_cfg": 0, "med_cap": 4096, "wmed_cap": 1024, "ms": 250.9},
+//     ...
+//   ]}
+// The file is rewritten whole: a temporary file, then rename.
+std::mutex g_tune_mu;
+std::string g_tune_path;
+bool g_tune_path_set = false;
+
+std::string tune_path() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tune_path_set) {
+        const char* e = getenv("BRM_TUNE_CACHE");
+        g_tune_path = e ? e : "";
+        g_tune_path_set = true;
+    }
+    return g_tune_path;
+}
+
+int lg2_bucket(int64_t x) {
+    int b = 0;
+    while (x > 1) {
+        x >>= 1;
+        ++b;
+    }
+    return b;
+}
+
+std::string tune_key(const Pattern& pat) {
+    std::string k = "rows" + std::to_string(lg2_bucket(pat.a.M)) + "_ip" + std::to_string(lg2_bucket(pat.total_ip)) +
+                    "_n" + std::to_string(lg2_bucket(pat.a.N)) + "_bins";
+    for (int b = 0; b < BIN_COUNT; ++b) k += (b ? "." : "") + std::to_string(lg2_bucket((int64_t)pat.bc[b] + 1));
+    return k;
+}
+
+struct TuneEntry {
+    std::string key;
+    int lds_cfg, med_cap, wmed_cap;
+    double ms;
+};
+
+// Value of "name" (a number) in one entry object's text, or false.
+bool json_num(const std::string& obj, const char* name, double* v) {
+    const std::string pat = std::string("\"") + name + "\"";
+    const size_t at = obj.find(pat);
+    if (at == std::string::npos) return false;
+    size_t i = at + pat.size();
+    while (i < obj.size() && (obj[i] == ' ' || obj[i] == ':' || obj[i] == '\t' || obj[i] == '\n')) ++i;
+    const char* p = obj.c_str() + i;
+    char* end = nullptr;
+    *v = strtod(p, &end);
+    return end != p;
+}
+
+// Every entry object of the file: the text from each "key" to the next '}'.
+std::vector<TuneEntry> tune_load(const std::string& path) {
+    std::vector<TuneEntry> out;
+    std::ifstream in(path);
+    if (!in) return out;
+    std::stringstream ss;
+    ss << in.rdbuf();
+    const std::string text = ss.str();
+    for (size_t at = text.find("\"key\""); at != std::string::npos; at = text.find("\"key\"", at + 5)) {
+        const size_t close = text.find('}', at);
+        if (close == std::string::npos) break;
+        const std::string obj = text.substr(at, close - at);
+        size_t i = 5;
+        while (i < obj.size() && (obj[i] == ' ' || obj[i] == ':' || obj[i] == '\t' || obj[i] == '\n')) ++i;
+        if (i >= obj.size() || obj[i] != '"') continue;
+        const size_t ke = obj.find('"', i + 1);
+        if (ke == std::string::npos) continue;
+        TuneEntry e{obj.substr(i + 1, ke - i - 1), 0, 0, 0, 0.0};
+        double a, b, c, ms = 0;
+        if (!json_num(obj, "lds_cfg", &a) || !json_num(obj, "med_cap", &b) || !json_num(obj, "wmed_cap", &c)) continue;
+        (void)json_num(obj, "ms", &ms);
+        e.lds_cfg = (int)a;
+        e.med_cap = (int)b;
+        e.wmed_cap = (int)c;
+        e.ms = ms;
+        out.push_back(e);
+    }
+    return out;
+}
+
+// The candidate a cached entry names, or -1 (absent, or not a candidate).
+int tune_lookup(const std::string& path, const std::string& key) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const TuneEntry& e : tune_load(path)) {
+        if (e.key != key) continue;
+        for (int c = 0; c < (int)(sizeof(kCands) / sizeof(kCands[0])); ++c)
+            if (kCands[c].lds_cfg == e.lds_cfg && kCands[c].med_cap == e.med_cap && kCands[c].wmed_cap == e.wmed_cap)
+                return c;
+    }
+    return -1;
+}
+
+int tune_store(const std::string& path, const std::string& key, int cand, double ms) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::vector<TuneEntry> es = tune_load(path);
+    const TuneCand& c = kCands[cand];
+    bool found = false;
+    for (TuneEntry& e : es) {
+        if (e.key == key) {
+            e = TuneEntry{key, c.lds_cfg, c.med_cap, c.wmed_cap, ms};
+            found = true;
+        }
+    }
+    if (!found) es.push_back(TuneEntry{key, c.lds_cfg, c.med_cap, c.wmed_cap, ms});
+    const std::string tmp = path + ".tmp";
+    {
+        std::ofstream o(tmp, std::ios::trunc);
+        if (!o) {
+            set_err("tune cache: cannot write %s", tmp.c_str());
+            return BRM_E_INVALID;
+        }
+        o << "{\"version\": 1, \"entries\": [\n";
+        for (size_t i = 0; i < es.size(); ++i) {
+            char item[512];
+            snprintf(item, sizeof(item),
+                     "  {\"key\": \"%s\", \"lds_cfg\": %d, \"med_cap\": %d, \"wmed_cap\": %d, \"ms\": %.6f}%s\n",
+                     es[i].key.c_str(), es[i].lds_cfg, es[i].med_cap, es[i].wmed_cap, es[i].ms,
+                     i + 1 < es.size() ? "," : "");
+            o << item;
+        }
+        o << "]}\n";
+        if (!o) {
+            set_err("tune cache: write to %s failed", tmp.c_str());
+            return BRM_E_INVALID;
+        }
+    }
+    if (rename(tmp.c_str(), path.c_str()) != 0) {
+        set_err("tune cache: cannot rename %s to %s", tmp.c_str(), path.c_str());
+        return BRM_E_INVALID;
+    }
+    return BRM_OK;
+}
 
 }  // namespace
 
@@ -348,6 +529,8 @@ struct brm_batch_plan {
     hipGraphExec_t exec = nullptr;
     std::vector<std::pair<int, double>> tuned;  // candidate index, best ms
     int chosen = -1;
+    std::string tune_key;
+    const char* tune_source = "none";  // "timed", "cache" or "none"
     double analyse_s = 0;
     ~brm_batch_plan() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -416,8 +599,12 @@ int plan_build(brm_batch_plan* p, const brm_pair* pairs, int n, int autotune, hi
         }
         BRMC(check_dev_err(pat.derr, s, "batch plan"));
     } else {
+        const std::string path = tune_path();
+        p->tune_key = tune_key(pat);
+        p->chosen = path.empty() ? -1 : tune_lookup(path, p->tune_key);
         double best = -1;
-        const int nc = (int)(sizeof(kCands) / sizeof(kCands[0]));
+        const int nc = p->chosen >= 0 ? 0 : (int)(sizeof(kCands) / sizeof(kCands[0]));
+        p->tune_source = p->chosen >= 0 ? "cache" : "timed";
         for (int c = 0; c < nc; ++c) {
             BRMC(pattern_bin(pat, kCands[c].lds_cfg, kCands[c].med_cap, kCands[c].wmed_cap, 0, s));
             const double ms = time_numeric(p, s, 2);
@@ -432,8 +619,17 @@ int plan_build(brm_batch_plan* p, const brm_pair* pairs, int n, int autotune, hi
                 p->chosen = c;
             }
         }
+        if (nc > 0 && !path.empty()) BRMC(tune_store(path, p->tune_key, p->chosen, best));
         const TuneCand& best_c = kCands[p->chosen];
         BRMC(pattern_bin(pat, best_c.lds_cfg, best_c.med_cap, best_c.wmed_cap, 0, s));
+        if (nc == 0) {
+            // a cached shape: one eager pass, as without the tuner
+            if (time_numeric(p, s, 1) < 0) {
+                set_err("batch plan: numeric pass failed");
+                return BRM_E_DEVICE;
+            }
+            BRMC(check_dev_err(pat.derr, s, "batch plan"));
+        }
     }
     // The graph: value refresh from the caller's arrays, then the short and
     // medium tiers.  Kernels only (no memsets, no host round trips).
@@ -444,7 +640,7 @@ int plan_build(brm_batch_plan* p, const brm_pair* pairs, int n, int autotune, hi
     hipError_t e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal);
     int st = hip_status(e, "hipStreamBeginCapture");
     if (st == BRM_OK) {
-        hipError_t e1 = refresh_values(p->fb.f, pat.derr, p->cap);
+        hipError_t e1 = refresh_values(p->fb, pat.derr, p->cap);
         int st1 = e1 == hipSuccess ? pattern_numeric_tiers(pat, p->ccol, p->cval, p->cap) : hip_status(e1, "refresh");
         hipError_t e2 = hipStreamEndCapture(p->cap, &p->graph);
         st = st1 != BRM_OK ? st1 : hip_status(e2, "hipStreamEndCapture");
@@ -562,9 +758,16 @@ int brm_batch_plan_info(const brm_batch_plan* p, char* buf, uint64_t cap) {
                  p->tuned[i].second);
         s += item;
     }
-    s += "]}";
+    s += "], \"tune_key\": \"" + p->tune_key + "\", \"tune_source\": \"" + p->tune_source + "\"}";
     if (s.size() + 1 > cap) return BRM_E_INVALID;
     memcpy(buf, s.c_str(), s.size() + 1);
+    return BRM_OK;
+}
+
+int brm_set_tune_cache(const char* path) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_path = path ? path : "";
+    g_tune_path_set = true;
     return BRM_OK;
 }
 

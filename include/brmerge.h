@@ -279,8 +279,20 @@ int brm_batch_plan_create(const brm_pair *pairs, int npairs, int autotune,
 int brm_batch_plan_execute(brm_batch_plan *plan, void *stream);
 int brm_batch_plan_result(const brm_batch_plan *plan, const int64_t **crpt, const int **ccol,
                           const double **cval, int64_t *cnnz, const int64_t **row_off);
-/* JSON summary: sizes, tier counts, chosen shape, tuner timings. */
+/* JSON summary: sizes, tier counts, chosen shape, tuner timings, the
+ * batch's shape key and where the shape came from ("timed", "cache" or
+ * "none" without autotune). */
 int brm_batch_plan_info(const brm_batch_plan *plan, char *buf, uint64_t cap);
+/* Tuned-shape cache (no counterpart in the reference): a JSON file of the
+ * tier shapes the plan tuner chose, keyed by the batch's shape statistics
+ * (log2 buckets of rows, intermediate products, columns and the row count of
+ * every tier bin).  With a cache set, brm_batch_plan_create with `autotune`
+ * first looks the batch's key up and, on a hit, uses that shape without
+ * timing the candidates; on a miss it tunes and records the choice.  The
+ * file is rewritten whole (temporary file, then rename).  NULL or "" turns
+ * the cache off; until this is called, the BRM_TUNE_CACHE environment
+ * variable names the file. */
+int brm_set_tune_cache(const char *path);
 void brm_batch_plan_destroy(brm_batch_plan *plan);
 
 /*

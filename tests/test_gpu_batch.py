@@ -150,3 +150,46 @@ def test_batch_plan_with_long_rows(gpu):
             b.free()
     finally:
         gpu.set_param("force_long", old)
+
+
+def test_batch_plan_tune_cache(gpu, tmp_path):
+    """The tuner's choice is written to the tune cache under the batch's
+    shape key; a second plan over a batch of the same shape takes the shape
+    from the file without timing candidates, and both are exact."""
+    import json as _json
+    path = tmp_path / "tune.json"
+    pairs = _rand_pairs(31, 6)
+    dev = [(_upload(gpu, A, len(B[0]) - 1), _upload(gpu, B, B[3])) for A, B in pairs]
+    gpu.set_tune_cache(str(path))
+    try:
+        p1 = gpu.BatchPlan(dev, autotune=True)
+        i1 = p1.info()
+        assert i1["tune_source"] == "timed" and len(i1["tuning_ms"]) >= 2
+        doc = _json.loads(path.read_text())
+        ent = [e for e in doc["entries"] if e["key"] == i1["tune_key"]]
+        assert len(ent) == 1
+        assert (ent[0]["lds_cfg"], ent[0]["med_cap"], ent[0]["wmed_cap"]) == (i1["lds_cfg"], i1["med_cap"],
+                                                                              i1["wmed_cap"])
+        p1.execute()
+        _check_products(p1.products(), pairs)
+        p1.close()
+        p2 = gpu.BatchPlan(dev, autotune=True)
+        i2 = p2.info()
+        assert i2["tune_source"] == "cache" and i2["tuning_ms"] == [] and i2["tune_key"] == i1["tune_key"]
+        assert (i2["lds_cfg"], i2["med_cap"], i2["wmed_cap"]) == (i1["lds_cfg"], i1["med_cap"], i1["wmed_cap"])
+        p2.execute()
+        _check_products(p2.products(), pairs)
+        p2.close()
+        # a different entry for another key is kept when the file is rewritten
+        doc["entries"].append({"key": "other", "lds_cfg": 0, "med_cap": 4096, "wmed_cap": 0, "ms": 1.0})
+        path.write_text(_json.dumps(doc))
+        gpu.set_tune_cache(str(path))
+        p3 = gpu.BatchPlan(dev, autotune=True)
+        assert p3.info()["tune_source"] == "cache"
+        p3.close()
+        assert {e["key"] for e in _json.loads(path.read_text())["entries"]} == {i1["tune_key"], "other"}
+    finally:
+        gpu.set_tune_cache(None)
+        for a, b in dev:
+            a.free()
+            b.free()
