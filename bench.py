@@ -62,6 +62,8 @@ def kernel_bytes(name, A, L, ip, nnz_row, tier):
     sel = {"k_short_numeric": 1, "k_short_count": 1, "k_medium": 2, "k_sym_hash": 2, "k_sym_bitmap": 3,
            "k_medium_w": 4, "k_rmed": 4, "k_sym_hash_w": 4,
            "k_bgroup": 3, "k_group": 3, "k_leaf": 3}
+    if name == "k_small":  # the fused short-row path finishes every row
+        return 20 * int(L.sum()) + 12 * int(ip.sum()) + 12 * int(nnz_row.sum()) + 12 * len(L)
     if name not in sel:
         return None
     m = tier == sel[name]
