@@ -20,11 +20,14 @@
 //      (BRMerge-SpGEMM/inc/spgemm.h:639-707: siblings added left + right,
 //      a node with one present child passes it through).
 // The work per element does not depend on the number of children.  The
-// fan-in (BGROUP_KMAX = 2^BPASS_BITS) is set by the take rate: the bound is
-// the smallest end among the partially loaded children, so the longer each
-// child's share of the window, the more of the window is taken per round.
-// With 2^5 children a round takes most of its window (with 2^8, about half),
-// which outweighs the extra pass over the rows with more than 2^10 lists.
+// fan-in (BGROUP_KMAX = 2^BPASS_BITS) trades the take rate against the
+// number of HBM passes: the bound is the smallest end among the partially
+// loaded children, so the longer each child's share of the window, the more
+// of the window is taken per round, while every pass streams the row's
+// nodes through HBM once more.  With 2^6 children a round of R-MAT-20 takes
+// 1,120 of its 2,048 elements (1,156 with 2^5), and 9% fewer elements pass
+// through the engine (rows of 33 to 64 lists take one pass, of 1,025 to
+// 4,096 two); 2^7 children do not fit three workgroups per CU.
 #include <limits.h>
 #include <stddef.h>
 

@@ -55,7 +55,7 @@ constexpr int PASS_BITS = 5;
 constexpr int GROUP_KMAX = 1 << PASS_BITS;
 constexpr int MAX_PASSES = 8;
 // The bitmap-rank group merge (bmerge.hip) takes up to 2^BPASS_BITS children.
-constexpr int BPASS_BITS = 5;
+constexpr int BPASS_BITS = 6;
 constexpr int BGROUP_KMAX = 1 << BPASS_BITS;
 constexpr int GROUP_CFG_BITMAP = 4;
 

@@ -11,7 +11,7 @@
 // materialises every node at height h_j = min(j * pass_bits, H) from its
 // children -- the highest materialised nodes below it (frontier nodes or
 // pass j-1 nodes), at most 2^pass_bits of them.  (pass_bits is 5 for this
-// file's merge-path engine, 8 for the bitmap engine of bmerge.hip, which
+// file's merge-path engine, 6 for the bitmap engine of bmerge.hip, which
 // also runs without a frontier: every list is a child of its pass-1 node.)  One workgroup streams a
 // node's children through LDS in rounds: every child contributes a window
 // proportional to what it has left, and all elements up to the smallest

@@ -307,14 +307,14 @@ def test_signed_zero_products(gpu, tier):
 def test_wide_columns_and_hub_columns(gpu, engine, force_long):
     """Columns spread over a million (several bitmap spans per node), a few
     columns shared by most B rows (many contributions per output, deep
-    trees of pass-throughs), and rows on both sides of 2^5, 2^8 and 2^10
-    lists (one, two and three passes of the bitmap engine, whose fan-in is
-    2^5; the merge-path engine's is the same)."""
+    trees of pass-throughs), and rows on both sides of 2^5, 2^6, 2^8, 2^10
+    and 2^12 lists (pass boundaries of the merge-path engine, fan-in 2^5,
+    and of the bitmap engine, fan-in 2^6)."""
     old = gpu.get_param("force_long")
     gpu.set_param("force_long", force_long)
     try:
         rng = np.random.default_rng(11)
-        K, N = 4000, 1_000_000
+        K, N = 4500, 1_000_000
         hubs = np.array([0, 17, 500_000, 999_999])
         rows = []
         for r in range(K):
@@ -324,7 +324,8 @@ def test_wide_columns_and_hub_columns(gpu, engine, force_long):
             rows.append(np.sort(c))
         brpt = np.concatenate([[0], np.cumsum([len(b) for b in rows])]).astype(np.int32)
         B = (brpt, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, brpt[-1]), N)
-        acols = [np.sort(rng.choice(K, size=sz, replace=False)) for sz in (3000, 1025, 1024, 600, 257, 256, 255, 40, 33, 32, 3)]
+        acols = [np.sort(rng.choice(K, size=sz, replace=False))
+                 for sz in (4097, 4096, 3000, 1025, 1024, 600, 257, 256, 255, 65, 64, 40, 33, 32, 3)]
         arpt = np.concatenate([[0], np.cumsum([len(a) for a in acols])]).astype(np.int32)
         A = (arpt, np.concatenate(acols).astype(np.int32), rng.uniform(-1, 1, arpt[-1]), K)
         R = oracle.brmerge_precise(A, B)
@@ -336,8 +337,9 @@ def test_wide_columns_and_hub_columns(gpu, engine, force_long):
 
 
 def test_four_pass_rows(gpu, engine):
-    """Rows on both sides of 2^15 lists: three passes at fan-in 2^5 for
-    32,768 lists, four for 32,769, with B rows of 0 to 12 entries (empty
+    """Rows on both sides of 2^15 lists: three passes at fan-in 2^5 (the
+    merge-path engine) for 32,768 lists, four for 32,769 (three either way
+    at the bitmap engine's 2^6), with B rows of 0 to 12 entries (empty
     lists among them) and columns over 300,000 (three bitmap spans)."""
     rng = np.random.default_rng(23)
     K, N = 40_000, 300_000
@@ -352,6 +354,31 @@ def test_four_pass_rows(gpu, engine):
     Cp, Cu = run_both(gpu, A, B)
     assert_csr_equal(Cp, R)
     assert_csr_equal(Cu, R)
+
+
+def test_bitmap_four_pass_rows(gpu):
+    """Rows on both sides of 2^18 lists: three passes of the bitmap engine
+    (fan-in 2^6) for 262,144 lists, four for 262,145; B rows of 0 to 3
+    entries over 200,000 columns."""
+    old = gpu.get_param("group_cfg")
+    gpu.set_param("group_cfg", 4)
+    try:
+        rng = np.random.default_rng(29)
+        K, N = 270_000, 200_000
+        lens = rng.integers(0, 4, K)
+        brpt = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        bcol = np.concatenate([np.sort(rng.choice(N, size=int(d), replace=False)) for d in lens]).astype(np.int32)
+        B = (brpt, bcol, rng.uniform(-1, 1, brpt[-1]), N)
+        acols = [np.sort(rng.choice(K, size=sz, replace=False)) for sz in (262_145, 262_144)]
+        arpt = np.concatenate([[0], np.cumsum([len(a) for a in acols])]).astype(np.int32)
+        A = (arpt, np.concatenate(acols).astype(np.int32), rng.uniform(-1, 1, arpt[-1]), K)
+        R = oracle.brmerge_precise(A, B)
+        Cp, Cu = run_both(gpu, A, B)
+        assert gpu.last_stats()["passes"] == 4
+        assert_csr_equal(Cp, R)
+        assert_csr_equal(Cu, R)
+    finally:
+        gpu.set_param("group_cfg", old)
 
 
 @pytest.mark.parametrize("ncols", [1 << 16, (1 << 16) + 1, 1 << 17, (1 << 17) + 1, 1 << 20], ids=lambda n: f"N{n}")
