@@ -38,6 +38,12 @@ void release_cached_locked() {
     g_cached = 0;
 }
 
+// Growable arrays released by their owners stay reserved and mapped for
+// reuse (below); these give their memory back to the driver.  g_mu held.
+bool vm_release_cached_one_locked();
+void vm_release_cached_all_locked();
+size_t vm_cached_bytes_locked();
+
 }  // namespace
 
 namespace {
@@ -88,7 +94,10 @@ void* dev_alloc(size_t bytes) {
     while (hipMalloc(&p, sz) != hipSuccess) {
         (void)hipGetLastError();
         p = nullptr;
-        if (g_free.empty()) return nullptr;
+        if (g_free.empty()) {
+            if (!vm_release_cached_one_locked()) return nullptr;
+            continue;
+        }
         auto big = std::prev(g_free.end());
         (void)hipFree(big->second);
         g_cached -= big->first;
@@ -98,13 +107,13 @@ void* dev_alloc(size_t bytes) {
     return p;
 }
 
-static bool vm_free(void* p);
+static bool vm_free(void* p);  // g_mu held
 
 void dev_free(void* p) {
     if (!p) return;
     if (t_arena.base && (char*)p >= t_arena.base && (char*)p < t_arena.base + t_arena.size) return;
-    if (vm_free(p)) return;
     std::lock_guard<std::mutex> lk(g_mu);
+    if (vm_free(p)) return;
     auto it = g_used.find(p);
     if (it == g_used.end()) return;  // not ours
     g_free.emplace(it->second, p);
@@ -128,11 +137,12 @@ void dev_flush_deferred() {
 void dev_trim() {
     std::lock_guard<std::mutex> lk(g_mu);
     release_cached_locked();
+    vm_release_cached_all_locked();
 }
 
 size_t dev_cached_bytes() {
     std::lock_guard<std::mutex> lk(g_mu);
-    return g_cached;
+    return g_cached + vm_cached_bytes_locked();
 }
 
 // ---------------------------------------------------------------- growable arrays
@@ -141,7 +151,12 @@ size_t dev_cached_bytes() {
 // reach, backed by physical memory (hipMemCreate) only as it grows, in pieces
 // of exactly VM_STEP (the reservation is a whole number of pieces; on MI355X
 // access could not be granted to a larger piece).  BRMerge-Upper's row waves
-// compact into C this way.
+// compact into C this way.  Creating and mapping the pieces costs about
+// 20 ms per GiB, so a released array stays reserved and mapped in a cache,
+// like the caching allocator's blocks: a later reservation of at most its
+// size (and at least half of it) takes it over with its mapped pieces.  When
+// memory runs out, cached arrays go back to the driver after the cached
+// blocks.  All of it is under g_mu.
 namespace {
 constexpr size_t VM_STEP = size_t(2) << 30;
 struct VmBlock {
@@ -151,8 +166,8 @@ struct VmBlock {
     int dev = 0;
     std::vector<std::pair<hipMemGenericAllocationHandle_t, size_t>> handles;
 };
-std::mutex v_mu;
-std::unordered_map<void*, VmBlock> v_blocks;  // by base address
+std::unordered_map<void*, VmBlock> v_blocks;  // in use, by base address
+std::vector<std::pair<void*, VmBlock>> v_cache;  // released, still mapped
 
 hipMemAllocationProp vm_prop(int dev) {
     hipMemAllocationProp prop{};
@@ -172,6 +187,28 @@ void vm_release_block(void* va, VmBlock& b) {
     (void)hipMemAddressFree(va, b.reserved);
     (void)hipGetLastError();
 }
+
+// the cached array with the most mapped memory
+bool vm_release_cached_one_locked() {
+    if (v_cache.empty()) return false;
+    size_t bi = 0;
+    for (size_t i = 1; i < v_cache.size(); ++i)
+        if (v_cache[i].second.mapped > v_cache[bi].second.mapped) bi = i;
+    vm_release_block(v_cache[bi].first, v_cache[bi].second);
+    v_cache.erase(v_cache.begin() + (std::ptrdiff_t)bi);
+    return true;
+}
+
+void vm_release_cached_all_locked() {
+    while (vm_release_cached_one_locked()) {
+    }
+}
+
+size_t vm_cached_bytes_locked() {
+    size_t n = 0;
+    for (const auto& c : v_cache) n += c.second.mapped;
+    return n;
+}
 }  // namespace
 
 void* vm_reserve(size_t bytes, int dev) {
@@ -183,22 +220,38 @@ void* vm_reserve(size_t bytes, int dev) {
     }
     const size_t piece = (VM_STEP + gran - 1) / gran * gran;
     const size_t res = (std::max<size_t>(bytes, 1) + piece - 1) / piece * piece;
+    std::lock_guard<std::mutex> lk(g_mu);
+    // a cached array of this device reserving at least res and at most 2 res
+    // (the smallest such)
+    size_t best = v_cache.size();
+    for (size_t i = 0; i < v_cache.size(); ++i) {
+        const VmBlock& c = v_cache[i].second;
+        if (c.dev == dev && c.gran == piece && c.reserved >= res && c.reserved <= 2 * res &&
+            (best == v_cache.size() || c.reserved < v_cache[best].second.reserved))
+            best = i;
+    }
+    if (best < v_cache.size()) {
+        void* va = v_cache[best].first;
+        v_blocks[va] = std::move(v_cache[best].second);
+        v_cache.erase(v_cache.begin() + (std::ptrdiff_t)best);
+        return va;
+    }
     void* va = nullptr;
-    if (hipMemAddressReserve(&va, res, 0, nullptr, 0) != hipSuccess || !va) {
+    while (hipMemAddressReserve(&va, res, 0, nullptr, 0) != hipSuccess || !va) {
         (void)hipGetLastError();
-        return nullptr;
+        va = nullptr;
+        if (!vm_release_cached_one_locked()) return nullptr;
     }
     VmBlock b;
     b.reserved = res;
     b.gran = piece;
     b.dev = dev;
-    std::lock_guard<std::mutex> lk(v_mu);
     v_blocks[va] = std::move(b);
     return va;
 }
 
 bool vm_grow(void* va, size_t bytes) {
-    std::lock_guard<std::mutex> lk(v_mu);
+    std::lock_guard<std::mutex> lk(g_mu);
     auto it = v_blocks.find(va);
     if (it == v_blocks.end()) return false;
     VmBlock& b = it->second;
@@ -208,15 +261,18 @@ bool vm_grow(void* va, size_t bytes) {
         const size_t sz = b.gran;  // one piece (the reservation is a whole number of them)
         hipMemGenericAllocationHandle_t h;
         // physical memory beside the caching allocator: when it runs out,
-        // cached blocks go back to the driver, largest first
+        // cached blocks go back to the driver, largest first, then cached
+        // growable arrays
         while (hipMemCreate(&h, sz, &prop, 0) != hipSuccess) {
             (void)hipGetLastError();
-            std::lock_guard<std::mutex> gk(g_mu);
-            if (g_free.empty()) return false;
-            auto big = std::prev(g_free.end());
-            (void)hipFree(big->second);
-            g_cached -= big->first;
-            g_free.erase(big);
+            if (!g_free.empty()) {
+                auto big = std::prev(g_free.end());
+                (void)hipFree(big->second);
+                g_cached -= big->first;
+                g_free.erase(big);
+            } else if (!vm_release_cached_one_locked()) {
+                return false;
+            }
         }
         if (hipMemMap((char*)va + b.mapped, sz, 0, h, 0) != hipSuccess) {
             (void)hipGetLastError();
@@ -238,11 +294,11 @@ bool vm_grow(void* va, size_t bytes) {
     return true;
 }
 
+// g_mu held
 static bool vm_free(void* p) {
-    std::lock_guard<std::mutex> lk(v_mu);
     auto it = v_blocks.find(p);
     if (it == v_blocks.end()) return false;
-    vm_release_block(p, it->second);
+    v_cache.emplace_back(p, std::move(it->second));
     v_blocks.erase(it);
     return true;
 }

@@ -949,9 +949,11 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     }
     if (ext) p.crpt_ext = ext->crpt;
     // the fused path when every row is likely short: lists per row times
-    // entries per B row at most 64 on average
+    // entries per B row at most 64 on average.  It serves BRMerge-Upper as
+    // well: it has no symbolic phase, and it writes C into a buffer sized by
+    // an upper bound (G entries per row), compacted as it goes.
     const double avg_l = in.M > 0 ? (double)in.nnzA / in.M : 0.0, avg_b = in.K > 0 ? (double)in.nnzB / in.K : 0.0;
-    if (!upper && !ext && params().small_fused && in.M > 0 && in.M <= SMALL_MAX_ROWS && !params().force_long &&
+    if (!ext && params().small_fused && in.M > 0 && in.M <= SMALL_MAX_ROWS && !params().force_long &&
         params().hash_cap == 0 && avg_l * avg_b <= 64.0) {
         bool handled = false;
         BRMC(run_small(in, avg_l * avg_b, max_nnz, s, clk, out, &handled));
