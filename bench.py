@@ -50,6 +50,17 @@ def tiers(L, ip, med_cap, wmed_cap, force_long=False):
     return t
 
 
+def workload_label(w):
+    """The product a workload name stands for, as the metric names it."""
+    if w.startswith("rmat"):
+        return "R-MAT A*A"
+    if w == "er16k":
+        return "Erdos-Renyi 16384^2 A*A"
+    if w.startswith("amg"):
+        return "AMG stencil*aggregation A*P"
+    return w
+
+
 def kernel_bytes(name, A, L, ip, nnz_row, tier):
     """Algorithmic bytes of all launches of kernel `name` in one SpGEMM, by
     the per-product model bytes_alg = 4(M+1) + 20 nnzA + 12 ip + 8(M+1) +
@@ -583,7 +594,7 @@ def main():
                 roof["workspace_bytes_per_step"] = int(eb - by)
 
     out = {
-        "metric": "SpGEMM GFLOP/s (2*intermediate products/s), BRMerge-Precise, R-MAT A*A",
+        "metric": "SpGEMM GFLOP/s (2*intermediate products/s), BRMerge-Precise, " + workload_label(args.workload),
         "value": round(gflops, 3),
         "unit": "GFLOP/s",
         "n_gpus": 1,
