@@ -395,6 +395,24 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     BRMC(wsc.alloc(&leaves, mx_leaf, "leaf tasks"));
     BRMC(wsc.alloc(&tasks, mx_task, "group tasks"));
     BRMC(wsc.alloc(&children, mx_child, "group children"));
+    // Bitmap engine: each pass's tasks run largest node first, so that the
+    // longest merges start early and the short ones fill the end of the
+    // pass (not in the fixed-buffer arena, whose budget does not hold the
+    // sort's buffers)
+    uint32_t *tkey = nullptr, *tkey_out = nullptr;
+    int *tidx_in = nullptr, *tidx_out = nullptr;
+    unsigned char* ttmp = nullptr;
+    size_t ttmp_bytes = 0;
+    if (bitmap && arena < 0 && mx_task <= INT32_MAX) {
+        ttmp_bytes = order_tasks_tmp_bytes((int)mx_task);
+        if (ttmp_bytes > 0) {
+            BRMC(wsc.alloc(&tkey, mx_task, "task keys"));
+            BRMC(wsc.alloc(&tkey_out, mx_task, "task keys"));
+            BRMC(wsc.alloc(&tidx_in, mx_task, "task order"));
+            BRMC(wsc.alloc(&tidx_out, mx_task, "task order"));
+            BRMC(wsc.alloc(&ttmp, (int64_t)ttmp_bytes, "task sort"));
+        }
+    }
     tr.mark("workspace allocated", mx_ws[0] + mx_ws[1]);
     int* const wcol[2] = {wcol0, wcol1};
     double* const wval[2] = {wval0, wval1};
@@ -425,6 +443,7 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         la.leaves = leaves;
         la.leaves_cap = w.leaf_cap;
         la.tasks = tasks;
+        la.task_key = tkey;
         la.task_seg = seg_d;
         la.children = children;
         la.child_cap = w.child_cap;
@@ -444,8 +463,14 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         const int passes = (int)cnt[CNT_MAXP];
         int64_t ntask = 0;
         for (int j = 1; j <= passes && j <= MAX_PASSES; ++j) {
+            const int* order = nullptr;
+            if (tkey && cnt[CNT_TASK0 + j] > 1) {
+                HIPC(order_tasks(tkey + w.seg[j], (int)cnt[CNT_TASK0 + j], tkey_out, tidx_in, tidx_out, ttmp,
+                                 ttmp_bytes, s));
+                order = tidx_out;
+            }
             HIPC(launch_group(tasks + w.seg[j], cnt[CNT_TASK0 + j], children, a, wcol, wval, slots, counters, o, s,
-                              group_cfg));
+                              group_cfg, order));
             ntask += cnt[CNT_TASK0 + j];
             tr.mark("group pass", cnt[CNT_TASK0 + j]);
         }

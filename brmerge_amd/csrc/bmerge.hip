@@ -423,12 +423,13 @@ __device__ void bg_wave_sum2(const BgSmem& g, int s0, int m0, int s1, int m1, do
 // CU's LDS: three workgroups per CU, so that one workgroup's barrier and
 // memory waits are covered by the other two.
 __global__ __launch_bounds__(BG_NT, 3) void k_bgroup(const GroupTask* __restrict__ tasks,
+                                                  const int* __restrict__ order,
                                                   const Child* __restrict__ children, RowArgs a, WsCols ws,
                                                   int64_t* __restrict__ slots, int64_t* counters, OutArgs o) {
     extern __shared__ __align__(16) unsigned char smraw[];
     BgSmem& g = *reinterpret_cast<BgSmem*>(smraw);
     const int t = threadIdx.x;
-    const GroupTask T = tasks[blockIdx.x];
+    const GroupTask T = tasks[order ? order[blockIdx.x] : (int)blockIdx.x];
     const int k = T.nchild;
     Slots sl;
     if (t < k) {
@@ -912,7 +913,7 @@ __global__ __launch_bounds__(BG_NT, 3) void k_bgroup(const GroupTask* __restrict
 
 hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
                          int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
-                         const OutArgs& o, hipStream_t s) {
+                         const OutArgs& o, hipStream_t s, const int* order) {
     if (n <= 0) return hipSuccess;
     static SmemLimit lim;
     if (hipError_t e = lim.raise((const void*)k_bgroup, (int)sizeof(BgSmem)); e != hipSuccess) return e;
@@ -921,8 +922,8 @@ hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* childre
         ProfScope ps_("k_bgroup", s);
         const int64_t ch = launch_chunk(BG_NT);
         for (int64_t b = 0; b < n; b += ch)
-            k_bgroup<<<(unsigned)std::min<int64_t>(ch, n - b), BG_NT, sizeof(BgSmem), s>>>(tasks + b, children, a, w,
-                                                                                            slots, counters, o);
+            k_bgroup<<<(unsigned)std::min<int64_t>(ch, n - b), BG_NT, sizeof(BgSmem), s>>>(
+                order ? tasks : tasks + b, order ? order + b : nullptr, children, a, w, slots, counters, o);
     }
     return hipGetLastError();
 }

@@ -135,6 +135,7 @@ struct LongArgs {
     LeafTask* leaves;
     int64_t leaves_cap;
     GroupTask* tasks;           // segmented by pass
+    uint32_t* task_key;         // beside tasks: the node's product count (saturated), or nullptr
     const int64_t* task_seg;    // [MAX_PASSES+2] segment start per pass
     Child* children;
     int64_t child_cap;

@@ -61,9 +61,17 @@ hipError_t launch_leaf(const LeafTask* tasks, int64_t n, const RowArgs& a, int* 
 // long.hip
 hipError_t launch_plan(const LongArgs& la, int* err, hipStream_t s);
 // group_cfg: the engine the tasks were planned for (GROUP_CFG_BITMAP: bmerge.hip)
+// order: task indices to run, in that order (workgroup b runs tasks[order[b]]), or nullptr
+// for tasks[b]; the bitmap engine only
 hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
                         int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
-                        const OutArgs& o, hipStream_t s, int group_cfg);
+                        const OutArgs& o, hipStream_t s, int group_cfg, const int* order);
+// Task order of one pass, largest node first: idx_out = the indices 0..n-1
+// sorted by keys descending (keys_out receives the sorted keys).  tmp of
+// order_tasks_tmp_bytes(n) bytes.
+size_t order_tasks_tmp_bytes(int n);
+hipError_t order_tasks(const uint32_t* keys, int n, uint32_t* keys_out, int* idx_in, int* idx_out, void* tmp,
+                       size_t tmp_bytes, hipStream_t s);
 
 // rmed.hip (register tier; cap = products per row, <= WMED_CAP)
 hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s, int cap);
@@ -99,7 +107,7 @@ hipError_t launch_hash_count(const int* rows, int n, const RowArgs& a, int64_t* 
 // bmerge.hip
 hipError_t launch_bgroup(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
                          int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
-                         const OutArgs& o, hipStream_t s);
+                         const OutArgs& o, hipStream_t s, const int* order);
 
 // assemble.hip
 hipError_t launch_compact(const int64_t* crpt, int M, const RowArgs& a, const int* cbar_col,

@@ -25,6 +25,8 @@
 
 #include <algorithm>
 
+#include <hipcub/hipcub.hpp>
+
 #include "brm_device.h"
 #include "kernels.h"
 #include "lds_merge.h"
@@ -232,8 +234,14 @@ __device__ void plan_no_frontier(const LongArgs& la, int row, int a0, const int6
                 }
                 const unsigned long long idx =
                     atomicAdd((unsigned long long*)&la.counters[CNT_TASK0 + j], 1ull);
-                if ((int64_t)idx >= la.task_seg[j + 1] - la.task_seg[j]) atomicOr(err, (int)DERR_CAPACITY);
-                else la.tasks[la.task_seg[j] + (int64_t)idx] = T;
+                if ((int64_t)idx >= la.task_seg[j + 1] - la.task_seg[j]) {
+                    atomicOr(err, (int)DERR_CAPACITY);
+                } else {
+                    la.tasks[la.task_seg[j] + (int64_t)idx] = T;
+                    if (la.task_key)
+                        la.task_key[la.task_seg[j] + (int64_t)idx] =
+                            (uint32_t)min(ipr[e] - ipr[s], (int64_t)UINT32_MAX);
+                }
             }
         }
     }
@@ -543,10 +551,11 @@ hipError_t launch_plan(const LongArgs& la, int* err, hipStream_t s) {
 
 hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children, const RowArgs& a,
                         int* const ws_col[2], double* const ws_val[2], int64_t* slots, int64_t* counters,
-                        const OutArgs& o, hipStream_t s, int group_cfg) {
+                        const OutArgs& o, hipStream_t s, int group_cfg, const int* order) {
     if (n <= 0) return hipSuccess;
     if (group_cfg == GROUP_CFG_BITMAP)
-        return launch_bgroup(tasks, n, children, a, ws_col, ws_val, slots, counters, o, s);
+        return launch_bgroup(tasks, n, children, a, ws_col, ws_val, slots, counters, o, s, order);
+    if (order) return hipErrorInvalidValue;
     WsView w{ws_col[0], ws_col[1], ws_val[0], ws_val[1]};
     ProfScope ps_("k_group", s);
     const int64_t ch = launch_chunk(256);
@@ -561,6 +570,32 @@ hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children
         }
     }
     return hipGetLastError();
+}
+
+namespace {
+__global__ void k_iota(int* __restrict__ p, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = i;
+}
+}  // namespace
+
+size_t order_tasks_tmp_bytes(int n) {
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                     (const int*)nullptr, (int*)nullptr, n, 0, 32) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return bytes;
+}
+
+hipError_t order_tasks(const uint32_t* keys, int n, uint32_t* keys_out, int* idx_in, int* idx_out, void* tmp,
+                       size_t tmp_bytes, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_iota<<<(n + 255) / 256, 256, 0, s>>>(idx_in, n);
+    if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+    size_t bytes = tmp_bytes;
+    return hipcub::DeviceRadixSort::SortPairsDescending(tmp, bytes, keys, keys_out, idx_in, idx_out, n, 0, 32, s);
 }
 
 }  // namespace brm
