@@ -961,6 +961,64 @@ int run_small(const DevIn& in, double avg_ip, int64_t max_nnz, hipStream_t s, Ph
 }
 }  // namespace
 
+// A second stream of the calling thread (default priority, created once per
+// device and kept until the process ends) for work that runs beside the
+// main stream: fork() makes it wait for what the main stream has queued,
+// mark() records where its work ends, join() makes the main stream wait for
+// it.  When not `on`, everything runs on the main stream.  The destructor
+// waits for the side stream, so no early return leaves its kernels running
+// over buffers about to be released.
+class SideStream {
+  public:
+    SideStream(hipStream_t main, bool on) : main_(main), on_(on) {}
+    ~SideStream() {
+        if (forked_) (void)hipStreamSynchronize(side_);
+    }
+    hipStream_t stream() const { return forked_ ? side_ : main_; }
+    int fork() {
+        if (!on_) return BRM_OK;
+        int dev = 0;
+        HIPC(hipGetDevice(&dev));
+        static thread_local std::vector<std::pair<int, Res>> per_dev;
+        Res* r = nullptr;
+        for (auto& pd : per_dev)
+            if (pd.first == dev) r = &pd.second;
+        if (!r) {
+            Res n{};
+            HIPC(hipStreamCreateWithFlags(&n.side, hipStreamNonBlocking));
+            HIPC(hipEventCreateWithFlags(&n.e0, hipEventDisableTiming));
+            HIPC(hipEventCreateWithFlags(&n.e1, hipEventDisableTiming));
+            per_dev.push_back({dev, n});
+            r = &per_dev.back().second;
+        }
+        side_ = r->side;
+        e1_ = r->e1;
+        HIPC(hipEventRecord(r->e0, main_));
+        HIPC(hipStreamWaitEvent(side_, r->e0, 0));
+        forked_ = true;
+        return BRM_OK;
+    }
+    int mark() {
+        if (forked_) HIPC(hipEventRecord(e1_, side_));
+        return BRM_OK;
+    }
+    int join() {
+        if (forked_) HIPC(hipStreamWaitEvent(main_, e1_, 0));
+        return BRM_OK;
+    }
+
+  private:
+    struct Res {
+        hipStream_t side;
+        hipEvent_t e0, e1;
+    };
+    hipStream_t main_;
+    bool on_;
+    hipStream_t side_ = nullptr;
+    hipEvent_t e1_ = nullptr;
+    bool forked_ = false;
+};
+
 int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph,
                const ExtOut* ext) {
     PhaseClock clk(s, ph != nullptr);
@@ -1035,9 +1093,17 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
         }
         tr.mark("C allocated", cap);
         clk.begin(P_COMPUTE);
-        BRMC(pattern_numeric_tiers(p, ccol, cval, s));
+        // With long rows, the short and medium tiers run on a side stream,
+        // beside the long-row engine: they fill the CUs its passes leave idle
+        // (planning round trips, the last passes of the heaviest rows).
+        // Different rows, disjoint parts of C.
+        SideStream side(s, p.bc[BIN_LONG] > 0 && !ext);
+        BRMC(side.fork());
+        BRMC(pattern_numeric_tiers(p, ccol, cval, side.stream()));
+        BRMC(side.mark());
         tr.mark("short+medium numeric");
         BRMC(pattern_numeric_long(p, ccol, cval, s));
+        BRMC(side.join());
         tr.mark("long numeric");
         if (small) {
             int e = 0;
