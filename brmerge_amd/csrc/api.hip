@@ -219,6 +219,66 @@ int ceil_log2_host(int x) {
     return h;
 }
 
+// A second stream of the calling thread (default priority, created once per
+// device and slot and kept until the process ends) for work that runs beside
+// the main stream: fork() makes it wait for what the main stream has queued,
+// mark() records where its work ends, join() makes the main stream wait for
+// it.  When not `on`, everything runs on the main stream.  Slots are
+// independent streams (0: the short and medium tiers, 1: alternate long-row
+// waves).  The destructor waits for the side stream, so no early return
+// leaves its kernels running over buffers about to be released.
+class SideStream {
+  public:
+    SideStream(hipStream_t main, bool on, int slot = 0) : main_(main), on_(on), slot_(slot) {}
+    ~SideStream() {
+        if (forked_) (void)hipStreamSynchronize(side_);
+    }
+    hipStream_t stream() const { return forked_ ? side_ : main_; }
+    int fork() {
+        if (!on_) return BRM_OK;
+        int dev = 0;
+        HIPC(hipGetDevice(&dev));
+        static thread_local std::vector<std::pair<int, Res>> per_dev;  // (device * 2 + slot)
+        Res* r = nullptr;
+        for (auto& pd : per_dev)
+            if (pd.first == dev * 2 + slot_) r = &pd.second;
+        if (!r) {
+            Res n{};
+            HIPC(hipStreamCreateWithFlags(&n.side, hipStreamNonBlocking));
+            HIPC(hipEventCreateWithFlags(&n.e0, hipEventDisableTiming));
+            HIPC(hipEventCreateWithFlags(&n.e1, hipEventDisableTiming));
+            per_dev.push_back({dev * 2 + slot_, n});
+            r = &per_dev.back().second;
+        }
+        side_ = r->side;
+        e1_ = r->e1;
+        HIPC(hipEventRecord(r->e0, main_));
+        HIPC(hipStreamWaitEvent(side_, r->e0, 0));
+        forked_ = true;
+        return BRM_OK;
+    }
+    int mark() {
+        if (forked_) HIPC(hipEventRecord(e1_, side_));
+        return BRM_OK;
+    }
+    int join() {
+        if (forked_) HIPC(hipStreamWaitEvent(main_, e1_, 0));
+        return BRM_OK;
+    }
+
+  private:
+    struct Res {
+        hipStream_t side;
+        hipEvent_t e0, e1;
+    };
+    hipStream_t main_;
+    bool on_;
+    int slot_;
+    hipStream_t side_ = nullptr;
+    hipEvent_t e1_ = nullptr;
+    bool forked_ = false;
+};
+
 // ---------------------------------------------------------------- long rows
 int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, const int64_t* final_off,
              hipStream_t s, Scratch& sc) {
@@ -376,41 +436,53 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         mx_rows = (int)round_geo(mx_rows, (int64_t)1 << 10);
     }
     Scratch wsc;
-    int *rows_w, *wcol0, *wcol1;
+    int *wcol0, *wcol1;
     double *wval0, *wval1;
-    int64_t *wsb_d, *slb_d, *seg_d, *slots, *counters;
     LeafTask* leaves;
-    GroupTask* tasks;
-    Child* children;
-    BRMC(wsc.alloc(&rows_w, mx_rows, "wave rows"));
-    BRMC(wsc.alloc(&wsb_d, 2 * (int64_t)mx_rows, "wave ws_base"));
-    BRMC(wsc.alloc(&slb_d, mx_rows, "wave slot_base"));
-    BRMC(wsc.alloc(&seg_d, MAX_PASSES + 2, "task segments"));
     BRMC(wsc.alloc(&wcol0, mx_ws[0], "workspace"));
     BRMC(wsc.alloc(&wcol1, mx_ws[1], "workspace"));
     BRMC(wsc.alloc(&wval0, mx_ws[0], "workspace"));
     BRMC(wsc.alloc(&wval1, mx_ws[1], "workspace"));
-    BRMC(wsc.alloc(&slots, mx_sl, "length slots"));
-    BRMC(wsc.alloc(&counters, CNT_N, "planner counters"));
     BRMC(wsc.alloc(&leaves, mx_leaf, "leaf tasks"));
-    BRMC(wsc.alloc(&tasks, mx_task, "group tasks"));
-    BRMC(wsc.alloc(&children, mx_child, "group children"));
-    // Bitmap engine: each pass's tasks run largest node first, so that the
-    // longest merges start early and the short ones fill the end of the
-    // pass (not in the fixed-buffer arena, whose budget does not hold the
-    // sort's buffers)
-    uint32_t *tkey = nullptr, *tkey_out = nullptr;
-    int *tidx_in = nullptr, *tidx_out = nullptr;
-    unsigned char* ttmp = nullptr;
-    size_t ttmp_bytes = 0;
-    if (bitmap && arena < 0 && mx_task <= INT32_MAX) {
-        ttmp_bytes = order_tasks_tmp_bytes((int)mx_task);
+    // Per-wave tables.  Bitmap engine: each pass's tasks run largest node
+    // first, so that the longest merges start early and the short ones fill
+    // the end of the pass (not in the fixed-buffer arena, whose budget does
+    // not hold the sort's buffers).
+    struct WaveTables {
+        int* rows_w = nullptr;
+        int64_t *wsb_d = nullptr, *slb_d = nullptr, *seg_d = nullptr, *slots = nullptr, *counters = nullptr;
+        GroupTask* tasks = nullptr;
+        Child* children = nullptr;
+        uint32_t *tkey = nullptr, *tkey_out = nullptr;
+        int *tidx_in = nullptr, *tidx_out = nullptr;
+        unsigned char* ttmp = nullptr;
+    };
+    const size_t ttmp_bytes = bitmap && arena < 0 && mx_task <= INT32_MAX ? order_tasks_tmp_bytes((int)mx_task) : 0;
+    // Consecutive waves overlap on two streams (bitmap engine with ordered
+    // tasks, more than one wave): each parity has its own tables, and the
+    // workspace buffers are handed over by events -- wave w+1's pass 1
+    // (writes buffer 1) waits for the last pass of wave w that touches buffer
+    // 1, its pass 2 (writes buffer 0) for the end of wave w.  Wave w's last
+    // pass, the roots of its heaviest rows, then runs beside wave w+1's
+    // planning and first pass.
+    const bool overlap = bitmap && arena < 0 && ttmp_bytes > 0 && waves.size() > 1;
+    WaveTables wt[2];
+    for (int par = 0; par < (overlap ? 2 : 1); ++par) {
+        WaveTables& t = wt[par];
+        BRMC(wsc.alloc(&t.rows_w, mx_rows, "wave rows"));
+        BRMC(wsc.alloc(&t.wsb_d, 2 * (int64_t)mx_rows, "wave ws_base"));
+        BRMC(wsc.alloc(&t.slb_d, mx_rows, "wave slot_base"));
+        BRMC(wsc.alloc(&t.seg_d, MAX_PASSES + 2, "task segments"));
+        BRMC(wsc.alloc(&t.slots, mx_sl, "length slots"));
+        BRMC(wsc.alloc(&t.counters, CNT_N, "planner counters"));
+        BRMC(wsc.alloc(&t.tasks, mx_task, "group tasks"));
+        BRMC(wsc.alloc(&t.children, mx_child, "group children"));
         if (ttmp_bytes > 0) {
-            BRMC(wsc.alloc(&tkey, mx_task, "task keys"));
-            BRMC(wsc.alloc(&tkey_out, mx_task, "task keys"));
-            BRMC(wsc.alloc(&tidx_in, mx_task, "task order"));
-            BRMC(wsc.alloc(&tidx_out, mx_task, "task order"));
-            BRMC(wsc.alloc(&ttmp, (int64_t)ttmp_bytes, "task sort"));
+            BRMC(wsc.alloc(&t.tkey, mx_task, "task keys"));
+            BRMC(wsc.alloc(&t.tkey_out, mx_task, "task keys"));
+            BRMC(wsc.alloc(&t.tidx_in, mx_task, "task order"));
+            BRMC(wsc.alloc(&t.tidx_out, mx_task, "task order"));
+            BRMC(wsc.alloc(&t.ttmp, (int64_t)ttmp_bytes, "task sort"));
         }
     }
     tr.mark("workspace allocated", mx_ws[0] + mx_ws[1]);
@@ -418,80 +490,154 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     double* const wval[2] = {wval0, wval1};
     int* derr = o.err;
     const LdsShape sh = lds_shape(params().lds_cfg);
+    SideStream wave2(s, overlap, 1);
+    BRMC(wave2.fork());
+    struct Hand {
+        hipEvent_t b1_free = nullptr, done = nullptr;  // handed to the next wave
+    };
+    static thread_local std::vector<std::pair<int, Hand>> hand_pool;  // per device, two per parity
+    Hand hand[2];
+    if (overlap) {
+        int dev = 0;
+        HIPC(hipGetDevice(&dev));
+        for (int par = 0; par < 2; ++par) {
+            Hand* h = nullptr;
+            for (auto& hp : hand_pool)
+                if (hp.first == dev * 2 + par) h = &hp.second;
+            if (!h) {
+                Hand n;
+                HIPC(hipEventCreateWithFlags(&n.b1_free, hipEventDisableTiming));
+                HIPC(hipEventCreateWithFlags(&n.done, hipEventDisableTiming));
+                hand_pool.push_back({dev * 2 + par, n});
+                h = &hand_pool.back().second;
+            }
+            hand[par] = *h;
+        }
+    }
+    // the work counters of a wave, read once its stream has finished it
+    struct PendingStats {
+        bool on = false;
+        int par = 0;
+        hipStream_t st = nullptr;
+        std::vector<int64_t> cnt;
+        int64_t ntask = 0;
+        int passes = 0;
+    };
+    auto read_stats = [&](PendingStats& ps) -> int {
+        if (!ps.on) return BRM_OK;
+        ps.on = false;
+        std::vector<int64_t> c2(CNT_N);
+        BRMC(d2h(c2.data(), wt[ps.par].counters, sizeof(int64_t) * CNT_N, ps.st));
+        t_stats.long_waves += 1;
+        t_stats.leaf_tasks += ps.cnt[CNT_LEAF];
+        t_stats.leaf_ip += ps.cnt[CNT_LEAF_IP];
+        t_stats.leaf_out += c2[CNT_LEAF_OUT];
+        t_stats.group_tasks += ps.ntask;
+        t_stats.group_rounds += c2[CNT_ROUNDS];
+        t_stats.group_in += c2[CNT_GRP_IN];
+        t_stats.group_out += c2[CNT_GRP_OUT];
+        t_stats.passes = std::max<int64_t>(t_stats.passes, ps.passes);
+        if (tr.on && bitmap) {
+            fprintf(stderr, "[brm] bitmap engine cycles by phase (sum over workgroups):");
+            for (int p = 0; p < N_PHASES; ++p) fprintf(stderr, " %.3g", (double)c2[CNT_PH0 + p]);
+            fprintf(stderr, "\n");
+        }
+        return BRM_OK;
+    };
+    PendingStats prev_stats;
 
-    for (const Wave& w : waves) {
+    for (size_t wi = 0; wi < waves.size(); ++wi) {
+        const Wave& w = waves[wi];
+        const int par = overlap ? (int)(wi & 1) : 0;
+        const hipStream_t ws = par ? wave2.stream() : s;
+        WaveTables& t = wt[par];
         const int nw = w.k1 - w.k0;
-        HIPC(hipMemcpyAsync(rows_w, rows.data() + w.k0, sizeof(int) * nw, hipMemcpyHostToDevice, s));
-        HIPC(hipMemcpyAsync(wsb_d, w.wsb.data(), sizeof(int64_t) * 2 * nw, hipMemcpyHostToDevice, s));
-        HIPC(hipMemcpyAsync(slb_d, w.slb.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice, s));
-        HIPC(hipMemcpyAsync(seg_d, w.seg.data(), sizeof(int64_t) * (MAX_PASSES + 2), hipMemcpyHostToDevice, s));
-        HIPC(launch_fill_i64(counters, CNT_N, 0, s));
-        HIPC(launch_fill_i64(slots, w.sl_tot, 0, s));
+        HIPC(hipMemcpyAsync(t.rows_w, rows.data() + w.k0, sizeof(int) * nw, hipMemcpyHostToDevice, ws));
+        HIPC(hipMemcpyAsync(t.wsb_d, w.wsb.data(), sizeof(int64_t) * 2 * nw, hipMemcpyHostToDevice, ws));
+        HIPC(hipMemcpyAsync(t.slb_d, w.slb.data(), sizeof(int64_t) * nw, hipMemcpyHostToDevice, ws));
+        HIPC(hipMemcpyAsync(t.seg_d, w.seg.data(), sizeof(int64_t) * (MAX_PASSES + 2), hipMemcpyHostToDevice, ws));
+        HIPC(launch_fill_i64(t.counters, CNT_N, 0, ws));
+        HIPC(launch_fill_i64(t.slots, w.sl_tot, 0, ws));
 
         LongArgs la;
-        la.rows = rows_w;
+        la.rows = t.rows_w;
         la.nrows = nw;
-        la.ws_base = wsb_d;
-        la.slot_base = slb_d;
+        la.ws_base = t.wsb_d;
+        la.slot_base = t.slb_d;
         la.arpt = a.arpt;
         la.acol = a.acol;
         la.aval = a.aval;
         la.brpt = a.brpt;
         la.ipA = a.ipA;
         la.final_off = final_off;
-        la.counters = counters;
+        la.counters = t.counters;
         la.leaves = leaves;
         la.leaves_cap = w.leaf_cap;
-        la.tasks = tasks;
-        la.task_key = tkey;
-        la.task_seg = seg_d;
-        la.children = children;
+        la.tasks = t.tasks;
+        la.task_key = t.tkey;
+        la.task_seg = t.seg_d;
+        la.children = t.children;
         la.child_cap = w.child_cap;
         la.frontier_cap = bitmap ? 0 : std::min(params().leaf_cap, sh.cap);
         la.lmax = sh.lmax;
         la.log_lmax = sh.log_lmax;
         la.pass_bits = pass_bits;
         tr.mark("wave uploaded", nw);
-        HIPC(launch_plan(la, derr, s));
+        HIPC(launch_plan(la, derr, ws));
         tr.mark("planned");
         std::vector<int64_t> cnt(CNT_N);
-        BRMC(d2h(cnt.data(), counters, sizeof(int64_t) * CNT_N, s));
-        BRMC(check_dev_err(derr, s, "long-row planner"));
+        BRMC(d2h(cnt.data(), t.counters, sizeof(int64_t) * CNT_N, ws));
+        BRMC(check_dev_err(derr, ws, "long-row planner"));
 
-        HIPC(launch_leaf(leaves, cnt[CNT_LEAF], a, wcol, wval, slots, counters, o, s));
+        HIPC(launch_leaf(leaves, cnt[CNT_LEAF], a, wcol, wval, t.slots, t.counters, o, ws));
         tr.mark("leaves", cnt[CNT_LEAF]);
         const int passes = (int)cnt[CNT_MAXP];
+        // the last pass that reads or writes buffer 1: passes j writing
+        // nodes below the root into buffer j & 1, reading buffer (j-1) & 1
+        int b1_last = 0;
+        for (int j = 1; j <= passes && j <= MAX_PASSES; ++j)
+            if ((j % 2 == 0) || j < passes) b1_last = j;
+        if (overlap && b1_last == 0) {
+            // no pass of this wave uses buffer 1: it is free once the
+            // previous wave has let it go
+            if (wi > 0) HIPC(hipStreamWaitEvent(ws, hand[par ^ 1].b1_free, 0));
+            HIPC(hipEventRecord(hand[par].b1_free, ws));
+        }
         int64_t ntask = 0;
         for (int j = 1; j <= passes && j <= MAX_PASSES; ++j) {
+            if (overlap && wi > 0 && j <= 2) HIPC(hipStreamWaitEvent(ws, j == 1 ? hand[par ^ 1].b1_free : hand[par ^ 1].done, 0));
             const int* order = nullptr;
-            if (tkey && cnt[CNT_TASK0 + j] > 1) {
-                HIPC(order_tasks(tkey + w.seg[j], (int)cnt[CNT_TASK0 + j], tkey_out, tidx_in, tidx_out, ttmp,
-                                 ttmp_bytes, s));
-                order = tidx_out;
+            if (t.tkey && cnt[CNT_TASK0 + j] > 1) {
+                HIPC(order_tasks(t.tkey + w.seg[j], (int)cnt[CNT_TASK0 + j], t.tkey_out, t.tidx_in, t.tidx_out, t.ttmp,
+                                 ttmp_bytes, ws));
+                order = t.tidx_out;
             }
-            HIPC(launch_group(tasks + w.seg[j], cnt[CNT_TASK0 + j], children, a, wcol, wval, slots, counters, o, s,
-                              group_cfg, order));
+            HIPC(launch_group(t.tasks + w.seg[j], cnt[CNT_TASK0 + j], t.children, a, wcol, wval, t.slots, t.counters, o,
+                              ws, group_cfg, order));
             ntask += cnt[CNT_TASK0 + j];
             tr.mark("group pass", cnt[CNT_TASK0 + j]);
+            if (overlap && j == b1_last) HIPC(hipEventRecord(hand[par].b1_free, ws));
         }
-        std::vector<int64_t> c2(CNT_N);
-        BRMC(d2h(c2.data(), counters, sizeof(int64_t) * CNT_N, s));
-        t_stats.long_waves += 1;
-        t_stats.leaf_tasks += cnt[CNT_LEAF];
-        t_stats.leaf_ip += cnt[CNT_LEAF_IP];
-        t_stats.leaf_out += c2[CNT_LEAF_OUT];
-        t_stats.group_tasks += ntask;
-        t_stats.group_rounds += c2[CNT_ROUNDS];
-        t_stats.group_in += c2[CNT_GRP_IN];
-        t_stats.group_out += c2[CNT_GRP_OUT];
-        t_stats.passes = std::max<int64_t>(t_stats.passes, passes);
-        if (tr.on && bitmap) {
-            fprintf(stderr, "[brm] bitmap engine cycles by phase (sum over workgroups):");
-            for (int p = 0; p < N_PHASES; ++p) fprintf(stderr, " %.3g", (double)c2[CNT_PH0 + p]);
-            fprintf(stderr, "\n");
+        if (overlap) {
+            // a wave of one pass still waits for the previous wave's end
+            if (wi > 0 && passes < 2) HIPC(hipStreamWaitEvent(ws, hand[par ^ 1].done, 0));
+            HIPC(hipEventRecord(hand[par].done, ws));
         }
+        // the previous wave's counters (its stream finishes it beside this
+        // wave's passes), or this wave's when waves do not overlap
+        BRMC(read_stats(prev_stats));
+        prev_stats.on = true;
+        prev_stats.par = par;
+        prev_stats.st = ws;
+        prev_stats.cnt = cnt;
+        prev_stats.ntask = ntask;
+        prev_stats.passes = passes;
+        if (!overlap) BRMC(read_stats(prev_stats));
         dev_flush_deferred();
     }
+    BRMC(read_stats(prev_stats));
+    BRMC(wave2.mark());
+    BRMC(wave2.join());
     return BRM_OK;
 }
 
@@ -960,64 +1106,6 @@ int run_small(const DevIn& in, double avg_ip, int64_t max_nnz, hipStream_t s, Ph
     return run_small_g(in, 64, max_nnz, s, clk, out, handled);
 }
 }  // namespace
-
-// A second stream of the calling thread (default priority, created once per
-// device and kept until the process ends) for work that runs beside the
-// main stream: fork() makes it wait for what the main stream has queued,
-// mark() records where its work ends, join() makes the main stream wait for
-// it.  When not `on`, everything runs on the main stream.  The destructor
-// waits for the side stream, so no early return leaves its kernels running
-// over buffers about to be released.
-class SideStream {
-  public:
-    SideStream(hipStream_t main, bool on) : main_(main), on_(on) {}
-    ~SideStream() {
-        if (forked_) (void)hipStreamSynchronize(side_);
-    }
-    hipStream_t stream() const { return forked_ ? side_ : main_; }
-    int fork() {
-        if (!on_) return BRM_OK;
-        int dev = 0;
-        HIPC(hipGetDevice(&dev));
-        static thread_local std::vector<std::pair<int, Res>> per_dev;
-        Res* r = nullptr;
-        for (auto& pd : per_dev)
-            if (pd.first == dev) r = &pd.second;
-        if (!r) {
-            Res n{};
-            HIPC(hipStreamCreateWithFlags(&n.side, hipStreamNonBlocking));
-            HIPC(hipEventCreateWithFlags(&n.e0, hipEventDisableTiming));
-            HIPC(hipEventCreateWithFlags(&n.e1, hipEventDisableTiming));
-            per_dev.push_back({dev, n});
-            r = &per_dev.back().second;
-        }
-        side_ = r->side;
-        e1_ = r->e1;
-        HIPC(hipEventRecord(r->e0, main_));
-        HIPC(hipStreamWaitEvent(side_, r->e0, 0));
-        forked_ = true;
-        return BRM_OK;
-    }
-    int mark() {
-        if (forked_) HIPC(hipEventRecord(e1_, side_));
-        return BRM_OK;
-    }
-    int join() {
-        if (forked_) HIPC(hipStreamWaitEvent(main_, e1_, 0));
-        return BRM_OK;
-    }
-
-  private:
-    struct Res {
-        hipStream_t side;
-        hipEvent_t e0, e1;
-    };
-    hipStream_t main_;
-    bool on_;
-    hipStream_t side_ = nullptr;
-    hipEvent_t e1_ = nullptr;
-    bool forked_ = false;
-};
 
 int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevOut* out, double* ph,
                const ExtOut* ext) {

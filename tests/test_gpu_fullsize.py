@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests._util import as_tuple, assert_csr_equal, bits
+from tests._util import csr_from_tuple, as_tuple, assert_csr_equal, bits
 
 pytestmark = pytest.mark.gpu
 
@@ -138,6 +138,35 @@ def test_rmat15_many_waves(knobs, engine, vm):
     R2 = oracle.brmerge_precise(as_tuple(A2), as_tuple(A2))
     assert_csr_equal(gpu.brmerge_upper(A2, A2), R2)
     assert gpu.last_stats()["upper_waves"] == int((row_ip(A2, A2) > 0).sum())
+
+
+@pytest.mark.parametrize("budget", [1, 3 << 20], ids=["one-row-waves", "few-row-waves"])
+def test_overlapped_waves_mixed_passes(knobs, budget):
+    """Consecutive long-row waves overlap on two streams (bitmap engine),
+    handing the workspace buffers over by events.  Rows of one, two and three
+    passes (64, 65 to 4,096 and more than 4,096 lists) in mixed order over many
+    waves: every handover order between a wave and the next one occurs."""
+    gpu = knobs
+    gpu.set_param("group_cfg", 4)
+    gpu.set_param("force_long", 1)
+    rng = np.random.default_rng(41)
+    K, N = 6000, 400_000
+    lens = rng.integers(0, 9, K)
+    brpt = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    bcol = np.concatenate([np.sort(rng.choice(N, size=int(d), replace=False)) for d in lens]).astype(np.int32)
+    B = (brpt, bcol, rng.uniform(-1, 1, brpt[-1]), N)
+    sizes = (4500, 64, 3000, 65, 5000, 40, 4097, 1, 200, 4096, 5900, 2, 64)
+    acols = [np.sort(rng.choice(K, size=sz, replace=False)) for sz in sizes]
+    arpt = np.concatenate([[0], np.cumsum([len(a) for a in acols])]).astype(np.int32)
+    A = (arpt, np.concatenate(acols).astype(np.int32), rng.uniform(-1, 1, arpt[-1]), K)
+    R = oracle.brmerge_precise(A, B)
+    gpu.set_param("ws_budget_bytes", budget)
+    Ah, Bh = csr_from_tuple(A), csr_from_tuple(B)
+    for _ in range(2):
+        assert_csr_equal(gpu.brmerge_precise(Ah, Bh), R)
+        st = gpu.last_stats()
+        assert st["long_waves"] >= (4 if budget == 1 else 2) and st["passes"] == 3
+    assert_csr_equal(gpu.brmerge_upper(Ah, Bh), R)
 
 
 @pytest.mark.parametrize("ncols", [(1 << 20) + (1 << 19), 3 * (1 << 20) + 5], ids=["N1.5M", "N3M"])
