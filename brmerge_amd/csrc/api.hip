@@ -931,15 +931,21 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
         o.ccol = bcol_bar - wbase[w];
         o.cval = bval_bar - wbase[w];
         o.rowlen = p.rowcnt + wb[w];
-        HIPC(launch_short_numeric(rows(BIN_TINY), bc[BIN_TINY], aw, o, s, 32));
-        HIPC(launch_short_numeric(rows(BIN_SHORT), bc[BIN_SHORT], aw, o, s));
-        HIPC(launch_wmed_numeric(rows(BIN_WMED), bc[BIN_WMED], aw, o, s, std::min(p.wmed_cap, WMED_SPLIT0)));
-        HIPC(launch_wmed_numeric(rows(BIN_WMED2), bc[BIN_WMED2], aw, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
-        HIPC(launch_wmed_numeric(rows(BIN_WMED3), bc[BIN_WMED3], aw, o, s, p.wmed_cap));
-        HIPC(launch_medium_numeric(rows(BIN_MEDIUM), bc[BIN_MEDIUM], aw, o, s, p.lds_cfg));
-        HIPC(launch_hash_numeric(rows(BIN_HASH), bc[BIN_HASH], aw, o, s, HASH_SMALL));
-        HIPC(launch_hash_numeric(rows(BIN_HASH2), bc[BIN_HASH2], aw, o, s, HASH_CAP));
+        // the short and medium tiers beside the long-row engine, as for Precise
+        SideStream side(s, bc[BIN_LONG] > 0 && arena_remaining() < 0);
+        BRMC(side.fork());
+        const hipStream_t ts = side.stream();
+        HIPC(launch_short_numeric(rows(BIN_TINY), bc[BIN_TINY], aw, o, ts, 32));
+        HIPC(launch_short_numeric(rows(BIN_SHORT), bc[BIN_SHORT], aw, o, ts));
+        HIPC(launch_wmed_numeric(rows(BIN_WMED), bc[BIN_WMED], aw, o, ts, std::min(p.wmed_cap, WMED_SPLIT0)));
+        HIPC(launch_wmed_numeric(rows(BIN_WMED2), bc[BIN_WMED2], aw, o, ts, std::min(p.wmed_cap, WMED_SPLIT)));
+        HIPC(launch_wmed_numeric(rows(BIN_WMED3), bc[BIN_WMED3], aw, o, ts, p.wmed_cap));
+        HIPC(launch_medium_numeric(rows(BIN_MEDIUM), bc[BIN_MEDIUM], aw, o, ts, p.lds_cfg));
+        HIPC(launch_hash_numeric(rows(BIN_HASH), bc[BIN_HASH], aw, o, ts, HASH_SMALL));
+        HIPC(launch_hash_numeric(rows(BIN_HASH2), bc[BIN_HASH2], aw, o, ts, HASH_CAP));
+        BRMC(side.mark());
         BRMC(run_long(aw, rows(BIN_LONG), bc[BIN_LONG], o, nullptr, s, sc));
+        BRMC(side.join());
         BRMC(check_dev_err(p.derr, s, "numeric"));
         if (nwaves == 1) {
             clk.begin(P_PNNZ);
