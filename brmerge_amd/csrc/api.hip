@@ -722,6 +722,31 @@ int pattern_bin(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int force_lo
 
 int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_short, bool read_nnz) {
     if (clk) clk->begin(P_SYM);
+    // Long rows, largest first (the heaviest rows' counts start early; the
+    // long-row engine later takes them in this order too), counted on a side
+    // stream beside the other tiers' counts.
+    const int nlong = p.bc[BIN_LONG];
+    const bool own_mem = arena_remaining() < 0;
+    // (tsc's blocks are reused only by work ordered after the join below)
+    Scratch tsc;
+    SideStream side(s, nlong > 0 && own_mem);
+    BRMC(side.fork());
+    if (nlong > 1 && own_mem) {
+        const size_t tb = order_rows_tmp_bytes(nlong);
+        if (tb > 0) {
+            uint64_t *k0, *k1;
+            int* ro;
+            unsigned char* tmp;
+            BRMC(tsc.alloc(&k0, nlong, "long-row keys"));
+            BRMC(tsc.alloc(&k1, nlong, "long-row keys"));
+            BRMC(tsc.alloc(&ro, nlong, "long-row order"));
+            BRMC(tsc.alloc(&tmp, (int64_t)tb, "long-row sort"));
+            HIPC(order_rows(const_cast<int*>(p.rows(BIN_LONG)), nlong, p.a.arpt, p.a.ipA, k0, k1, ro, tmp, tb,
+                            side.stream()));
+        }
+    }
+    HIPC(launch_long_count(p.rows(BIN_LONG), nlong, p.a, p.rowcnt, side.stream()));
+    BRMC(side.mark());
     const int64_t n32 = (int64_t)p.bc[BIN_TINY] * 32, n64 = (int64_t)p.bc[BIN_SHORT] * 64;
     p.stashed = stash_short && n32 + n64 > 0 && 12 * (n32 + n64) <= SHORT_STASH_MAX_BYTES;
     if (p.stashed) {
@@ -751,7 +776,7 @@ int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_shor
     HIPC(launch_medium_count(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, p.rowcnt, s));
     HIPC(launch_hash_count(p.rows(BIN_HASH), p.bc[BIN_HASH], p.a, p.rowcnt, s, HASH_SMALL));
     HIPC(launch_hash_count(p.rows(BIN_HASH2), p.bc[BIN_HASH2], p.a, p.rowcnt, s, HASH_CAP));
-    HIPC(launch_long_count(p.rows(BIN_LONG), p.bc[BIN_LONG], p.a, p.rowcnt, s));
+    BRMC(side.join());
     if (clk) clk->begin(P_PNNZ);
     HIPC(scan_i64(p.rowcnt, p.a.M, p.crpt, s));
     p.nnzC = -1;

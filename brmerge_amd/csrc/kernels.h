@@ -70,6 +70,11 @@ hipError_t launch_group(const GroupTask* tasks, int64_t n, const Child* children
 // sorted by keys descending (keys_out receives the sorted keys).  tmp of
 // order_tasks_tmp_bytes(n) bytes.
 size_t order_tasks_tmp_bytes(int n);
+// Rows sorted in place by product count, largest first (keys, keys_out and
+// rows_out: n entries each; tmp of order_rows_tmp_bytes(n) bytes).
+size_t order_rows_tmp_bytes(int n);
+hipError_t order_rows(int* rows, int n, const int* arpt, const int64_t* ipA, uint64_t* keys, uint64_t* keys_out,
+                      int* rows_out, void* tmp, size_t tmp_bytes, hipStream_t s);
 hipError_t order_tasks(const uint32_t* keys, int n, uint32_t* keys_out, int* idx_in, int* idx_out, void* tmp,
                        size_t tmp_bytes, hipStream_t s);
 

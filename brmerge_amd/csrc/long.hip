@@ -579,6 +579,39 @@ __global__ void k_iota(int* __restrict__ p, int n) {
 }
 }  // namespace
 
+namespace {
+__global__ void k_row_ip_keys(const int* __restrict__ rows, int n, const int* __restrict__ arpt,
+                              const int64_t* __restrict__ ipA, uint64_t* __restrict__ keys) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int r = rows[i];
+    keys[i] = (uint64_t)(ipA[arpt[r + 1]] - ipA[arpt[r]]);
+}
+}  // namespace
+
+size_t order_rows_tmp_bytes(int n) {
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                     (const int*)nullptr, (int*)nullptr, n, 0, 64) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return bytes;
+}
+
+hipError_t order_rows(int* rows, int n, const int* arpt, const int64_t* ipA, uint64_t* keys, uint64_t* keys_out,
+                      int* rows_out, void* tmp, size_t tmp_bytes, hipStream_t s) {
+    if (n <= 1) return hipSuccess;
+    k_row_ip_keys<<<(n + 255) / 256, 256, 0, s>>>(rows, n, arpt, ipA, keys);
+    if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;
+    size_t bytes = tmp_bytes;
+    if (hipError_t e = hipcub::DeviceRadixSort::SortPairsDescending(tmp, bytes, keys, keys_out, rows, rows_out, n, 0,
+                                                                    64, s);
+        e != hipSuccess)
+        return e;
+    return hipMemcpyAsync(rows, rows_out, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, s);
+}
+
 size_t order_tasks_tmp_bytes(int n) {
     size_t bytes = 0;
     if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
