@@ -40,11 +40,14 @@ thread_local std::string t_err;
 thread_local brm_stats t_stats;
 thread_local int t_device = 0;
 thread_local int64_t t_ws_budget = 0;
+thread_local bool t_side_streams = true;
 }  // namespace
 
 brm_stats& thread_stats() { return t_stats; }
 int64_t thread_ws_budget() { return t_ws_budget; }
 void set_thread_ws_budget(int64_t bytes) { t_ws_budget = bytes; }
+bool thread_side_streams() { return t_side_streams; }
+void set_thread_side_streams(bool on) { t_side_streams = on; }
 int thread_device() { return t_device; }
 void set_thread_device(int device) { t_device = device; }
 
@@ -229,7 +232,7 @@ int ceil_log2_host(int x) {
 // leaves its kernels running over buffers about to be released.
 class SideStream {
   public:
-    SideStream(hipStream_t main, bool on, int slot = 0) : main_(main), on_(on), slot_(slot) {}
+    SideStream(hipStream_t main, bool on, int slot = 0) : main_(main), on_(on && thread_side_streams()), slot_(slot) {}
     ~SideStream() {
         if (forked_) (void)hipStreamSynchronize(side_);
     }
@@ -465,7 +468,7 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     // 1, its pass 2 (writes buffer 0) for the end of wave w.  Wave w's last
     // pass, the roots of its heaviest rows, then runs beside wave w+1's
     // planning and first pass.
-    const bool overlap = bitmap && arena < 0 && ttmp_bytes > 0 && waves.size() > 1;
+    const bool overlap = bitmap && arena < 0 && ttmp_bytes > 0 && waves.size() > 1 && thread_side_streams();
     WaveTables wt[2];
     for (int par = 0; par < (overlap ? 2 : 1); ++par) {
         WaveTables& t = wt[par];
@@ -727,13 +730,12 @@ int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_shor
     // stream beside the other tiers' counts.
     const int nlong = p.bc[BIN_LONG];
     const bool own_mem = arena_remaining() < 0;
-    // (tsc's blocks are reused only by work ordered after the join below)
-    Scratch tsc;
     SideStream side(s, nlong > 0 && own_mem);
     BRMC(side.fork());
     if (nlong > 1 && own_mem) {
         const size_t tb = order_rows_tmp_bytes(nlong);
         if (tb > 0) {
+            Scratch tsc;
             uint64_t *k0, *k1;
             int* ro;
             unsigned char* tmp;
@@ -743,6 +745,12 @@ int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_shor
             BRMC(tsc.alloc(&tmp, (int64_t)tb, "long-row sort"));
             HIPC(order_rows(const_cast<int*>(p.rows(BIN_LONG)), nlong, p.a.arpt, p.a.ipA, k0, k1, ro, tmp, tb,
                             side.stream()));
+            // the queued sort still uses these: released when the call ends,
+            // after its final synchronisation
+            for (void* q : {(void*)k0, (void*)k1, (void*)ro, (void*)tmp}) {
+                tsc.release(q);
+                dev_defer_free(q);
+            }
         }
     }
     HIPC(launch_long_count(p.rows(BIN_LONG), nlong, p.a, p.rowcnt, side.stream()));

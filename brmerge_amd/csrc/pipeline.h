@@ -183,6 +183,12 @@ void set_thread_device(int device);
 // device's free memory at each call).
 int64_t thread_ws_budget();
 void set_thread_ws_budget(int64_t bytes);
+// Whether the calling thread's calls may put work on side streams beside
+// their own (the request server's workers do not: several workers with two
+// side streams each would outnumber the GPU's hardware queues, and streams
+// sharing a queue wait on each other).
+bool thread_side_streams();
+void set_thread_side_streams(bool on);
 // Device address of the no-allocation entry's fixed scratch on `device`
 // (the current device), resolved once per device.
 int noalloc_setup(int device);

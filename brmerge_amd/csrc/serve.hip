@@ -204,6 +204,7 @@ int run_batch(brm_server* S, std::vector<std::shared_ptr<Request>>& batch, hipSt
 void worker_main(brm_server* S) {
     set_thread_device(S->device);
     set_thread_ws_budget(S->ws_share);
+    set_thread_side_streams(false);  // the workers' own streams run beside each other
     if (use_device() != BRM_OK) return;
     hipStream_t s = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return;
