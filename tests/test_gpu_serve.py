@@ -99,3 +99,38 @@ def test_server_rejects_bad_requests(gpu):
         ok.free()
     dA.free()
     dB.free()
+
+
+def test_server_small_batches_with_long_rows(gpu):
+    """Many small batches (below 2^24 products, so the pipeline skips its
+    read of nnz(C) after the symbolic phase) whose requests have long rows
+    (their rows are sorted by product count before the long-row count), on
+    two workers at once: every result equals its single call.  The sort's
+    scratch must stay out of the allocator's cache until each call is done,
+    since the other worker allocates meanwhile."""
+    B = gpu.gen.rmat(12, seed=8)
+    dB = gpu.DeviceCSR.upload(B)
+    rng = np.random.default_rng(17)
+    reqs = []
+    for i in range(48):
+        M = int(rng.integers(20, 120))
+        # rows of 300 to 2,000 lists over B's rows: long rows of few products each
+        lens = rng.integers(300, 2000, size=M)
+        rpt = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        col = np.concatenate([np.sort(rng.choice(B.M, size=int(n), replace=False)) for n in lens]).astype(np.int32)
+        reqs.append(gpu.CSR(rpt, col, rng.uniform(-1, 1, len(col)), M, B.M))
+    dAs = [gpu.DeviceCSR.upload(A) for A in reqs]
+    ips = [gpu.compute_flop(A, B) for A in reqs]
+    assert max(ips) < (1 << 24) // 2
+    with gpu.Server(budget_bytes=8 << 30, max_batch=2, streams=2) as srv:
+        b = srv.add_b(dB)
+        rids = [srv.submit(b, dA) for dA in dAs]
+        for k, rid in enumerate(rids):
+            srv.wait(rid)
+            C = srv.result(rid)
+            alone = gpu.spgemm_dev(dAs[k], dB)
+            assert_csr_equal(C, alone.download())
+            alone.free()
+            srv.release(rid)
+    for d in dAs + [dB]:
+        d.free()
