@@ -177,25 +177,37 @@ __device__ __forceinline__ int bg_elem(int i) {
     return (int)(threadIdx.x / BRM_WAVE) * (BG_IPT * BRM_WAVE) + i * BRM_WAVE + lane_id();
 }
 
-// Proportional windows for the next round: everything left if it fits,
-// else a share of BG_W - BG_KMAX proportional to what each child has left
-// (at least one element).  Returns true when nothing is left.  The windows
-// are published by the caller's next barrier.
+// Children with at most BG_WHOLE elements left load all of them: a partly
+// loaded child bounds the round at its last loaded column, and a short child
+// given a proportional share of one or two elements would bound it early.
+constexpr int BG_WHOLE = 16;
+static_assert(BG_KMAX * BG_WHOLE <= BG_W - BG_KMAX, "whole short children fit the window");
+
+// Window of child t (rem elements left, R left over all children) for the
+// next round: everything left if it all fits; a short child's whole rest;
+// else a share of what the short children leave of BG_W - BG_KMAX,
+// proportional to what the child has left (at least one element).  Called by
+// all of wavefront 0, which holds every child (k <= 64).
+__device__ __forceinline__ int bg_share(int64_t rem, int64_t R) {
+    const bool whole = rem > 0 && rem <= BG_WHOLE;
+    const int64_t ws = wave_sum(whole ? rem : (int64_t)0);
+    if (rem <= 0) return 0;
+    if (R <= BG_W || whole) return (int)rem;
+    int64_t x = (int64_t)((double)(BG_W - BG_KMAX - ws) * (double)rem / (double)(R - ws));
+    if (x < 1) x = 1;
+    if (x > rem) x = rem;
+    return (int)x;
+}
+
+// Windows for the first round (bg_share).  Returns true when nothing is
+// left.  The windows are published by the caller's next barrier.
 __device__ __forceinline__ bool bg_windows(BgSmem& g, Slots& sl, int k) {
+    static_assert(BG_KMAX <= BRM_WAVE, "every child in wavefront 0");
     const int t = threadIdx.x;
     const int64_t rem = t < k ? g.len[t] - g.cur[t] : 0;
     const int64_t R = block_sum_l(g, sl, rem);
     int wi = 0;
-    if (rem > 0) {
-        if (R <= BG_W) {
-            wi = (int)rem;
-        } else {
-            int64_t x = (int64_t)((double)(BG_W - BG_KMAX) * (double)rem / (double)R);
-            if (x < 1) x = 1;
-            if (x > rem) x = rem;
-            wi = (int)x;
-        }
-    }
+    if (t < BRM_WAVE) wi = bg_share(rem, R);
     // one scan gives the offsets (low 16 bits) and the rank among children
     // with a window (high bits)
     int tot;
@@ -737,21 +749,10 @@ __global__ __launch_bounds__(BG_NT, 3) void k_bgroup(const GroupTask* __restrict
         __syncthreads();
         mark(4);
         // ---- bucket offsets (exclusive scan of the counts) and, in the
-        // same scan, the next round's proportional windows: everything left
-        // if it fits, else a share of BG_W - BG_KMAX proportional to what
-        // each child has left (at least one element)
+        // same scan, the next round's windows (bg_share)
         {
             int wi = 0;
-            if (remt > 0) {
-                if (R <= BG_W) {
-                    wi = (int)remt;
-                } else {
-                    int64_t x = (int64_t)((double)(BG_W - BG_KMAX) * (double)remt / (double)R);
-                    if (x < 1) x = 1;
-                    if (x > remt) x = remt;
-                    wi = (int)x;
-                }
-            }
+            if (t < BRM_WAVE) wi = bg_share(remt, R);
             int loc[BG_IPT];
             int sloc = 0;
 #pragma unroll
