@@ -2,12 +2,13 @@
 // products held in registers (R per lane, ip <= 64 R <= 1024, at most 64 lists).
 //
 //   1. every lane l < L holds list l's product offset, B row start and A
-//      value, and writes the B index and list position of each of its
-//      list's products to an LDS map; each product reads its entry, loads
-//      its B entry and forms key = (column, list position, index) and
-//      value = a * b (rounded, BRMerge-SpGEMM/inc/spgemm.h:621).  Keys are
-//      32-bit when the column fits beside 6 bits of list position and
-//      log2(64 R) bits of index (half the shuffles and compares), else 64-bit.
+//      value; the non-empty lists mark their first product in an LDS bitmap
+//      and describe themselves in an LDS table, so each product finds its
+//      list by a popcount rank, loads its B entry and forms
+//      key = (column, list position, index) and value = a * b (rounded,
+//      BRMerge-SpGEMM/inc/spgemm.h:621).  Keys are 32-bit when the column
+//      fits beside 6 bits of list position and log2(64 R) bits of index
+//      (half the shuffles and compares), else 64-bit.
 //   Sort variant:
 //   2. a bitonic sort over the 64 R keys (cross-lane steps with shuffles,
 //      intra-lane steps between registers) orders the products by column,
@@ -21,8 +22,9 @@
 //      to C-bar with the row length (Upper).
 //   Bitmap variant (B of at most RB_SPAN columns, R >= 8): the columns set
 //   bits of an LDS bitmap over the row's column span, popcount ranks give
-//   each product its output position, products are bucketed by position and
-//   each bucket is folded in list-position order -- no sort.
+//   each product its output position, products (still in registers) are
+//   bucketed by position into the bitmap's LDS, and each bucket is folded in
+//   list-position order -- no sort.
 // The symbolic count of these rows uses the wavefront hash kernel of lds.hip.
 #include <limits.h>
 
@@ -70,28 +72,46 @@ struct RmSmem {
     KT key[64 * R];
 };
 
+// Stage 1: the row's non-empty lists, k-th at ent[k] (first product, list
+// position, B row start, A value); bit j of start is set when a non-empty
+// list begins at product j.  Overlays the start of each variant's LDS.
+struct RmListEnt {
+    int off_lp;  // first product | list position << 16
+    int bst;
+    double sc;
+};
+struct RmLists {
+    unsigned long long start[16];
+    RmListEnt ent[64];
+};
+static_assert(sizeof(RmLists) <= sizeof(RmSmem<2, uint32_t>), "list table fits the smallest variant");
+
 // Bitmap variant (B with at most RB_SPAN columns): the row's columns set
 // bits of an LDS bitmap, popcount ranks give every product its
 // output position, products are bucketed by position and each bucket is
 // summed in list-position order -- O(ip) work instead of a sort.
+// Phase A (ranks) and phase B (buckets) share the LDS; the products stay in
+// registers until phase B places them.
 constexpr int RB_WORDS = 1024;
 constexpr int RB_SPAN = RB_WORDS * 64;
-template <int R>
-struct RbSmem {
-    double val[64 * R];
-    union {
-        unsigned long long bm[RB_WORDS];
-        struct {
-            int cnt[64 * R];
-            int outcol[64 * R];
-        } c;
-    } u;
-    union {
-        unsigned short wrank[RB_WORDS];
-        unsigned short contrib[64 * R];  // list position << 10 | product index
-    } w;
+struct RbRanks {
+    unsigned long long bm[RB_WORDS];
+    unsigned short wrank[RB_WORDS];
 };
-static_assert(64 * 16 <= 1024, "product index in 10 bits");
+template <int R>
+struct RbBuckets {
+    double val[64 * R];        // contributions, bucket by bucket
+    unsigned cnt[32 * R];      // 16-bit bucket counters, two per word (bucket 2i low, 2i + 1 high)
+    unsigned short oc[64 * R]; // bucket column minus the span's first column
+    unsigned char lp[64 * R];  // list position of each contribution
+};
+template <int R>
+union RbSmem {
+    RbRanks a;
+    RbBuckets<R> b;
+    RmLists l;
+};
+static_assert(64 * 16 <= 0xffff, "bucket offsets in 16 bits");
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -114,27 +134,16 @@ __device__ __forceinline__ int wave_incl_scan_dpp(int x) {
     return x;
 }
 
-// Sum of one bucket (m >= 2 contributions from distinct lists) in the
-// aligned tree over list positions: leaves are visited in increasing list
-// position (the bits of `mask`), each found in the bucket by a scan; the
-// fold keeps one pending left subtree per height (siblings added left +
-// right, a node with one present child passes it through).
-__device__ double rb_bucket_sum(const unsigned short* c, int m, const double* val) {
-    unsigned long long mask = 0ull;
-    for (int t = 0; t < m; ++t) mask |= 1ull << (c[t] >> 10);
-    auto leaf = [&](int l) {
-        int t = 0;
-        while ((c[t] >> 10) != l) ++t;
-        return val[c[t] & 1023];
-    };
-    int prev = __ffsll((long long)mask) - 1;
-    mask &= mask - 1ull;
-    double cur = leaf(prev);
+// Aligned-tree fold over leaves visited in increasing list position: keeps
+// one pending left subtree per height (siblings added left + right, a node
+// with one present child passes it through).
+struct TreeFold {
     double lv[RM_LOGL];
+    double cur;
     unsigned pend = 0u;
-    while (mask) {
-        const int l = __ffsll((long long)mask) - 1;
-        mask &= mask - 1ull;
+    int prev;
+    __device__ TreeFold(int l, double v) : cur(v), prev(l) {}
+    __device__ void push(int l, double x) {
         const int d = 31 - __clz(prev ^ l);
         double v = cur;
 #pragma unroll
@@ -144,24 +153,61 @@ __device__ double rb_bucket_sum(const unsigned short* c, int m, const double* va
         for (int h = 0; h < RM_LOGL; ++h)
             if (h == d) lv[h] = v;
         pend = (pend & ~((1u << d) - 1u)) | (1u << d);
-        cur = leaf(l);
+        cur = x;
         prev = l;
     }
-    double v = cur;
+    __device__ double finish() const {
+        double v = cur;
 #pragma unroll
-    for (int h = 0; h < RM_LOGL; ++h)
-        if ((pend >> h) & 1u) v = fadd(lv[h], v);
-    return v;
+        for (int h = 0; h < RM_LOGL; ++h)
+            if ((pend >> h) & 1u) v = fadd(lv[h], v);
+        return v;
+    }
+};
+
+// Sum of one bucket (m >= 3 contributions from distinct lists, list
+// positions lp[0..m), values val[0..m)).  Contributions placed by one
+// register step of the wavefront may be out of list order; one pass folds
+// a bucket that is in order, others visit their leaves by a search.
+__device__ double rb_bucket_sum(const unsigned char* lp, const double* val, int m) {
+    {
+        TreeFold f(lp[0], val[0]);
+        int t = 1;
+        for (; t < m; ++t) {
+            const int l = lp[t];
+            if (l < f.prev) break;
+            f.push(l, val[t]);
+        }
+        if (t == m) return f.finish();
+    }
+    unsigned long long mask = 0ull;
+    for (int t = 0; t < m; ++t) mask |= 1ull << lp[t];
+    auto leaf = [&](int l) {
+        int t = 0;
+        while (lp[t] != l) ++t;
+        return val[t];
+    };
+    int l = __ffsll((long long)mask) - 1;
+    mask &= mask - 1ull;
+    TreeFold f(l, leaf(l));
+    while (mask) {
+        l = __ffsll((long long)mask) - 1;
+        mask &= mask - 1ull;
+        f.push(l, leaf(l));
+    }
+    return f.finish();
 }
 
 // Columns index the bitmap directly (B has at most RB_SPAN columns, nw
-// words of them in use).
+// words of them in use).  prod[r] is the value of key[r]'s product.
 template <int R, typename KT>
-__device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t base, const OutArgs& o,
-                            RbSmem<R>& b) {
+__device__ void rmed_bitmap(const KT (&key)[R], const double (&prod)[R], int ip, int nw, int row, int64_t base,
+                            const OutArgs& o, RbSmem<R>& sh) {
     using Key = RmKey<R, KT>;
     constexpr int WPL = RB_WORDS / BRM_WAVE;  // bitmap words per lane, at most
     const int lane = lane_id();
+    RbRanks& A = sh.a;
+    RbBuckets<R>& B = sh.b;
     // only the words between the row's smallest and largest column are used
     int cmn = INT_MAX, cmx = 0;
 #pragma unroll
@@ -180,17 +226,18 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
     // words w0 + 64 i + lane (consecutive lanes on consecutive words)
     const int w0 = cmn >> 6, w1 = min((cmx >> 6) + 1, nw);
     const int nit = (w1 - w0 + BRM_WAVE - 1) / BRM_WAVE;
+    const int cbase = w0 * 64;  // columns of the bitmap's span are cbase + [0, 2^16)
 #pragma unroll
     for (int i = 0; i < WPL; ++i) {
         const int w = w0 + i * BRM_WAVE + lane;
-        if (i < nit && w < w1) b.u.bm[w] = 0ull;
+        if (i < nit && w < w1) A.bm[w] = 0ull;
     }
     wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (r * 64 + lane < ip) {
             const int bit = (int)Key::col(key[r]);
-            atomicOr(&b.u.bm[bit >> 6], 1ull << (bit & 63));
+            atomicOr(&A.bm[bit >> 6], 1ull << (bit & 63));
         }
     }
     wave_lds_sync();
@@ -199,9 +246,9 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
     for (int i = 0; i < WPL; ++i) {
         if (i >= nit) break;  // uniform
         const int w = w0 + i * BRM_WAVE + lane;
-        const int pc = w < w1 ? __popcll(b.u.bm[w]) : 0;
+        const int pc = w < w1 ? __popcll(A.bm[w]) : 0;
         const int inc = wave_incl_scan_dpp(pc);
-        if (w < w1) b.w.wrank[w] = (unsigned short)(nout + inc - pc);
+        if (w < w1) A.wrank[w] = (unsigned short)(nout + inc - pc);
         nout += __shfl(inc, BRM_WAVE - 1, BRM_WAVE);
     }
     wave_lds_sync();
@@ -212,31 +259,35 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
         if (r * 64 + lane < ip) {
             const int bit = (int)Key::col(key[r]);
             const int wd = bit >> 6;
-            pos[r] = b.w.wrank[wd] + __popcll(b.u.bm[wd] & ((1ull << (bit & 63)) - 1ull));
+            pos[r] = A.wrank[wd] + __popcll(A.bm[wd] & ((1ull << (bit & 63)) - 1ull));
         }
     }
     wave_lds_sync();  // bitmap and ranks are dead from here on
+    // bucket counters: 16 bits each, buckets 2i and 2i + 1 in word i
+    const int ncw = (nout + 1) >> 1;
 #pragma unroll
-    for (int i = 0; i < R; ++i)
-        if (i * BRM_WAVE < nout) b.u.c.cnt[i * BRM_WAVE + lane] = 0;
+    for (int i = 0; i < R / 2; ++i)
+        if (i * BRM_WAVE + lane < ncw) B.cnt[i * BRM_WAVE + lane] = 0u;
     wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (r * 64 + lane < ip) {
-            atomicAdd(&b.u.c.cnt[pos[r]], 1);
-            b.u.c.outcol[pos[r]] = (int)Key::col(key[r]);  // every writer stores the same column
+            atomicAdd(&B.cnt[pos[r] >> 1], 1u << ((pos[r] & 1) << 4));
+            B.oc[pos[r]] = (unsigned short)((int)Key::col(key[r]) - cbase);  // every writer stores the same column
         }
     }
     wave_lds_sync();
     {
-        int carry = 0;  // bucket offsets: a scan per step of 64 buckets, carried
+        int carry = 0;  // bucket offsets: a scan per step of 128 buckets, carried
 #pragma unroll
-        for (int i = 0; i < R; ++i) {
-            if (i * BRM_WAVE >= nout) break;  // uniform
-            const int q = i * BRM_WAVE + lane;
-            const int c = b.u.c.cnt[q];
-            const int inc = wave_incl_scan_dpp(c);
-            b.u.c.cnt[q] = carry + inc - c;
+        for (int i = 0; i < R / 2; ++i) {
+            if (i * BRM_WAVE >= ncw) break;  // uniform
+            const int q2 = i * BRM_WAVE + lane;
+            const unsigned w = q2 < ncw ? B.cnt[q2] : 0u;
+            const int c0 = (int)(w & 0xffffu), t = c0 + (int)(w >> 16);
+            const int inc = wave_incl_scan_dpp(t);
+            const int ex = carry + inc - t;
+            if (q2 < ncw) B.cnt[q2] = (unsigned)ex | ((unsigned)(ex + c0) << 16);
             carry += __shfl(inc, BRM_WAVE - 1, BRM_WAVE);
         }
     }
@@ -244,12 +295,15 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (r * 64 + lane < ip) {
-            const int slot = atomicAdd(&b.u.c.cnt[pos[r]], 1);
-            b.w.contrib[slot] = (unsigned short)((Key::lp(key[r]) << 10) | (r * 64 + lane));
+            const unsigned sft = (unsigned)(pos[r] & 1) << 4;
+            const int slot = (int)((atomicAdd(&B.cnt[pos[r] >> 1], 1u << sft) >> sft) & 0xffffu);
+            B.val[slot] = prod[r];
+            B.lp[slot] = (unsigned char)Key::lp(key[r]);
         }
     }
     wave_lds_sync();
-    // cnt[q] is now the end of bucket q
+    // counter q is now the end of bucket q
+    auto bend = [&](int q) { return (int)((B.cnt[q >> 1] >> ((q & 1) << 4)) & 0xffffu); };
     int* dcol;
     double* dval;
     int64_t dcap;
@@ -264,21 +318,19 @@ __device__ void rmed_bitmap(const KT (&key)[R], int ip, int nw, int row, int64_t
         dcap = ip;
     }
     for (int q = lane; q < nout; q += BRM_WAVE) {
-        const int s = q ? b.u.c.cnt[q - 1] : 0;
-        const int m = b.u.c.cnt[q] - s;
-        const unsigned short* c = b.w.contrib + s;
+        const int s = q ? bend(q - 1) : 0;
+        const int m = bend(q) - s;
         double v;
         if (m == 1) {
-            v = b.val[c[0] & 1023];
+            v = B.val[s];
         } else if (m == 2) {
-            const bool first = (c[0] >> 10) < (c[1] >> 10);
-            const double x = b.val[c[0] & 1023], y = b.val[c[1] & 1023];
-            v = first ? fadd(x, y) : fadd(y, x);
+            const double x = B.val[s], y = B.val[s + 1];
+            v = B.lp[s] < B.lp[s + 1] ? fadd(x, y) : fadd(y, x);
         } else {
-            v = rb_bucket_sum(c, m, b.val);
+            v = rb_bucket_sum(B.lp + s, B.val + s, m);
         }
         if (q < dcap) {
-            dcol[q] = b.u.c.outcol[q];
+            dcol[q] = cbase + (int)B.oc[q];
             dval[q] = v;
         }
     }
@@ -334,9 +386,11 @@ __device__ __forceinline__ void wave_bitonic_sort(KT (&key)[R], int rr) {
 template <int R, typename KT, bool BITMAP>
 __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows, int nrows, RowArgs a, OutArgs o) {
     using Key = RmKey<R, KT>;
-    // one of the two layouts, sized for the variant (val first in both)
+    // one of the two layouts, sized for the variant; the list table of
+    // stage 1 overlays the start of either
     __shared__ typename std::conditional<BITMAP, RbSmem<R>, RmSmem<R, KT>>::type shm;
     RmSmem<R, KT>& sm = *reinterpret_cast<RmSmem<R, KT>*>(&shm);
+    RmLists& ls = *reinterpret_cast<RmLists*>(&shm);
     const int lane = lane_id();
     const int row = rows[blockIdx.x];
     const int a0 = a.arpt[row];
@@ -345,7 +399,8 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
     const int ip = (int)(a.ipA[a0 + L] - base);
 
     // ---- 1. list descriptors in lanes, products in registers
-    int off = INT_MAX, bst = 0, len = 0;
+    if (lane < R) ls.start[lane] = 0ull;
+    int off = 0, bst = 0, len = 0;
     double sc = 0.0;
     if (lane < L) {
         off = (int)(a.ipA[a0 + lane] - base);
@@ -353,29 +408,21 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
         bst = a.brpt[a.acol[a0 + lane]];
         sc = a.aval[a0 + lane];
     }
-    KT key[R];
     int bi[R], lp[R];
     double sv[R];
     {
-        // every list lane writes its products' B index and list position to
-        // LDS (areas unused until later: the bitmap area, or the keys and
-        // values of the sort), one read each
-        int* pbi;
-        unsigned char* plp;
-        if constexpr (BITMAP) {
-            static_assert(64 * R * 5 <= (int)sizeof(RbSmem<R>::u), "product map fits the bitmap area");
-            auto& b = *reinterpret_cast<RbSmem<R>*>(&shm);
-            pbi = reinterpret_cast<int*>(&b.u);
-            plp = reinterpret_cast<unsigned char*>(pbi + 64 * R);
-        } else {
-            pbi = reinterpret_cast<int*>(sm.key);
-            plp = reinterpret_cast<unsigned char*>(sm.val);
-        }
-        for (int k = 0; k < len; ++k) {
-            pbi[off + k] = bst + k;
-            plp[off + k] = (unsigned char)lane;
+        // the k-th non-empty list marks its first product and fills entry k;
+        // product j's list is then entry (non-empty lists starting at or
+        // before j) - 1
+        const uint64_t nzl = __ballot(len > 0);
+        wave_lds_sync();  // start words cleared
+        if (len > 0) {
+            atomicOr(&ls.start[off >> 6], 1ull << (off & 63));
+            ls.ent[__popcll(nzl & ((1ull << lane) - 1ull))] = RmListEnt{off | (lane << 16), bst, sc};
         }
         wave_lds_sync();
+        const uint64_t upto = ~0ull >> (63 - lane);  // bits 0..lane
+        int before = 0;                               // non-empty lists starting before product 64 r
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = r * 64 + lane;
@@ -383,35 +430,38 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
             bi[r] = 0;
             sv[r] = 0.0;
             if (r * 64 >= ip) continue;  // uniform
+            const uint64_t w = ls.start[r];
             if (j < ip) {
-                bi[r] = pbi[j];
-                lp[r] = plp[j];
+                const RmListEnt e = ls.ent[before + __popcll(w & upto) - 1];
+                lp[r] = e.off_lp >> 16;
+                bi[r] = e.bst + j - (e.off_lp & 0xffff);
+                sv[r] = e.sc;
             }
-            sv[r] = __shfl(sc, lp[r], BRM_WAVE);
+            before += __popcll(w);
         }
-        wave_lds_sync();  // the map is read before its areas are reused
+        wave_lds_sync();  // the table is read before its area is reused
     }
-    double val[R];
+    KT key[R];
+    double prod[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = r * 64 + lane;
         key[r] = ~(KT)0;
-        val[r] = 0.0;
+        prod[r] = 0.0;
         if (j < ip) {
             key[r] = Key::make(a.bcol[bi[r]], lp[r], j);
-            val[r] = a.bval[bi[r]];
+            prod[r] = fmul(sv[r], a.bval[bi[r]]);
         }
+    }
+
+    // ---- B with at most RB_SPAN columns: bitmap ranks instead of a sort
+    if constexpr (BITMAP) {
+        rmed_bitmap<R, KT>(key, prod, ip, (a.N + 63) >> 6, row, base, o, shm);
+        return;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        if (r * 64 < ip) sm.val[r * 64 + lane] = fmul(sv[r], val[r]);
-
-    // ---- B with at most RB_SPAN columns: bitmap ranks instead of a sort
-    if (BITMAP) {
-        wave_lds_sync();  // the values above are read by other lanes
-        rmed_bitmap<R, KT>(key, ip, (a.N + 63) >> 6, row, base, o, *reinterpret_cast<RbSmem<R>*>(&shm));
-        return;
-    }
+        if (r * 64 < ip) sm.val[r * 64 + lane] = prod[r];
 
     // ---- 2. sort by (column, list position): only the registers the row uses
     int rr = 1;
