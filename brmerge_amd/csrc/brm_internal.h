@@ -193,6 +193,7 @@ struct Params {
     int hash_cap;              // rows of at most hash_cap products take the hash tier (0 = off, <= HASH_CAP)
     int upper_vm;              // Upper waves grow C in reserved address ranges (1) or copy chunks (0)
     int small_fused;           // products of few short rows in three launches (1) or the general pipeline (0)
+    int wmed_count;            // wavefront tier symbolic: 1 column bitmap (B of <= 65,536 columns), 0 hash
 };
 
 // Shapes of the LDS tree-merge kernels, selectable at run time.

@@ -390,6 +390,7 @@ hipError_t launch_medium_count(const int* rows, int n, const RowArgs& a, int64_t
 hipError_t launch_wmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s,
                              int wmed_cap) {
     if (n <= 0) return hipSuccess;
+    if (params().wmed_count == 1 && rmed_count_fits(a.N)) return launch_rmed_count(rows, n, a, nnzrow, s, wmed_cap);
     {
         ProfScope ps_("k_sym_hash_w", s);
         const int64_t ch = launch_chunk(BRM_WAVE);

@@ -198,6 +198,53 @@ __device__ double rb_bucket_sum(const unsigned char* lp, const double* val, int 
     return f.finish();
 }
 
+// Stage 1 of a row (A entries a0 .. a0 + L, products base .. base + ip):
+// product j = 64 r + lane (j < ip) reads B entry bi[r] of list lp[r], whose
+// A value is sv[r] (VALS).  The k-th non-empty list marks its first product
+// in a bitmap and fills table entry k; product j's list is then entry
+// (non-empty lists starting at or before j) - 1.  Ends with the table read
+// and its LDS free for reuse.
+template <int R, bool VALS>
+__device__ __forceinline__ void rm_products(const RowArgs& a, int a0, int L, int64_t base, int ip, RmLists& ls,
+                                            int (&bi)[R], int (&lp)[R], double (&sv)[R]) {
+    const int lane = lane_id();
+    if (lane < R) ls.start[lane] = 0ull;
+    int off = 0, bst = 0, len = 0;
+    double sc = 0.0;
+    if (lane < L) {
+        off = (int)(a.ipA[a0 + lane] - base);
+        len = (int)(a.ipA[a0 + lane + 1] - base) - off;
+        bst = a.brpt[a.acol[a0 + lane]];
+        if (VALS) sc = a.aval[a0 + lane];
+    }
+    const uint64_t nzl = __ballot(len > 0);
+    wave_lds_sync();  // start words cleared
+    if (len > 0) {
+        atomicOr(&ls.start[off >> 6], 1ull << (off & 63));
+        ls.ent[__popcll(nzl & ((1ull << lane) - 1ull))] = RmListEnt{off | (lane << 16), bst, sc};
+    }
+    wave_lds_sync();
+    const uint64_t upto = ~0ull >> (63 - lane);  // bits 0..lane
+    int before = 0;                               // non-empty lists starting before product 64 r
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = r * 64 + lane;
+        lp[r] = 0;
+        bi[r] = 0;
+        sv[r] = 0.0;
+        if (r * 64 >= ip) continue;  // uniform
+        const uint64_t w = ls.start[r];
+        if (j < ip) {
+            const RmListEnt e = ls.ent[before + __popcll(w & upto) - 1];
+            lp[r] = e.off_lp >> 16;
+            bi[r] = e.bst + j - (e.off_lp & 0xffff);
+            if (VALS) sv[r] = e.sc;
+        }
+        before += __popcll(w);
+    }
+    wave_lds_sync();  // the table is read before its area is reused
+}
+
 // Columns index the bitmap directly (B has at most RB_SPAN columns, nw
 // words of them in use).  prod[r] is the value of key[r]'s product.
 template <int R, typename KT>
@@ -399,48 +446,9 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
     const int ip = (int)(a.ipA[a0 + L] - base);
 
     // ---- 1. list descriptors in lanes, products in registers
-    if (lane < R) ls.start[lane] = 0ull;
-    int off = 0, bst = 0, len = 0;
-    double sc = 0.0;
-    if (lane < L) {
-        off = (int)(a.ipA[a0 + lane] - base);
-        len = (int)(a.ipA[a0 + lane + 1] - base) - off;
-        bst = a.brpt[a.acol[a0 + lane]];
-        sc = a.aval[a0 + lane];
-    }
     int bi[R], lp[R];
     double sv[R];
-    {
-        // the k-th non-empty list marks its first product and fills entry k;
-        // product j's list is then entry (non-empty lists starting at or
-        // before j) - 1
-        const uint64_t nzl = __ballot(len > 0);
-        wave_lds_sync();  // start words cleared
-        if (len > 0) {
-            atomicOr(&ls.start[off >> 6], 1ull << (off & 63));
-            ls.ent[__popcll(nzl & ((1ull << lane) - 1ull))] = RmListEnt{off | (lane << 16), bst, sc};
-        }
-        wave_lds_sync();
-        const uint64_t upto = ~0ull >> (63 - lane);  // bits 0..lane
-        int before = 0;                               // non-empty lists starting before product 64 r
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int j = r * 64 + lane;
-            lp[r] = 0;
-            bi[r] = 0;
-            sv[r] = 0.0;
-            if (r * 64 >= ip) continue;  // uniform
-            const uint64_t w = ls.start[r];
-            if (j < ip) {
-                const RmListEnt e = ls.ent[before + __popcll(w & upto) - 1];
-                lp[r] = e.off_lp >> 16;
-                bi[r] = e.bst + j - (e.off_lp & 0xffff);
-                sv[r] = e.sc;
-            }
-            before += __popcll(w);
-        }
-        wave_lds_sync();  // the table is read before its area is reused
-    }
+    rm_products<R, true>(a, a0, L, base, ip, ls, bi, lp, sv);
     KT key[R];
     double prod[R];
 #pragma unroll
@@ -540,6 +548,80 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
     }
 }
 
+// Symbolic count of register-tier rows when B has at most RB_SPAN columns:
+// the row's columns set bits of an LDS bitmap of CB_BITS columns, one pass
+// per CB_BITS of the row's column span (at most two), and the row's length
+// is the number of bits set.
+constexpr int CB_WORDS = 512;
+constexpr int CB_BITS = CB_WORDS * 64;
+union RcSmem {
+    RmLists l;
+    unsigned long long bm[CB_WORDS];
+};
+
+template <int R>
+__global__ __launch_bounds__(BRM_WAVE) void k_rmed_count(const int* __restrict__ rows, int nrows, RowArgs a,
+                                                         int64_t* __restrict__ nnzrow) {
+    __shared__ RcSmem sh;
+    const int lane = lane_id();
+    const int row = rows[blockIdx.x];
+    const int a0 = a.arpt[row];
+    const int L = a.arpt[row + 1] - a0;
+    const int64_t base = a.ipA[a0];
+    const int ip = (int)(a.ipA[a0 + L] - base);
+    int bi[R], lp[R];
+    double sv[R];
+    rm_products<R, false>(a, a0, L, base, ip, sh.l, bi, lp, sv);
+    int col[R];
+    int cmn = INT_MAX, cmx = -1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        col[r] = INT_MAX;
+        if (r * 64 + lane < ip) {
+            col[r] = a.bcol[bi[r]];
+            cmn = min(cmn, col[r]);
+            cmx = max(cmx, col[r]);
+        }
+    }
+#pragma unroll
+    for (int d = BRM_WAVE / 2; d > 0; d >>= 1) {
+        cmn = min(cmn, __shfl_xor(cmn, d, BRM_WAVE));
+        cmx = max(cmx, __shfl_xor(cmx, d, BRM_WAVE));
+    }
+    int cnt = 0;
+    // columns < RB_SPAN: c0 stays far from overflow; no products, no pass
+    for (int c0 = cmn & ~63; c0 <= cmx; c0 += CB_BITS) {  // uniform
+        const int nwd = min(CB_WORDS, ((cmx - c0) >> 6) + 1);
+#pragma unroll
+        for (int i = 0; i < CB_WORDS / BRM_WAVE; ++i)
+            if (i * BRM_WAVE + lane < nwd) sh.bm[i * BRM_WAVE + lane] = 0ull;
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const unsigned d = (unsigned)(col[r] - c0);
+            if (d < (unsigned)CB_BITS) atomicOr(&sh.bm[d >> 6], 1ull << (d & 63));
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < CB_WORDS / BRM_WAVE; ++i)
+            if (i * BRM_WAVE + lane < nwd) cnt += __popcll(sh.bm[i * BRM_WAVE + lane]);
+        wave_lds_sync();  // counted before the next pass clears
+    }
+#pragma unroll
+    for (int d = BRM_WAVE / 2; d > 0; d >>= 1) cnt += __shfl_xor(cnt, d, BRM_WAVE);
+    if (lane == 0) nnzrow[row] = cnt;
+}
+
+template <int R>
+hipError_t rmed_count_t(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s) {
+    const int64_t ch = launch_chunk(BRM_WAVE);
+    for (int64_t b = 0; b < n; b += ch) {
+        const int c = (int)std::min<int64_t>(ch, n - b);
+        k_rmed_count<R><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+    }
+    return hipGetLastError();
+}
+
 template <int R>
 hipError_t rmed_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
     const int64_t ch = launch_chunk(BRM_WAVE);
@@ -568,6 +650,18 @@ int rmed_auto_cap(int N) {
     // of position and index; with 64-bit keys they run slower than the
     // workgroup LDS tier, so the register tier then stops at 512 products.
     return (int64_t)N - 1 <= RmKey<16, uint32_t>::COL_MAX ? 1024 : 512;
+}
+
+bool rmed_count_fits(int N) { return N <= RB_SPAN; }
+
+hipError_t launch_rmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s, int cap) {
+    if (n <= 0) return hipSuccess;
+    if (!rmed_count_fits(a.N)) return hipErrorInvalidValue;
+    ProfScope ps_("k_rmed_count", s);
+    if (cap <= 128) return rmed_count_t<2>(rows, n, a, nnzrow, s);
+    if (cap <= 256) return rmed_count_t<4>(rows, n, a, nnzrow, s);
+    if (cap <= 512) return rmed_count_t<8>(rows, n, a, nnzrow, s);
+    return rmed_count_t<16>(rows, n, a, nnzrow, s);
 }
 
 hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s, int cap) {

@@ -157,7 +157,7 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
 /* Tuning knobs (names: "ws_budget_bytes", "med_cap", "leaf_cap",
  * "force_long", "profile", "lds_cfg", "group_cfg", "wmed_cap",
  * "wmed_engine", "launch_chunk", "cbar_budget_bytes", "sym_chunk_products",
- * "hash_cap", "upper_vm");
+ * "hash_cap", "upper_vm", "small_fused", "wmed_count");
  * returns BRM_E_INVALID for unknown names or
  * values.  Knobs other than hash_cap never change results, only speed.  group_cfg 0-3:
  * merge-path engine shapes for long rows; 4 (default): the bitmap-rank
@@ -173,7 +173,11 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
  * in list order); this knob, alone among them, changes value bits (within
  * 1e-12 of BRMerge's), never the structure.  upper_vm (1, the default):
  * BRMerge-Upper's row waves compact into C grown in a reserved address range;
- * 0 (or a driver without address reservation): into chunks copied into C. */
+ * 0 (or a driver without address reservation): into chunks copied into C.
+ * small_fused (1, the default): products whose rows are all short run in
+ * the three-launch fused path.  wmed_count (1, the default): the
+ * wavefront tier's row lengths are counted with a column bitmap when B has
+ * at most 65,536 columns; 0: with the hash count. */
 int brm_set_param(const char *name, int64_t value);
 int64_t brm_get_param(const char *name);
 

@@ -31,8 +31,10 @@ TIER_SETTINGS = {
     "register_medium_1024": {"wmed_cap": 1024, "wmed_engine": 1},
     "lds_wave_medium": {"wmed_cap": 512, "wmed_engine": 0},
     "general_pipeline": {"small_fused": 0},
+    "wave_medium_hash_count": {"wmed_cap": 1024, "wmed_engine": 1, "wmed_count": 0},
 }
-KNOBS = ("force_long", "leaf_cap", "med_cap", "wmed_cap", "group_cfg", "launch_chunk", "wmed_engine", "small_fused")
+KNOBS = ("force_long", "leaf_cap", "med_cap", "wmed_cap", "group_cfg", "launch_chunk", "wmed_engine", "small_fused",
+         "wmed_count")
 
 
 @pytest.fixture
