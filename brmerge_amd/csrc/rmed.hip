@@ -20,8 +20,9 @@
 //      keeps one pending left subtree per height;
 //   4. run heads are compacted with ballots and written to C (Precise), or
 //      to C-bar with the row length (Upper).
-//   Bitmap variant (B of at most RB_SPAN columns, R >= 8): the columns set
-//   bits of an LDS bitmap over the row's column span, popcount ranks give
+//   Bitmap variant (B of at most RB_SPAN columns, R >= 4): the columns set
+//   bits of an LDS bitmap over the row's column span (in chunks of 2^15
+//   columns), popcount ranks give
 //   each product its output position, products (still in registers) are
 //   bucketed by position into the bitmap's LDS, and each bucket is folded in
 //   list-position order -- no sort.
@@ -92,8 +93,8 @@ static_assert(sizeof(RmLists) <= sizeof(RmSmem<2, uint32_t>), "list table fits t
 // summed in list-position order -- O(ip) work instead of a sort.
 // Phase A (ranks) and phase B (buckets) share the LDS; the products stay in
 // registers until phase B places them.
-constexpr int RB_WORDS = 1024;
-constexpr int RB_SPAN = RB_WORDS * 64;
+constexpr int RB_SPAN = 1 << 16;  // columns of B the variant takes
+constexpr int RB_WORDS = 512;      // bitmap words of one chunk of the row's column span
 struct RbRanks {
     unsigned long long bm[RB_WORDS];
     unsigned short wrank[RB_WORDS];
@@ -251,7 +252,7 @@ template <int R, typename KT>
 __device__ void rmed_bitmap(const KT (&key)[R], const double (&prod)[R], int ip, int nw, int row, int64_t base,
                             const OutArgs& o, RbSmem<R>& sh) {
     using Key = RmKey<R, KT>;
-    constexpr int WPL = RB_WORDS / BRM_WAVE;  // bitmap words per lane, at most
+    constexpr int WPL = RB_WORDS / BRM_WAVE;  // bitmap words per lane and chunk, at most
     const int lane = lane_id();
     RbRanks& A = sh.a;
     RbBuckets<R>& B = sh.b;
@@ -270,46 +271,57 @@ __device__ void rmed_bitmap(const KT (&key)[R], const double (&prod)[R], int ip,
         cmn = min(cmn, __shfl_xor(cmn, d, BRM_WAVE));
         cmx = max(cmx, __shfl_xor(cmx, d, BRM_WAVE));
     }
-    // words w0 + 64 i + lane (consecutive lanes on consecutive words)
     const int w0 = cmn >> 6, w1 = min((cmx >> 6) + 1, nw);
-    const int nit = (w1 - w0 + BRM_WAVE - 1) / BRM_WAVE;
-    const int cbase = w0 * 64;  // columns of the bitmap's span are cbase + [0, 2^16)
+    const int cbase = w0 * 64;  // columns of the row's span are cbase + [0, 2^16)
+    // column - cbase of each product; lanes past the row hold a value no
+    // chunk takes
+    int bc[R];
 #pragma unroll
-    for (int i = 0; i < WPL; ++i) {
-        const int w = w0 + i * BRM_WAVE + lane;
-        if (i < nit && w < w1) A.bm[w] = 0ull;
-    }
-    wave_lds_sync();
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        if (r * 64 + lane < ip) {
-            const int bit = (int)Key::col(key[r]);
-            atomicOr(&A.bm[bit >> 6], 1ull << (bit & 63));
-        }
-    }
-    wave_lds_sync();
-    int nout = 0;  // ranks: a scan per step of 64 words, carried
-#pragma unroll
-    for (int i = 0; i < WPL; ++i) {
-        if (i >= nit) break;  // uniform
-        const int w = w0 + i * BRM_WAVE + lane;
-        const int pc = w < w1 ? __popcll(A.bm[w]) : 0;
-        const int inc = wave_incl_scan_dpp(pc);
-        if (w < w1) A.wrank[w] = (unsigned short)(nout + inc - pc);
-        nout += __shfl(inc, BRM_WAVE - 1, BRM_WAVE);
-    }
-    wave_lds_sync();
+    for (int r = 0; r < R; ++r) bc[r] = r * 64 + lane < ip ? (int)Key::col(key[r]) - cbase : (1 << 30);
+    // the span in chunks of RB_WORDS words (two at most): bits, ranks and
+    // the positions of the chunk's products; ranks carry across chunks
+    int nout = 0;
     int pos[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        pos[r] = 0;
-        if (r * 64 + lane < ip) {
-            const int bit = (int)Key::col(key[r]);
-            const int wd = bit >> 6;
-            pos[r] = A.wrank[wd] + __popcll(A.bm[wd] & ((1ull << (bit & 63)) - 1ull));
+    for (int r = 0; r < R; ++r) pos[r] = 0;
+    auto chunk = [&](int wc) {
+        const int nwc = min(RB_WORDS, w1 - w0 - wc);
+        const int nit = (nwc + BRM_WAVE - 1) / BRM_WAVE;
+#pragma unroll
+        for (int i = 0; i < WPL; ++i) {
+            const int w = i * BRM_WAVE + lane;
+            if (w < nwc) A.bm[w] = 0ull;
         }
-    }
-    wave_lds_sync();  // bitmap and ranks are dead from here on
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const unsigned bit = (unsigned)(bc[r] - wc * 64);
+            if (bit < (unsigned)(RB_WORDS * 64)) atomicOr(&A.bm[bit >> 6], 1ull << (bit & 63));
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < WPL; ++i) {
+            if (i >= nit) break;  // uniform
+            const int w = i * BRM_WAVE + lane;
+            const int pc = w < nwc ? __popcll(A.bm[w]) : 0;
+            const int inc = wave_incl_scan_dpp(pc);
+            if (w < nwc) A.wrank[w] = (unsigned short)(nout + inc - pc);
+            nout += __shfl(inc, BRM_WAVE - 1, BRM_WAVE);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const unsigned bit = (unsigned)(bc[r] - wc * 64);
+            if (bit < (unsigned)(RB_WORDS * 64)) {
+                const int wd = (int)(bit >> 6);
+                pos[r] = A.wrank[wd] + __popcll(A.bm[wd] & ((1ull << (bit & 63)) - 1ull));
+            }
+        }
+        wave_lds_sync();  // read before the next chunk clears
+    };
+    if (w1 > w0) chunk(0);
+    if (w1 - w0 > RB_WORDS) chunk(RB_WORDS);  // uniform
+    // bitmap and ranks are dead from here on
     // bucket counters: 16 bits each, buckets 2i and 2i + 1 in word i
     const int ncw = (nout + 1) >> 1;
 #pragma unroll
@@ -320,7 +332,7 @@ __device__ void rmed_bitmap(const KT (&key)[R], const double (&prod)[R], int ip,
     for (int r = 0; r < R; ++r) {
         if (r * 64 + lane < ip) {
             atomicAdd(&B.cnt[pos[r] >> 1], 1u << ((pos[r] & 1) << 4));
-            B.oc[pos[r]] = (unsigned short)((int)Key::col(key[r]) - cbase);  // every writer stores the same column
+            B.oc[pos[r]] = (unsigned short)bc[r];  // every writer stores the same column
         }
     }
     wave_lds_sync();
@@ -627,11 +639,11 @@ hipError_t rmed_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hi
     const int64_t ch = launch_chunk(BRM_WAVE);
     const bool narrow = a.N <= 0 || (int64_t)a.N - 1 <= RmKey<R, uint32_t>::COL_MAX;
     // the bitmap variant pays for its larger LDS footprint on rows of more
-    // than 256 products
-    const bool bitmap = R >= 8 && a.N <= RB_SPAN;
+    // than 128 products
+    const bool bitmap = R >= 4 && a.N <= RB_SPAN;
     for (int64_t b = 0; b < n; b += ch) {
         const int c = (int)std::min<int64_t>(ch, n - b);
-        if constexpr (R >= 8) {
+        if constexpr (R >= 4) {
             if (bitmap) {
                 k_rmed<R, uint32_t, true><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
                 continue;
