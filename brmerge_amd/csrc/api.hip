@@ -216,12 +216,6 @@ struct HostTrace {
     }
 };
 
-int ceil_log2_host(int x) {
-    int h = 0;
-    while ((1LL << h) < (long long)x) ++h;
-    return h;
-}
-
 // A second stream of the calling thread (default priority, created once per
 // device and slot and kept until the process ends) for work that runs beside
 // the main stream: fork() makes it wait for what the main stream has queued,
@@ -339,17 +333,19 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     // bitmap engine a row of one pass stores nothing and a row of two passes
     // only pass-1 nodes (buffer 1).  The merge-path engine's leaves may use
     // either buffer.
+    // tree height of every row (ceil log2 of its list count), computed once
+    std::vector<int> hk(nlong);
+    for (int k = 0; k < nlong; ++k) hk[k] = len[k] <= 1 ? 0 : 32 - __builtin_clz((unsigned)(len[k] - 1));
     auto ws_need = [&](int k, int b) -> int64_t {
         if (!bitmap) return ip[k];
-        const int H = ceil_log2_host(len[k]);
+        const int H = hk[k];
         const int J = H == 0 ? 1 : (H + pass_bits - 1) / pass_bits;
         return (b == 1 ? J >= 2 : J >= 3) ? ip[k] : 0;
     };
-    auto row_bytes = [&](int k) {
-        const int H = ceil_log2_host(len[k]);
-        return 12 * (ws_need(k, 0) + ws_need(k, 1)) + 8 * ((int64_t)2 << H) +
-               (int64_t)(sizeof(GroupTask) + sizeof(LeafTask) + 2 * sizeof(Child)) * (len[k] + 16);
-    };
+    std::vector<int64_t> rbytes(nlong);
+    for (int k = 0; k < nlong; ++k)
+        rbytes[k] = 12 * (ws_need(k, 0) + ws_need(k, 1)) + 8 * ((int64_t)2 << hk[k]) +
+                    (int64_t)(sizeof(GroupTask) + sizeof(LeafTask) + 2 * sizeof(Child)) * (len[k] + 16);
 
     tr.mark("long info read back");
     // ---- partition into waves and size every buffer for the largest wave
@@ -368,7 +364,7 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
     for (int k0 = 0; k0 < nlong;) {
         int k1 = k0;
         int64_t used = 0;
-        while (k1 < nlong && (k1 == k0 || used + row_bytes(k1) <= budget)) used += row_bytes(k1++);
+        while (k1 < nlong && (k1 == k0 || used + rbytes[k1] <= budget)) used += rbytes[k1++];
         Wave w;
         w.k0 = k0;
         w.k1 = k1;
@@ -378,7 +374,7 @@ int run_long(const RowArgs& a, const int* rows_d, int nlong, const OutArgs& o, c
         std::vector<int64_t> capP(MAX_PASSES + 1, 0);
         for (int k = 0; k < k1 - k0; ++k) {
             const int Lr = len[k0 + k];
-            const int H = ceil_log2_host(Lr);
+            const int H = hk[k0 + k];
             const int J = H == 0 ? 1 : (H + pass_bits - 1) / pass_bits;
             if (J > MAX_PASSES) {
                 set_err("long row with 2^%d lists exceeds the supported tree height", H);
