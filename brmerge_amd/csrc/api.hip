@@ -31,7 +31,7 @@
 namespace brm {
 
 Params& params() {
-    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, -1, 0, 1, 0, (int64_t)1 << 30, 0, 1, 1, 1};
+    static Params p{0, MED_CAP, 0, MED_CAP, 0, 0, GROUP_CFG_BITMAP, -1, 0, 1, 0, (int64_t)1 << 30, 0, 1, 1, 1, 1};
     return p;
 }
 
@@ -1607,6 +1607,7 @@ int brm_set_param(const char* name, int64_t value) {
     else if (!strcmp(name, "upper_vm")) { p.upper_vm = value ? 1 : 0; }
     else if (!strcmp(name, "small_fused")) { p.small_fused = value ? 1 : 0; }
     else if (!strcmp(name, "wmed_count")) { p.wmed_count = value ? 1 : 0; }
+    else if (!strcmp(name, "wmed_bitmap")) { p.wmed_bitmap = value ? 1 : 0; }
     else if (!strcmp(name, "hash_cap")) { if (value < 0 || value > HASH_CAP) return BRM_E_INVALID; p.hash_cap = (int)value; }
     else if (!strcmp(name, "sym_chunk_products")) { if (value < 1 || value > ((int64_t)1 << 30)) return BRM_E_INVALID; p.sym_chunk_products = value; }
     else return BRM_E_INVALID;
@@ -1632,6 +1633,7 @@ int64_t brm_get_param(const char* name) {
     if (!strcmp(name, "upper_vm")) return p.upper_vm;
     if (!strcmp(name, "small_fused")) return p.small_fused;
     if (!strcmp(name, "wmed_count")) return p.wmed_count;
+    if (!strcmp(name, "wmed_bitmap")) return p.wmed_bitmap;
     return -1;
 }
 

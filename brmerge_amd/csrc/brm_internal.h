@@ -193,7 +193,8 @@ struct Params {
     int hash_cap;              // rows of at most hash_cap products take the hash tier (0 = off, <= HASH_CAP)
     int upper_vm;              // Upper waves grow C in reserved address ranges (1) or copy chunks (0)
     int small_fused;           // products of few short rows in three launches (1) or the general pipeline (0)
-    int wmed_count;            // wavefront tier symbolic: 1 column bitmap (B of <= 65,536 columns), 0 hash
+    int wmed_count;            // wavefront tier symbolic: 1 column bitmap (B of <= 2^20 columns), 0 hash
+    int wmed_bitmap;           // register tier numeric: 1 bitmap ranks (B of <= 2^20 columns, rows > 128 products), 0 sort
 };
 
 // Shapes of the LDS tree-merge kernels, selectable at run time.

@@ -81,7 +81,7 @@ hipError_t order_tasks(const uint32_t* keys, int n, uint32_t* keys_out, int* idx
 // rmed.hip (register tier; cap = products per row, <= WMED_CAP)
 hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s, int cap);
 // Exact row lengths of register-tier rows (ip <= cap, at most 64 lists) by
-// an LDS column bitmap; only for B of at most 65,536 columns (rmed_count_fits).
+// an LDS column bitmap; only for B of at most 2^20 columns (rmed_count_fits).
 bool rmed_count_fits(int N);
 hipError_t launch_rmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s, int cap);
 // Wavefront-tier capacity chosen when wmed_cap is -1 (register engine): 1024

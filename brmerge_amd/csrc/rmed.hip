@@ -87,29 +87,43 @@ struct RmLists {
 };
 static_assert(sizeof(RmLists) <= sizeof(RmSmem<2, uint32_t>), "list table fits the smallest variant");
 
-// Bitmap variant (B with at most RB_SPAN columns): the row's columns set
-// bits of an LDS bitmap, popcount ranks give every product its
-// output position, products are bucketed by position and each bucket is
-// summed in list-position order -- O(ip) work instead of a sort.
-// Phase A (ranks) and phase B (buckets) share the LDS; the products stay in
-// registers until phase B places them.
-constexpr int RB_SPAN = 1 << 16;  // columns of B the variant takes
-constexpr int RB_WORDS = 512;      // bitmap words of one chunk of the row's column span
+// Bitmap variants: the row's columns set bits of an LDS bitmap, popcount
+// ranks give every product its output position, products are bucketed by
+// position and each bucket is summed in list-position order -- O(ip) work
+// instead of a sort.  Phase A (ranks) and phase B (buckets) share the LDS;
+// the products stay in registers until phase B places them.
+//   flat (B with at most RB_SPAN columns): one bitmap over the row's column
+//     span, in chunks of RB_WORDS words (two at most);
+//   two-level (B with at most RB2_SPAN columns): level 1 has one bit per
+//     64-column word of the span, and its popcount ranks number the words
+//     in use; level 2 holds those words only (at most one per product).
+constexpr int RB_SPAN = 1 << 16;   // flat: columns of B
+constexpr int RB_WORDS = 512;      // flat: bitmap words of one chunk of the row's column span
+constexpr int RB2_SPAN = 1 << 20;  // two-level: columns of B
+constexpr int RB2_L1 = RB2_SPAN / 64 / 64;  // level-1 words
 struct RbRanks {
     unsigned long long bm[RB_WORDS];
     unsigned short wrank[RB_WORDS];
 };
 template <int R>
+struct Rb2Ranks {
+    unsigned long long l1[RB2_L1];
+    unsigned short l1rank[RB2_L1];
+    unsigned long long l2[64 * R];
+    unsigned short l2rank[64 * R];
+};
+template <int R, bool TWO>
 struct RbBuckets {
-    double val[64 * R];        // contributions, bucket by bucket
-    unsigned cnt[32 * R];      // 16-bit bucket counters, two per word (bucket 2i low, 2i + 1 high)
-    unsigned short oc[64 * R]; // bucket column minus the span's first column
+    double val[64 * R];    // contributions, bucket by bucket
+    unsigned cnt[32 * R];  // 16-bit bucket counters, two per word (bucket 2i low, 2i + 1 high)
+    // bucket column (two-level) or bucket column minus the span's first column (flat)
+    typename std::conditional<TWO, int, unsigned short>::type oc[64 * R];
     unsigned char lp[64 * R];  // list position of each contribution
 };
-template <int R>
+template <int R, bool TWO>
 union RbSmem {
-    RbRanks a;
-    RbBuckets<R> b;
+    typename std::conditional<TWO, Rb2Ranks<R>, RbRanks>::type a;
+    RbBuckets<R, TWO> b;
     RmLists l;
 };
 static_assert(64 * 16 <= 0xffff, "bucket offsets in 16 bits");
@@ -246,24 +260,45 @@ __device__ __forceinline__ void rm_products(const RowArgs& a, int a0, int L, int
     wave_lds_sync();  // the table is read before its area is reused
 }
 
-// Columns index the bitmap directly (B has at most RB_SPAN columns, nw
-// words of them in use).  prod[r] is the value of key[r]'s product.
-template <int R, typename KT>
-__device__ void rmed_bitmap(const KT (&key)[R], const double (&prod)[R], int ip, int nw, int row, int64_t base,
-                            const OutArgs& o, RbSmem<R>& sh) {
-    using Key = RmKey<R, KT>;
-    constexpr int WPL = RB_WORDS / BRM_WAVE;  // bitmap words per lane and chunk, at most
+// Exclusive ranks of n words' popcounts into rank[0..n), consecutive lanes
+// on consecutive words, at most NI steps of 64 words; returns the total.
+template <int NI>
+__device__ __forceinline__ int rank_words(const unsigned long long* bm, unsigned short* rank, int n) {
     const int lane = lane_id();
-    RbRanks& A = sh.a;
-    RbBuckets<R>& B = sh.b;
+    int tot = 0;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        if (i * BRM_WAVE >= n) break;  // uniform
+        const int w = i * BRM_WAVE + lane;
+        const int pc = w < n ? __popcll(bm[w]) : 0;
+        const int inc = wave_incl_scan_dpp(pc);
+        if (w < n) rank[w] = (unsigned short)(tot + inc - pc);
+        tot += __shfl(inc, BRM_WAVE - 1, BRM_WAVE);
+    }
+    return tot;
+}
+
+// Products of product indices 64 r + lane < ip: column cl[r] & CL_COL, list
+// position cl[r] >> CL_LP, value prod[r]; B has at most RB_SPAN (flat) or
+// RB2_SPAN (two-level) columns, nw words of them.
+constexpr int CL_LP = 20;
+constexpr int CL_COL = (1 << CL_LP) - 1;
+static_assert(RB2_SPAN <= CL_COL + 1, "columns beside the list position");
+template <int R, bool TWO>
+__device__ void rmed_bitmap(const int (&cl)[R], const double (&prod)[R], int ip, int nw, int row, int64_t base,
+                            const OutArgs& o, RbSmem<R, TWO>& sh) {
+    const int lane = lane_id();
+    int col[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) col[r] = cl[r] & CL_COL;
+    RbBuckets<R, TWO>& B = sh.b;
     // only the words between the row's smallest and largest column are used
     int cmn = INT_MAX, cmx = 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (r * 64 + lane < ip) {
-            const int c = (int)Key::col(key[r]);
-            cmn = min(cmn, c);
-            cmx = max(cmx, c);
+            cmn = min(cmn, col[r]);
+            cmx = max(cmx, col[r]);
         }
     }
 #pragma unroll
@@ -272,56 +307,99 @@ __device__ void rmed_bitmap(const KT (&key)[R], const double (&prod)[R], int ip,
         cmx = max(cmx, __shfl_xor(cmx, d, BRM_WAVE));
     }
     const int w0 = cmn >> 6, w1 = min((cmx >> 6) + 1, nw);
-    const int cbase = w0 * 64;  // columns of the row's span are cbase + [0, 2^16)
-    // column - cbase of each product; lanes past the row hold a value no
-    // chunk takes
-    int bc[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) bc[r] = r * 64 + lane < ip ? (int)Key::col(key[r]) - cbase : (1 << 30);
-    // the span in chunks of RB_WORDS words (two at most): bits, ranks and
-    // the positions of the chunk's products; ranks carry across chunks
+    const int cbase = TWO ? 0 : w0 * 64;  // flat: columns of the span are cbase + [0, 2^16)
     int nout = 0;
     int pos[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) pos[r] = 0;
-    auto chunk = [&](int wc) {
-        const int nwc = min(RB_WORDS, w1 - w0 - wc);
-        const int nit = (nwc + BRM_WAVE - 1) / BRM_WAVE;
+    if constexpr (!TWO) {
+        constexpr int WPL = RB_WORDS / BRM_WAVE;  // bitmap words per lane and chunk, at most
+        RbRanks& A = sh.a;
+        // column - cbase of each product; lanes past the row hold a value no
+        // chunk takes
+        int bc[R];
 #pragma unroll
-        for (int i = 0; i < WPL; ++i) {
+        for (int r = 0; r < R; ++r) bc[r] = r * 64 + lane < ip ? col[r] - cbase : (1 << 30);
+        // the span in chunks of RB_WORDS words (two at most): bits, ranks and
+        // the positions of the chunk's products; ranks carry across chunks
+        auto chunk = [&](int wc) {
+            const int nwc = min(RB_WORDS, w1 - w0 - wc);
+#pragma unroll
+            for (int i = 0; i < WPL; ++i) {
+                const int w = i * BRM_WAVE + lane;
+                if (w < nwc) A.bm[w] = 0ull;
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const unsigned bit = (unsigned)(bc[r] - wc * 64);
+                if (bit < (unsigned)(RB_WORDS * 64)) atomicOr(&A.bm[bit >> 6], 1ull << (bit & 63));
+            }
+            wave_lds_sync();
+            const int base_rank = nout;
+            nout += rank_words<WPL>(A.bm, A.wrank, nwc);
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const unsigned bit = (unsigned)(bc[r] - wc * 64);
+                if (bit < (unsigned)(RB_WORDS * 64)) {
+                    const int wd = (int)(bit >> 6);
+                    pos[r] = base_rank + A.wrank[wd] + __popcll(A.bm[wd] & ((1ull << (bit & 63)) - 1ull));
+                }
+            }
+            wave_lds_sync();  // read before the next chunk clears
+        };
+        if (w1 > w0) chunk(0);
+        if (w1 - w0 > RB_WORDS) chunk(RB_WORDS);  // uniform
+    } else {
+        Rb2Ranks<R>& A = sh.a;
+        constexpr int L1PL = RB2_L1 / BRM_WAVE;  // level-1 words per lane, at most
+        const int n1 = (w1 - w0 + 63) >> 6;       // level-1 words in use
+#pragma unroll
+        for (int i = 0; i < L1PL; ++i) {
             const int w = i * BRM_WAVE + lane;
-            if (w < nwc) A.bm[w] = 0ull;
+            if (w < n1) A.l1[w] = 0ull;
         }
         wave_lds_sync();
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const unsigned bit = (unsigned)(bc[r] - wc * 64);
-            if (bit < (unsigned)(RB_WORDS * 64)) atomicOr(&A.bm[bit >> 6], 1ull << (bit & 63));
-        }
-        wave_lds_sync();
-#pragma unroll
-        for (int i = 0; i < WPL; ++i) {
-            if (i >= nit) break;  // uniform
-            const int w = i * BRM_WAVE + lane;
-            const int pc = w < nwc ? __popcll(A.bm[w]) : 0;
-            const int inc = wave_incl_scan_dpp(pc);
-            if (w < nwc) A.wrank[w] = (unsigned short)(nout + inc - pc);
-            nout += __shfl(inc, BRM_WAVE - 1, BRM_WAVE);
-        }
-        wave_lds_sync();
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const unsigned bit = (unsigned)(bc[r] - wc * 64);
-            if (bit < (unsigned)(RB_WORDS * 64)) {
-                const int wd = (int)(bit >> 6);
-                pos[r] = A.wrank[wd] + __popcll(A.bm[wd] & ((1ull << (bit & 63)) - 1ull));
+            if (r * 64 + lane < ip) {
+                const int wid = (col[r] >> 6) - w0;
+                atomicOr(&A.l1[wid >> 6], 1ull << (wid & 63));
             }
         }
-        wave_lds_sync();  // read before the next chunk clears
-    };
-    if (w1 > w0) chunk(0);
-    if (w1 - w0 > RB_WORDS) chunk(RB_WORDS);  // uniform
-    // bitmap and ranks are dead from here on
+        wave_lds_sync();
+        const int n2 = rank_words<L1PL>(A.l1, A.l1rank, n1);  // level-2 words (words in use)
+        wave_lds_sync();
+        int kk[R];  // level-2 word of each product
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            kk[r] = 0;
+            if (r * 64 + lane < ip) {
+                const int wid = (col[r] >> 6) - w0;
+                kk[r] = A.l1rank[wid >> 6] + __popcll(A.l1[wid >> 6] & ((1ull << (wid & 63)) - 1ull));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            const int q = i * BRM_WAVE + lane;
+            if (q < n2) A.l2[q] = 0ull;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (r * 64 + lane < ip) atomicOr(&A.l2[kk[r]], 1ull << (col[r] & 63));
+        wave_lds_sync();
+        nout = rank_words<R>(A.l2, A.l2rank, n2);
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r * 64 + lane < ip)
+                pos[r] = A.l2rank[kk[r]] + __popcll(A.l2[kk[r]] & ((1ull << (col[r] & 63)) - 1ull));
+        }
+        wave_lds_sync();
+    }
+    // ranks are dead from here on
     // bucket counters: 16 bits each, buckets 2i and 2i + 1 in word i
     const int ncw = (nout + 1) >> 1;
 #pragma unroll
@@ -332,7 +410,9 @@ __device__ void rmed_bitmap(const KT (&key)[R], const double (&prod)[R], int ip,
     for (int r = 0; r < R; ++r) {
         if (r * 64 + lane < ip) {
             atomicAdd(&B.cnt[pos[r] >> 1], 1u << ((pos[r] & 1) << 4));
-            B.oc[pos[r]] = (unsigned short)bc[r];  // every writer stores the same column
+            // every writer stores the same column
+            if constexpr (TWO) B.oc[pos[r]] = col[r];
+            else B.oc[pos[r]] = (unsigned short)(col[r] - cbase);
         }
     }
     wave_lds_sync();
@@ -357,7 +437,7 @@ __device__ void rmed_bitmap(const KT (&key)[R], const double (&prod)[R], int ip,
             const unsigned sft = (unsigned)(pos[r] & 1) << 4;
             const int slot = (int)((atomicAdd(&B.cnt[pos[r] >> 1], 1u << sft) >> sft) & 0xffffu);
             B.val[slot] = prod[r];
-            B.lp[slot] = (unsigned char)Key::lp(key[r]);
+            B.lp[slot] = (unsigned char)(cl[r] >> CL_LP);
         }
     }
     wave_lds_sync();
@@ -442,12 +522,14 @@ __device__ __forceinline__ void wave_bitonic_sort(KT (&key)[R], int rr) {
     }
 }
 
-template <int R, typename KT, bool BITMAP>
+// MODE 0: sort; 1: flat bitmap (B of at most RB_SPAN columns); 2: two-level
+// bitmap (B of at most RB2_SPAN columns).
+template <int R, typename KT, int MODE>
 __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows, int nrows, RowArgs a, OutArgs o) {
     using Key = RmKey<R, KT>;
-    // one of the two layouts, sized for the variant; the list table of
-    // stage 1 overlays the start of either
-    __shared__ typename std::conditional<BITMAP, RbSmem<R>, RmSmem<R, KT>>::type shm;
+    // one of the layouts, sized for the variant; the list table of stage 1
+    // overlays the start of each
+    __shared__ typename std::conditional<MODE == 0, RmSmem<R, KT>, RbSmem<R, MODE == 2>>::type shm;
     RmSmem<R, KT>& sm = *reinterpret_cast<RmSmem<R, KT>*>(&shm);
     RmLists& ls = *reinterpret_cast<RmLists*>(&shm);
     const int lane = lane_id();
@@ -461,23 +543,31 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
     int bi[R], lp[R];
     double sv[R];
     rm_products<R, true>(a, a0, L, base, ip, ls, bi, lp, sv);
-    KT key[R];
+    int col[R];
     double prod[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int j = r * 64 + lane;
-        key[r] = ~(KT)0;
+        col[r] = 0;
         prod[r] = 0.0;
-        if (j < ip) {
-            key[r] = Key::make(a.bcol[bi[r]], lp[r], j);
+        if (r * 64 + lane < ip) {
+            col[r] = a.bcol[bi[r]];
             prod[r] = fmul(sv[r], a.bval[bi[r]]);
         }
     }
 
-    // ---- B with at most RB_SPAN columns: bitmap ranks instead of a sort
-    if constexpr (BITMAP) {
-        rmed_bitmap<R, KT>(key, prod, ip, (a.N + 63) >> 6, row, base, o, shm);
+    // ---- bitmap ranks instead of a sort
+    if constexpr (MODE != 0) {
+        int cl[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) cl[r] = col[r] | (lp[r] << CL_LP);
+        rmed_bitmap<R, MODE == 2>(cl, prod, ip, (a.N + 63) >> 6, row, base, o, shm);
         return;
+    }
+    KT key[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = r * 64 + lane;
+        key[r] = j < ip ? Key::make(col[r], lp[r], j) : ~(KT)0;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -560,21 +650,29 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed(const int* __restrict__ rows,
     }
 }
 
-// Symbolic count of register-tier rows when B has at most RB_SPAN columns:
-// the row's columns set bits of an LDS bitmap of CB_BITS columns, one pass
-// per CB_BITS of the row's column span (at most two), and the row's length
-// is the number of bits set.
+// Symbolic count of register-tier rows: the row's length is the number of
+// distinct columns, the bits set in an LDS bitmap.
+//   flat (B with at most RB_SPAN columns): a bitmap of CB_BITS columns, one
+//     pass per CB_BITS of the row's column span (at most two);
+//   two-level (B with at most RB2_SPAN columns): as the numeric variant's
+//     ranks -- level 1 numbers the 64-column words in use, level 2 holds them.
 constexpr int CB_WORDS = 512;
 constexpr int CB_BITS = CB_WORDS * 64;
+template <int R, bool TWO>
 union RcSmem {
     RmLists l;
     unsigned long long bm[CB_WORDS];
+    struct {
+        unsigned long long l1[RB2_L1];
+        unsigned short l1rank[RB2_L1];
+        unsigned long long l2[TWO ? 64 * R : 1];
+    } a;
 };
 
-template <int R>
+template <int R, bool TWO>
 __global__ __launch_bounds__(BRM_WAVE) void k_rmed_count(const int* __restrict__ rows, int nrows, RowArgs a,
                                                          int64_t* __restrict__ nnzrow) {
-    __shared__ RcSmem sh;
+    __shared__ RcSmem<R, TWO> sh;
     const int lane = lane_id();
     const int row = rows[blockIdx.x];
     const int a0 = a.arpt[row];
@@ -601,23 +699,63 @@ __global__ __launch_bounds__(BRM_WAVE) void k_rmed_count(const int* __restrict__
         cmx = max(cmx, __shfl_xor(cmx, d, BRM_WAVE));
     }
     int cnt = 0;
-    // columns < RB_SPAN: c0 stays far from overflow; no products, no pass
-    for (int c0 = cmn & ~63; c0 <= cmx; c0 += CB_BITS) {  // uniform
-        const int nwd = min(CB_WORDS, ((cmx - c0) >> 6) + 1);
+    if constexpr (!TWO) {
+        // columns < RB_SPAN: c0 stays far from overflow; no products, no pass
+        for (int c0 = cmn & ~63; c0 <= cmx; c0 += CB_BITS) {  // uniform
+            const int nwd = min(CB_WORDS, ((cmx - c0) >> 6) + 1);
 #pragma unroll
-        for (int i = 0; i < CB_WORDS / BRM_WAVE; ++i)
-            if (i * BRM_WAVE + lane < nwd) sh.bm[i * BRM_WAVE + lane] = 0ull;
+            for (int i = 0; i < CB_WORDS / BRM_WAVE; ++i)
+                if (i * BRM_WAVE + lane < nwd) sh.bm[i * BRM_WAVE + lane] = 0ull;
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const unsigned d = (unsigned)(col[r] - c0);
+                if (d < (unsigned)CB_BITS) atomicOr(&sh.bm[d >> 6], 1ull << (d & 63));
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int i = 0; i < CB_WORDS / BRM_WAVE; ++i)
+                if (i * BRM_WAVE + lane < nwd) cnt += __popcll(sh.bm[i * BRM_WAVE + lane]);
+            wave_lds_sync();  // counted before the next pass clears
+        }
+    } else if (cmx >= 0) {  // uniform: the row has products
+        constexpr int L1PL = RB2_L1 / BRM_WAVE;
+        const int w0 = cmn >> 6;
+        const int n1 = ((cmx >> 6) - w0 + 64) >> 6;  // level-1 words in use
+#pragma unroll
+        for (int i = 0; i < L1PL; ++i)
+            if (i * BRM_WAVE + lane < n1) sh.a.l1[i * BRM_WAVE + lane] = 0ull;
         wave_lds_sync();
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const unsigned d = (unsigned)(col[r] - c0);
-            if (d < (unsigned)CB_BITS) atomicOr(&sh.bm[d >> 6], 1ull << (d & 63));
+            if (r * 64 + lane < ip) {
+                const int wid = (col[r] >> 6) - w0;
+                atomicOr(&sh.a.l1[wid >> 6], 1ull << (wid & 63));
+            }
         }
         wave_lds_sync();
+        const int n2 = rank_words<L1PL>(sh.a.l1, sh.a.l1rank, n1);
+        wave_lds_sync();
+        int kk[R];
 #pragma unroll
-        for (int i = 0; i < CB_WORDS / BRM_WAVE; ++i)
-            if (i * BRM_WAVE + lane < nwd) cnt += __popcll(sh.bm[i * BRM_WAVE + lane]);
-        wave_lds_sync();  // counted before the next pass clears
+        for (int r = 0; r < R; ++r) {
+            kk[r] = 0;
+            if (r * 64 + lane < ip) {
+                const int wid = (col[r] >> 6) - w0;
+                kk[r] = sh.a.l1rank[wid >> 6] + __popcll(sh.a.l1[wid >> 6] & ((1ull << (wid & 63)) - 1ull));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (i * BRM_WAVE + lane < n2) sh.a.l2[i * BRM_WAVE + lane] = 0ull;
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (r * 64 + lane < ip) atomicOr(&sh.a.l2[kk[r]], 1ull << (col[r] & 63));
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (i * BRM_WAVE + lane < n2) cnt += __popcll(sh.a.l2[i * BRM_WAVE + lane]);
     }
 #pragma unroll
     for (int d = BRM_WAVE / 2; d > 0; d >>= 1) cnt += __shfl_xor(cnt, d, BRM_WAVE);
@@ -629,7 +767,8 @@ hipError_t rmed_count_t(const int* rows, int n, const RowArgs& a, int64_t* nnzro
     const int64_t ch = launch_chunk(BRM_WAVE);
     for (int64_t b = 0; b < n; b += ch) {
         const int c = (int)std::min<int64_t>(ch, n - b);
-        k_rmed_count<R><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+        if (a.N <= RB_SPAN) k_rmed_count<R, false><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+        else k_rmed_count<R, true><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
     }
     return hipGetLastError();
 }
@@ -638,19 +777,24 @@ template <int R>
 hipError_t rmed_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s) {
     const int64_t ch = launch_chunk(BRM_WAVE);
     const bool narrow = a.N <= 0 || (int64_t)a.N - 1 <= RmKey<R, uint32_t>::COL_MAX;
-    // the bitmap variant pays for its larger LDS footprint on rows of more
+    // the bitmap variants pay for their larger LDS footprint on rows of more
     // than 128 products
-    const bool bitmap = R >= 4 && a.N <= RB_SPAN;
+    const bool bm_on = R >= 4 && params().wmed_bitmap == 1;
+    const int mode = !bm_on ? 0 : a.N <= RB_SPAN ? 1 : a.N <= RB2_SPAN ? 2 : 0;
     for (int64_t b = 0; b < n; b += ch) {
         const int c = (int)std::min<int64_t>(ch, n - b);
         if constexpr (R >= 4) {
-            if (bitmap) {
-                k_rmed<R, uint32_t, true><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
+            if (mode == 1) {
+                k_rmed<R, uint32_t, 1><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
+                continue;
+            }
+            if (mode == 2) {
+                k_rmed<R, uint32_t, 2><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
                 continue;
             }
         }
-        if (narrow) k_rmed<R, uint32_t, false><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
-        else k_rmed<R, uint64_t, false><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
+        if (narrow) k_rmed<R, uint32_t, 0><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
+        else k_rmed<R, uint64_t, 0><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
     }
     return hipGetLastError();
 }
@@ -658,13 +802,16 @@ hipError_t rmed_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hi
 }  // namespace
 
 int rmed_auto_cap(int N) {
-    // 1024-product rows sort 32-bit keys only while N fits beside 16 bits
-    // of position and index; with 64-bit keys they run slower than the
-    // workgroup LDS tier, so the register tier then stops at 512 products.
+    // The bitmap variants take rows of up to 1024 products while B has at
+    // most RB2_SPAN columns.  Otherwise 1024-product rows sort 32-bit keys
+    // only while N fits beside 16 bits of position and index; with 64-bit
+    // keys they run slower than the workgroup LDS tier, so the register tier
+    // then stops at 512 products.
+    if (params().wmed_bitmap == 1 && N <= RB2_SPAN) return 1024;
     return (int64_t)N - 1 <= RmKey<16, uint32_t>::COL_MAX ? 1024 : 512;
 }
 
-bool rmed_count_fits(int N) { return N <= RB_SPAN; }
+bool rmed_count_fits(int N) { return N <= RB2_SPAN; }
 
 hipError_t launch_rmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s, int cap) {
     if (n <= 0) return hipSuccess;

@@ -157,7 +157,7 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
 /* Tuning knobs (names: "ws_budget_bytes", "med_cap", "leaf_cap",
  * "force_long", "profile", "lds_cfg", "group_cfg", "wmed_cap",
  * "wmed_engine", "launch_chunk", "cbar_budget_bytes", "sym_chunk_products",
- * "hash_cap", "upper_vm", "small_fused", "wmed_count");
+ * "hash_cap", "upper_vm", "small_fused", "wmed_count", "wmed_bitmap");
  * returns BRM_E_INVALID for unknown names or
  * values.  Knobs other than hash_cap never change results, only speed.  group_cfg 0-3:
  * merge-path engine shapes for long rows; 4 (default): the bitmap-rank
@@ -177,7 +177,10 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
  * small_fused (1, the default): products whose rows are all short run in
  * the three-launch fused path.  wmed_count (1, the default): the
  * wavefront tier's row lengths are counted with a column bitmap when B has
- * at most 65,536 columns; 0: with the hash count. */
+ * at most 2^20 columns; 0: with the hash count.  wmed_bitmap (1, the
+ * default): the register tier ranks rows of more than 128 products with a
+ * column bitmap when B has at most 2^20 columns (and then takes rows of up
+ * to 1,024 products); 0: with a sort. */
 int brm_set_param(const char *name, int64_t value);
 int64_t brm_get_param(const char *name);
 

@@ -32,9 +32,10 @@ TIER_SETTINGS = {
     "lds_wave_medium": {"wmed_cap": 512, "wmed_engine": 0},
     "general_pipeline": {"small_fused": 0},
     "wave_medium_hash_count": {"wmed_cap": 1024, "wmed_engine": 1, "wmed_count": 0},
+    "register_medium_sort": {"wmed_cap": 1024, "wmed_engine": 1, "wmed_bitmap": 0},
 }
 KNOBS = ("force_long", "leaf_cap", "med_cap", "wmed_cap", "group_cfg", "launch_chunk", "wmed_engine", "small_fused",
-         "wmed_count")
+         "wmed_count", "wmed_bitmap")
 
 
 @pytest.fixture
@@ -383,15 +384,20 @@ def test_bitmap_four_pass_rows(gpu):
         gpu.set_param("group_cfg", old)
 
 
-@pytest.mark.parametrize("ncols", [1 << 16, (1 << 16) + 1, 1 << 17, (1 << 17) + 1, 1 << 20], ids=lambda n: f"N{n}")
+@pytest.mark.parametrize("ncols", [1 << 16, (1 << 16) + 1, 1 << 17, (1 << 17) + 1, 1 << 20, (1 << 20) + 1],
+                         ids=lambda n: f"N{n}")
 @pytest.mark.parametrize("wmed_cap", [128, 512, 1024])
-def test_register_medium_key_widths(gpu, ncols, wmed_cap):
+@pytest.mark.parametrize("ranks", ["sort", "bitmap"])
+def test_register_medium_key_widths(gpu, ncols, wmed_cap, ranks):
     """Medium rows of the register tier with columns up to N - 1 on both
     sides of the 32-bit key limit (2^16 columns at 1024 products, 2^17 at
-    512): keys of either width give the same bits."""
-    saved = {k: gpu.get_param(k) for k in ("wmed_cap", "wmed_engine")}
+    512) and of the bitmap variants' limits (2^16 columns flat, 2^20
+    two-level, sort beyond): keys of either width and ranks of either kind
+    give the same bits."""
+    saved = {k: gpu.get_param(k) for k in ("wmed_cap", "wmed_engine", "wmed_bitmap")}
     gpu.set_param("wmed_cap", wmed_cap)
     gpu.set_param("wmed_engine", 1)
+    gpu.set_param("wmed_bitmap", 1 if ranks == "bitmap" else 0)
     try:
         rng = np.random.default_rng(ncols + wmed_cap)
         K = 400
@@ -417,21 +423,23 @@ def test_register_medium_key_widths(gpu, ncols, wmed_cap):
             gpu.set_param(k, v)
 
 
+@pytest.mark.parametrize("stride", [5, 130001], ids=["narrow", "wide"])
 @pytest.mark.parametrize("engine_wm", [0, 1], ids=["lds", "registers"])
-def test_medium_rows_with_deep_column_trees(gpu, engine_wm):
+def test_medium_rows_with_deep_column_trees(gpu, engine_wm, stride):
     """Rows of 64 lists whose B rows share a handful of columns: every output
     column sums up to 64 contributions through all six tree levels, with
-    absent lists at irregular positions."""
+    absent lists at irregular positions.  The columns lie `stride` apart: 40
+    columns (flat bitmap ranks) or about 2^20 (two-level ranks)."""
     saved = {k: gpu.get_param(k) for k in ("wmed_cap", "wmed_engine")}
     gpu.set_param("wmed_cap", 1024)
     gpu.set_param("wmed_engine", engine_wm)
     try:
         rng = np.random.default_rng(13)
-        K, N = 300, 40
+        K, N = 300, 8 * stride
         rows = []
         for r in range(K):
             d = 0 if r % 7 == 3 else int(rng.integers(1, 9))
-            rows.append(np.sort(rng.choice(8, size=d, replace=False) * 5))
+            rows.append(np.sort(rng.choice(8, size=d, replace=False) * stride))
         brpt = np.concatenate([[0], np.cumsum([len(b) for b in rows])]).astype(np.int32)
         mags = rng.choice([1.0, 2.0 ** 40, 2.0 ** -40, 3.0], size=brpt[-1])
         B = (brpt, np.concatenate(rows).astype(np.int32), rng.uniform(-1, 1, brpt[-1]) * mags, N)
