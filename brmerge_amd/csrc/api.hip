@@ -650,7 +650,8 @@ hipError_t bin_launch(Pattern& p, int lds_cfg, int med_cap, int wmed_cap, int fo
     p.lds_cfg = lds_cfg;
     p.med_cap = std::min(med_cap, shape.cap);
     if (wmed_cap < 0) wmed_cap = params().wmed_engine == 1 ? rmed_auto_cap(p.a.N) : 512;
-    p.wmed_cap = std::min(wmed_cap, WMED_CAP);
+    // rows above WMED_CAP only where the register tier ranks with a bitmap
+    p.wmed_cap = std::min(wmed_cap, params().wmed_engine == 1 ? rmed_max_cap(p.a.N) : WMED_CAP);
     hipError_t e = launch_fill_i32(p.bin_cnt, BIN_COUNT, 0, s);
     if (e != hipSuccess) return e;
     return launch_rowstats(p.a, p.med_cap, shape.lmax, p.wmed_cap, force_long, params().hash_cap, p.iprow, p.rowcnt,
@@ -776,7 +777,8 @@ int pattern_symbolic(Pattern& p, hipStream_t s, PhaseClock* clk, bool stash_shor
     }
     HIPC(launch_wmed_count(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, p.rowcnt, s, std::min(p.wmed_cap, WMED_SPLIT0)));
     HIPC(launch_wmed_count(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, p.rowcnt, s, std::min(p.wmed_cap, WMED_SPLIT)));
-    HIPC(launch_wmed_count(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, p.rowcnt, s, p.wmed_cap));
+    HIPC(launch_wmed_count(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, p.rowcnt, s, std::min(p.wmed_cap, WMED_SPLIT2)));
+    HIPC(launch_wmed_count(p.rows(BIN_WMED4), p.bc[BIN_WMED4], p.a, p.rowcnt, s, p.wmed_cap));
     HIPC(launch_medium_count(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, p.rowcnt, s));
     HIPC(launch_hash_count(p.rows(BIN_HASH), p.bc[BIN_HASH], p.a, p.rowcnt, s, HASH_SMALL));
     HIPC(launch_hash_count(p.rows(BIN_HASH2), p.bc[BIN_HASH2], p.a, p.rowcnt, s, HASH_CAP));
@@ -813,7 +815,8 @@ int pattern_numeric_tiers(const Pattern& p, int* ccol, double* cval, hipStream_t
     }
     HIPC(launch_wmed_numeric(p.rows(BIN_WMED), p.bc[BIN_WMED], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT0)));
     HIPC(launch_wmed_numeric(p.rows(BIN_WMED2), p.bc[BIN_WMED2], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT)));
-    HIPC(launch_wmed_numeric(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, o, s, p.wmed_cap));
+    HIPC(launch_wmed_numeric(p.rows(BIN_WMED3), p.bc[BIN_WMED3], p.a, o, s, std::min(p.wmed_cap, WMED_SPLIT2)));
+    HIPC(launch_wmed_numeric(p.rows(BIN_WMED4), p.bc[BIN_WMED4], p.a, o, s, p.wmed_cap));
     HIPC(launch_medium_numeric(p.rows(BIN_MEDIUM), p.bc[BIN_MEDIUM], p.a, o, s, p.lds_cfg));
     HIPC(launch_hash_numeric(p.rows(BIN_HASH), p.bc[BIN_HASH], p.a, o, s, HASH_SMALL));
     HIPC(launch_hash_numeric(p.rows(BIN_HASH2), p.bc[BIN_HASH2], p.a, o, s, HASH_CAP));
@@ -968,7 +971,8 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
         HIPC(launch_short_numeric(rows(BIN_SHORT), bc[BIN_SHORT], aw, o, ts));
         HIPC(launch_wmed_numeric(rows(BIN_WMED), bc[BIN_WMED], aw, o, ts, std::min(p.wmed_cap, WMED_SPLIT0)));
         HIPC(launch_wmed_numeric(rows(BIN_WMED2), bc[BIN_WMED2], aw, o, ts, std::min(p.wmed_cap, WMED_SPLIT)));
-        HIPC(launch_wmed_numeric(rows(BIN_WMED3), bc[BIN_WMED3], aw, o, ts, p.wmed_cap));
+        HIPC(launch_wmed_numeric(rows(BIN_WMED3), bc[BIN_WMED3], aw, o, ts, std::min(p.wmed_cap, WMED_SPLIT2)));
+        HIPC(launch_wmed_numeric(rows(BIN_WMED4), bc[BIN_WMED4], aw, o, ts, p.wmed_cap));
         HIPC(launch_medium_numeric(rows(BIN_MEDIUM), bc[BIN_MEDIUM], aw, o, ts, p.lds_cfg));
         HIPC(launch_hash_numeric(rows(BIN_HASH), bc[BIN_HASH], aw, o, ts, HASH_SMALL));
         HIPC(launch_hash_numeric(rows(BIN_HASH2), bc[BIN_HASH2], aw, o, ts, HASH_CAP));
@@ -1175,7 +1179,7 @@ int run_spgemm(const DevIn& in, bool upper, int64_t max_nnz, hipStream_t s, DevO
     BRMC(pattern_prepare(in, s, &clk, p));
     const int M = in.M;
     for (int b = 0; b < 4; ++b) t_stats.rows_by_tier[b] = p.bc[b];
-    t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_WMED] + p.bc[BIN_WMED2] + p.bc[BIN_WMED3];
+    t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_WMED] + p.bc[BIN_WMED2] + p.bc[BIN_WMED3] + p.bc[BIN_WMED4];
     t_stats.rows_by_tier[BIN_SHORT] += p.bc[BIN_TINY];
     t_stats.rows_by_tier[BIN_MEDIUM] += p.bc[BIN_HASH] + p.bc[BIN_HASH2];
     t_stats.total_ip = p.total_ip;
@@ -1600,7 +1604,7 @@ int brm_set_param(const char* name, int64_t value) {
     else if (!strcmp(name, "profile")) { p.profile = value ? 1 : 0; }
     else if (!strcmp(name, "lds_cfg")) { if (value < 0 || value >= N_LDS_CFG) return BRM_E_INVALID; p.lds_cfg = (int)value; }
     else if (!strcmp(name, "group_cfg")) { if (value < 0 || value >= N_GROUP_CFG) return BRM_E_INVALID; p.group_cfg = (int)value; }
-    else if (!strcmp(name, "wmed_cap")) { if (value < -1 || value > WMED_CAP) return BRM_E_INVALID; p.wmed_cap = (int)value; }
+    else if (!strcmp(name, "wmed_cap")) { if (value < -1 || value > WMED_CAP_BITMAP) return BRM_E_INVALID; p.wmed_cap = (int)value; }
     else if (!strcmp(name, "launch_chunk")) { if (value < 0) return BRM_E_INVALID; p.launch_chunk = value; }
     else if (!strcmp(name, "wmed_engine")) { if (value < 0 || value > 1) return BRM_E_INVALID; p.wmed_engine = (int)value; }
     else if (!strcmp(name, "cbar_budget_bytes")) { if (value < 0) return BRM_E_INVALID; p.cbar_budget_bytes = value; }

@@ -749,7 +749,7 @@ int brm_batch_plan_info(const brm_batch_plan* p, char* buf, uint64_t cap) {
              "\"pairs\": %d, \"rows\": %lld, \"total_ip\": %lld, \"nnz_c\": %lld, \"rows_by_tier\": [%d, %d, %d, %d], "
              "\"analyse_s\": %.6f, \"lds_cfg\": %d, \"med_cap\": %d, \"wmed_cap\": %d, \"tuning_ms\": [",
              p->fb.f.n, (long long)p->fb.f.M, (long long)p->pat.total_ip, (long long)p->pat.nnzC, p->pat.bc[0],
-             p->pat.bc[1] + p->pat.bc[BIN_TINY], p->pat.bc[2] + p->pat.bc[BIN_WMED] + p->pat.bc[BIN_WMED2] + p->pat.bc[BIN_WMED3], p->pat.bc[3], p->analyse_s, p->pat.lds_cfg,
+             p->pat.bc[1] + p->pat.bc[BIN_TINY], p->pat.bc[2] + p->pat.bc[BIN_WMED] + p->pat.bc[BIN_WMED2] + p->pat.bc[BIN_WMED3] + p->pat.bc[BIN_WMED4], p->pat.bc[3], p->analyse_s, p->pat.lds_cfg,
              p->pat.med_cap, p->pat.wmed_cap);
     s += item;
     for (size_t i = 0; i < p->tuned.size(); ++i) {

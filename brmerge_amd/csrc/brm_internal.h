@@ -20,10 +20,11 @@ enum RowBin : int {
     BIN_WMED = 4,    // ip <= min(wmed_cap, WMED_SPLIT0) and L <= WMED_LMAX: one wavefront per row
     BIN_TINY = 5,    // ip <= TINY_MAX: two rows per wavefront, merge in registers
     BIN_WMED2 = 6,   // WMED_SPLIT0 < ip <= min(wmed_cap, WMED_SPLIT), L <= WMED_LMAX
-    BIN_WMED3 = 7,   // WMED_SPLIT < ip <= wmed_cap, L <= WMED_LMAX
+    BIN_WMED3 = 7,   // WMED_SPLIT < ip <= min(wmed_cap, WMED_SPLIT2), L <= WMED_LMAX
     BIN_HASH = 8,    // ip <= hash_cap (> 0), ip and L <= HASH_SMALL: Hash-SpGEMM comparator tier (hash.hip)
     BIN_HASH2 = 9,   // other rows of ip <= hash_cap and L <= HASH_CAP: the same tier with a larger table
-    BIN_COUNT = 10
+    BIN_WMED4 = 10,  // WMED_SPLIT2 < ip <= wmed_cap, L <= WMED_LMAX (register tier with bitmap ranks only)
+    BIN_COUNT = 11
 };
 
 constexpr int SHORT_MAX = 64;
@@ -37,10 +38,14 @@ constexpr int TINY_MAX = 32;
 // per CU instead of one 256-thread workgroup per row).
 constexpr int WMED_LMAX = 64;
 constexpr int WMED_CAP = 1024;
-// The wavefront tier runs its rows in three launches split at these
+// The register tier with bitmap ranks (B of at most 2^20 columns) takes rows
+// of up to WMED_CAP_BITMAP products.
+constexpr int WMED_CAP_BITMAP = 2048;
+// The wavefront tier runs its rows in up to four launches split at these
 // product counts, so that the smaller rows get the smaller kernel shapes.
 constexpr int WMED_SPLIT0 = 256;
 constexpr int WMED_SPLIT = 512;
+constexpr int WMED_SPLIT2 = 1024;
 
 // LDS tree-merge configurations (threads, element capacity, list capacity).
 constexpr int MED_NT = 256;
@@ -185,7 +190,7 @@ struct Params {
     int profile;               // bracket every kernel launch with hipEvents
     int lds_cfg;               // LDS tier shape (threads, elements, lists): see lds_shape()
     int group_cfg;             // group-merge shape (threads, window elements): see group_shape()
-    int wmed_cap;              // wavefront tier: products per row (0 = off, <= WMED_CAP, -1 = by N)
+    int wmed_cap;              // wavefront tier: products per row (0 = off, <= WMED_CAP_BITMAP, -1 = by N)
     int64_t launch_chunk;      // test hook: rows/tasks per launch (0 = the grid limit)
     int wmed_engine;           // wavefront tier numeric: 0 LDS merge levels, 1 register bitonic sort
     int64_t cbar_budget_bytes; // Upper: C-bar bytes per row wave (0 = auto)

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(NT) void k_rowstats(RowArgs a, int med_cap, int med
             } else if (ip <= SHORT_MAX) {
                 b_ = BIN_SHORT;
             } else if (ip <= wmed_cap && L <= WMED_LMAX) {
-                b_ = ip <= WMED_SPLIT0 ? BIN_WMED : ip <= WMED_SPLIT ? BIN_WMED2 : BIN_WMED3;
+                b_ = ip <= WMED_SPLIT0 ? BIN_WMED : ip <= WMED_SPLIT ? BIN_WMED2 : ip <= WMED_SPLIT2 ? BIN_WMED3 : BIN_WMED4;
             } else if (ip <= med_cap && L <= med_lmax) {
                 b_ = BIN_MEDIUM;
             } else {

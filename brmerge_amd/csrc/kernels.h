@@ -83,6 +83,9 @@ hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const O
 // Exact row lengths of register-tier rows (ip <= cap, at most 64 lists) by
 // an LDS column bitmap; only for B of at most 2^20 columns (rmed_count_fits).
 bool rmed_count_fits(int N);
+// Largest register-tier row (products) for B of N columns under the current
+// knobs: WMED_CAP_BITMAP where the bitmap ranks apply, else WMED_CAP.
+int rmed_max_cap(int N);
 hipError_t launch_rmed_count(const int* rows, int n, const RowArgs& a, int64_t* nnzrow, hipStream_t s, int cap);
 // Wavefront-tier capacity chosen when wmed_cap is -1 (register engine): 1024
 // products while 32-bit sort keys hold N columns, else 512.

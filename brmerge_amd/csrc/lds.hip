@@ -399,7 +399,8 @@ hipError_t launch_wmed_count(const int* rows, int n, const RowArgs& a, int64_t* 
             // table of twice the row's largest product count (smaller LDS, more waves)
             if (wmed_cap <= 256) k_sym_hash<BRM_WAVE, 512, WMED_LMAX><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
             else if (wmed_cap <= 512) k_sym_hash<BRM_WAVE, 1024, WMED_LMAX><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
-            else k_sym_hash<BRM_WAVE, 2 * WMED_CAP, WMED_LMAX><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+            else if (wmed_cap <= WMED_CAP) k_sym_hash<BRM_WAVE, 2 * WMED_CAP, WMED_LMAX><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
+            else k_sym_hash<BRM_WAVE, 2 * WMED_CAP_BITMAP, WMED_LMAX><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, nnzrow);
         }
     }
     return hipGetLastError();
@@ -409,6 +410,7 @@ hipError_t launch_wmed_numeric(const int* rows, int n, const RowArgs& a, const O
                                int wmed_cap) {
     if (n <= 0) return hipSuccess;
     if (params().wmed_engine == 1) return launch_rmed_numeric(rows, n, a, o, s, wmed_cap);
+    if (wmed_cap > WMED_CAP) return hipErrorInvalidValue;  // binned only for the register tier
     if (wmed_cap <= 256) return medium_numeric_t<BRM_WAVE, 256, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");
     if (wmed_cap <= 512) return medium_numeric_t<BRM_WAVE, 512, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");
     return medium_numeric_t<BRM_WAVE, WMED_CAP, WMED_LMAX>(rows, n, a, o, s, "k_medium_w");

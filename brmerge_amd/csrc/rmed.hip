@@ -82,7 +82,7 @@ struct RmListEnt {
     double sc;
 };
 struct RmLists {
-    unsigned long long start[16];
+    unsigned long long start[32];
     RmListEnt ent[64];
 };
 static_assert(sizeof(RmLists) <= sizeof(RmSmem<2, uint32_t>), "list table fits the smallest variant");
@@ -126,7 +126,7 @@ union RbSmem {
     RbBuckets<R, TWO> b;
     RmLists l;
 };
-static_assert(64 * 16 <= 0xffff, "bucket offsets in 16 bits");
+static_assert(64 * 32 <= 0xffff, "bucket offsets in 16 bits");
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -781,6 +781,8 @@ hipError_t rmed_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hi
     // than 128 products
     const bool bm_on = R >= 4 && params().wmed_bitmap == 1;
     const int mode = !bm_on ? 0 : a.N <= RB_SPAN ? 1 : a.N <= RB2_SPAN ? 2 : 0;
+    // rows of more than 1024 products are binned only where a bitmap variant runs
+    if (R > 16 && mode == 0) return hipErrorInvalidValue;
     for (int64_t b = 0; b < n; b += ch) {
         const int c = (int)std::min<int64_t>(ch, n - b);
         if constexpr (R >= 4) {
@@ -793,8 +795,10 @@ hipError_t rmed_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hi
                 continue;
             }
         }
-        if (narrow) k_rmed<R, uint32_t, 0><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
-        else k_rmed<R, uint64_t, 0><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
+        if constexpr (R <= 16) {
+            if (narrow) k_rmed<R, uint32_t, 0><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
+            else k_rmed<R, uint64_t, 0><<<c, BRM_WAVE, 0, s>>>(rows + b, c, a, o);
+        }
     }
     return hipGetLastError();
 }
@@ -802,13 +806,18 @@ hipError_t rmed_t(const int* rows, int n, const RowArgs& a, const OutArgs& o, hi
 }  // namespace
 
 int rmed_auto_cap(int N) {
-    // The bitmap variants take rows of up to 1024 products while B has at
-    // most RB2_SPAN columns.  Otherwise 1024-product rows sort 32-bit keys
-    // only while N fits beside 16 bits of position and index; with 64-bit
-    // keys they run slower than the workgroup LDS tier, so the register tier
-    // then stops at 512 products.
-    if (params().wmed_bitmap == 1 && N <= RB2_SPAN) return 1024;
+    // The bitmap variants take rows of up to WMED_CAP_BITMAP products while
+    // B has at most RB2_SPAN columns (rows of 1025-2048 products: 2,048
+    // products per wavefront ran faster than the workgroup LDS tier).
+    // Otherwise 1024-product rows sort 32-bit keys only while N fits beside
+    // 16 bits of position and index; with 64-bit keys they run slower than
+    // the workgroup LDS tier, so the register tier then stops at 512 products.
+    if (params().wmed_bitmap == 1 && N <= RB2_SPAN) return WMED_CAP_BITMAP;
     return (int64_t)N - 1 <= RmKey<16, uint32_t>::COL_MAX ? 1024 : 512;
+}
+
+int rmed_max_cap(int N) {
+    return params().wmed_bitmap == 1 && N <= RB2_SPAN ? WMED_CAP_BITMAP : WMED_CAP;
 }
 
 bool rmed_count_fits(int N) { return N <= RB2_SPAN; }
@@ -820,7 +829,8 @@ hipError_t launch_rmed_count(const int* rows, int n, const RowArgs& a, int64_t* 
     if (cap <= 128) return rmed_count_t<2>(rows, n, a, nnzrow, s);
     if (cap <= 256) return rmed_count_t<4>(rows, n, a, nnzrow, s);
     if (cap <= 512) return rmed_count_t<8>(rows, n, a, nnzrow, s);
-    return rmed_count_t<16>(rows, n, a, nnzrow, s);
+    if (cap <= 1024) return rmed_count_t<16>(rows, n, a, nnzrow, s);
+    return rmed_count_t<32>(rows, n, a, nnzrow, s);
 }
 
 hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const OutArgs& o, hipStream_t s, int cap) {
@@ -829,7 +839,8 @@ hipError_t launch_rmed_numeric(const int* rows, int n, const RowArgs& a, const O
     if (cap <= 128) return rmed_t<2>(rows, n, a, o, s);
     if (cap <= 256) return rmed_t<4>(rows, n, a, o, s);
     if (cap <= 512) return rmed_t<8>(rows, n, a, o, s);
-    return rmed_t<16>(rows, n, a, o, s);
+    if (cap <= 1024) return rmed_t<16>(rows, n, a, o, s);
+    return rmed_t<32>(rows, n, a, o, s);
 }
 
 }  // namespace brm

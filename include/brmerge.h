@@ -162,7 +162,9 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
  * values.  Knobs other than hash_cap never change results, only speed.  group_cfg 0-3:
  * merge-path engine shapes for long rows; 4 (default): the bitmap-rank
  * engine.  wmed_cap: products per row of the wavefront tier (0 off, up to
- * 1024; -1, the default: 1024 when B has at most 65,536 columns, else 512).
+ * 2048, above 1024 only where the register tier ranks with a bitmap; -1, the
+ * default: 2048 when B has at most 2^20 columns, else 1024 when it has at
+ * most 65,536, else 512).
  * cbar_budget_bytes: BRMerge-Upper's C-bar bytes per row wave (0, the
  * default: a quarter of the memory the call can get, at most an eighth of
  * the device).  sym_chunk_products: products per list chunk of the
@@ -180,7 +182,7 @@ int brm_flop_count_dev(const int *arpt, const int *acol, int64_t nnzA, const int
  * at most 2^20 columns; 0: with the hash count.  wmed_bitmap (1, the
  * default): the register tier ranks rows of more than 128 products with a
  * column bitmap when B has at most 2^20 columns (and then takes rows of up
- * to 1,024 products); 0: with a sort. */
+ * to 2,048 products); 0: with a sort. */
 int brm_set_param(const char *name, int64_t value);
 int64_t brm_get_param(const char *name);
 

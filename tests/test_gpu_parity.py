@@ -33,6 +33,7 @@ TIER_SETTINGS = {
     "general_pipeline": {"small_fused": 0},
     "wave_medium_hash_count": {"wmed_cap": 1024, "wmed_engine": 1, "wmed_count": 0},
     "register_medium_sort": {"wmed_cap": 1024, "wmed_engine": 1, "wmed_bitmap": 0},
+    "register_medium_2048": {"wmed_cap": 2048, "wmed_engine": 1},
 }
 KNOBS = ("force_long", "leaf_cap", "med_cap", "wmed_cap", "group_cfg", "launch_chunk", "wmed_engine", "small_fused",
          "wmed_count", "wmed_bitmap")
@@ -386,14 +387,15 @@ def test_bitmap_four_pass_rows(gpu):
 
 @pytest.mark.parametrize("ncols", [1 << 16, (1 << 16) + 1, 1 << 17, (1 << 17) + 1, 1 << 20, (1 << 20) + 1],
                          ids=lambda n: f"N{n}")
-@pytest.mark.parametrize("wmed_cap", [128, 512, 1024])
+@pytest.mark.parametrize("wmed_cap", [128, 512, 1024, 2048])
 @pytest.mark.parametrize("ranks", ["sort", "bitmap"])
 def test_register_medium_key_widths(gpu, ncols, wmed_cap, ranks):
     """Medium rows of the register tier with columns up to N - 1 on both
     sides of the 32-bit key limit (2^16 columns at 1024 products, 2^17 at
     512) and of the bitmap variants' limits (2^16 columns flat, 2^20
     two-level, sort beyond): keys of either width and ranks of either kind
-    give the same bits."""
+    give the same bits.  A cap of 2048 bins rows of 1025-2048 products into
+    the register tier only where the bitmap ranks run (else 1024 holds)."""
     saved = {k: gpu.get_param(k) for k in ("wmed_cap", "wmed_engine", "wmed_bitmap")}
     gpu.set_param("wmed_cap", wmed_cap)
     gpu.set_param("wmed_engine", 1)

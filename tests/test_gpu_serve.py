@@ -1,6 +1,8 @@
 """GPU tests of the request server: each request's result must be the same
 alone (one library call) and in the mix (fused with other requests in a
 batch), bit for bit, and admission must respect the HBM budget."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -93,6 +95,12 @@ def test_server_rejects_bad_requests(gpu):
         with pytest.raises(gpu.BrmError):
             srv.submit(b + 5, dA)
         ok = gpu.DeviceCSR.upload(gpu.gen.er(30, 100, 4, 3))
+        # row pointers that do not end at the entry count given (one entry
+        # fewer: every read stays inside the arrays)
+        rid = ctypes.c_int64()
+        st = gpu.lib().brm_server_submit(srv._h, b, ok.rpt.ptr, ok.col.ptr, ok.val.ptr, ok.nnz - 1, ok.M,
+                                         ctypes.byref(rid))
+        assert st == 1
         rid = srv.submit(b, ok)
         assert srv.result(rid).M == 30
         srv.release(rid)
