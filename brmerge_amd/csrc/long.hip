@@ -176,10 +176,14 @@ __device__ void emit_group(const LongArgs& la, int row, int a0, const int64_t* i
 // nodes at height h_{j-1} inside it, at most PL_NT of them: the workgroup
 // emits each node's children together, thread t taking child slot t, with
 // a block scan for their order.
+// Nodes of fan-in F = 2^(hj - hp) <= 2^pass_bits: the workgroup plans
+// PL_NT / F nodes per step, one child slot per thread, with one scan and one
+// reservation of child entries for all of them.
 __device__ void plan_no_frontier(const LongArgs& la, int row, int a0, const int64_t* ipr, int L, int H, int J,
                                  WsBase wsb, int64_t slb, int64_t final_off, int* err) {
     __shared__ int64_t sm[PL_NT / BRM_WAVE + 1];
     __shared__ int64_t cbase;
+    __shared__ int64_t lead[PL_NT + 1];  // child prefix at each node's first slot; [G]: the step's total
     const int t = threadIdx.x;
     const int pb = la.pass_bits;
     for (int j = 1; j <= J; ++j) {
@@ -187,19 +191,30 @@ __device__ void plan_no_frontier(const LongArgs& la, int row, int a0, const int6
         const int hp = j == 1 ? 0 : pass_height(j - 1, H, pb);
         const bool root = hj == H;
         const int nnodes = root ? 1 : ((L - 1) >> hj) + 1;
-        for (int nd = 0; nd < nnodes; ++nd) {
-            const int s = nd << hj;
-            const NodeK u = node_kind(ipr, L, s, hj);  // uniform
-            if (u.kind == SRC_EMPTY || (u.kind == SRC_VIRTUAL && !root)) continue;
-            const int e = node_end(L, s, hj);
-            const int q = s + (t << hp);
+        const int fb = hj - hp;      // log2 of the child slots per node
+        const int G = PL_NT >> fb;   // nodes per step
+        const int k = t >> fb;       // this thread's node in the step
+        const int c = t & ((1 << fb) - 1);
+        for (int nd0 = 0; nd0 < nnodes; nd0 += G) {
+            const int nd = nd0 + k;
+            bool live = false;
+            int s = 0, e = 0;
+            if (nd < nnodes) {
+                s = nd << hj;
+                const NodeK u = node_kind(ipr, L, s, hj);
+                live = !(u.kind == SRC_EMPTY || (u.kind == SRC_VIRTUAL && !root));
+                e = node_end(L, s, hj);
+            }
+            const int q = s + (c << hp);
             Src src{};
             src.kind = SRC_EMPTY;
-            if (q < e) src = make_src(la, a0, ipr, L, H, q, hp, (j - 1) & 1, wsb, slb);
+            if (live && q < e) src = make_src(la, a0, ipr, L, H, q, hp, (j - 1) & 1, wsb, slb);
             const int64_t has = src.kind != SRC_EMPTY ? 1 : 0;
             int64_t nc;
             const int64_t rank = block_excl_scan<PL_NT>(has, sm, &nc);
+            if (c == 0) lead[k] = rank;
             if (t == 0) {
+                lead[G] = nc;
                 const unsigned long long cb =
                     atomicAdd((unsigned long long*)&la.counters[CNT_CHILD], (unsigned long long)nc);
                 cbase = (int64_t)cb;
@@ -207,42 +222,43 @@ __device__ void plan_no_frontier(const LongArgs& la, int row, int a0, const int6
             }
             __syncthreads();
             const int64_t cb = cbase;
-            __syncthreads();  // cbase is rewritten for the next node
-            if (cb + nc > la.child_cap) continue;  // uniform
-            if (has) {
-                Child c;
-                c.src = src;
-                c.lid = t;  // (q - s) >> hp
-                c.pad = 0;
-                la.children[cb + rank] = c;
-            }
-            if (t == 0) {
-                GroupTask T;
-                T.row = row;
-                T.nchild = (int)nc;
-                T.cbase = cb;
-                if (root) {
-                    T.final_dst = 1;
-                    T.dst_buf = 0;
-                    T.dst_off = final_off;
-                    T.slot = -1;
-                } else {
-                    T.final_dst = 0;
-                    T.dst_buf = j & 1;
-                    T.dst_off = wsb.at(j & 1) + (ipr[s] - ipr[0]);
-                    T.slot = slb + heap_idx(H, s, hj);
+            if (cb + nc <= la.child_cap) {  // uniform
+                if (has) {
+                    Child ch;
+                    ch.src = src;
+                    ch.lid = c;  // (q - s) >> hp
+                    ch.pad = 0;
+                    la.children[cb + rank] = ch;
                 }
-                const unsigned long long idx =
-                    atomicAdd((unsigned long long*)&la.counters[CNT_TASK0 + j], 1ull);
-                if ((int64_t)idx >= la.task_seg[j + 1] - la.task_seg[j]) {
-                    atomicOr(err, (int)DERR_CAPACITY);
-                } else {
-                    la.tasks[la.task_seg[j] + (int64_t)idx] = T;
-                    if (la.task_key)
-                        la.task_key[la.task_seg[j] + (int64_t)idx] =
-                            (uint32_t)min(ipr[e] - ipr[s], (int64_t)UINT32_MAX);
+                if (c == 0 && live) {
+                    GroupTask T;
+                    T.row = row;
+                    T.nchild = (int)(lead[k + 1] - lead[k]);
+                    T.cbase = cb + lead[k];
+                    if (root) {
+                        T.final_dst = 1;
+                        T.dst_buf = 0;
+                        T.dst_off = final_off;
+                        T.slot = -1;
+                    } else {
+                        T.final_dst = 0;
+                        T.dst_buf = j & 1;
+                        T.dst_off = wsb.at(j & 1) + (ipr[s] - ipr[0]);
+                        T.slot = slb + heap_idx(H, s, hj);
+                    }
+                    const unsigned long long idx =
+                        atomicAdd((unsigned long long*)&la.counters[CNT_TASK0 + j], 1ull);
+                    if ((int64_t)idx >= la.task_seg[j + 1] - la.task_seg[j]) {
+                        atomicOr(err, (int)DERR_CAPACITY);
+                    } else {
+                        la.tasks[la.task_seg[j] + (int64_t)idx] = T;
+                        if (la.task_key)
+                            la.task_key[la.task_seg[j] + (int64_t)idx] =
+                                (uint32_t)min(ipr[e] - ipr[s], (int64_t)UINT32_MAX);
+                    }
                 }
             }
+            __syncthreads();  // cbase and lead are rewritten by the next step
         }
     }
 }
