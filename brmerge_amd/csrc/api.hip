@@ -922,15 +922,24 @@ int run_upper(Pattern& p, int64_t max_nnz, hipStream_t s, PhaseClock& clk, int**
             vval = nullptr;
         }
     }
-    // The long-row workspace of every wave gets the same budget (C-bar's), so
-    // that the waves' buffers are sized alike and reused from the cache.
+    // The long-row workspace of every wave gets the same budget, so that the
+    // waves' buffers are sized alike and reused from the cache: twice C-bar's
+    // (a wave's long rows store up to 24 bytes per product in the two
+    // workspace buffers; with C-bar's budget alone they split into several
+    // workspace waves), at most a quarter of the device.
+    int64_t ws_budget = 0;
+    if (nwaves > 1) {
+        size_t fr = 0, tot = 0;
+        HIPC(hipMemGetInfo(&fr, &tot));
+        ws_budget = std::min(2 * budget, (int64_t)tot / 4);
+    }
     struct WsBudgetScope {
         int64_t saved;
         explicit WsBudgetScope(int64_t b) : saved(thread_ws_budget()) {
             if (b > 0 && saved <= 0 && params().ws_budget_bytes <= 0) set_thread_ws_budget(b);
         }
         ~WsBudgetScope() { set_thread_ws_budget(saved); }
-    } ws_scope(nwaves > 1 ? budget : 0);
+    } ws_scope(ws_budget);
     Scratch chunks;
     std::vector<int*> ch_col;
     std::vector<double*> ch_val;
