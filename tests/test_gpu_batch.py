@@ -78,6 +78,31 @@ def test_batched_matches_single_calls(gpu, force_long):
         gpu.set_param("force_long", old)
 
 
+def test_batched_wide_columns(gpu):
+    """Pairs whose B has up to 200,000 columns (more than 65,536, at most
+    2^20) and A rows of 20-60 entries over B rows of 5-40: the fused product's
+    register tier ranks with its two-level bitmap at every row size from 129
+    to 2,048 products, and rows above take the LDS tier.  Each C_i equals
+    its single call and the oracle, bit for bit."""
+    pairs = []
+    for i in range(3):
+        N = [200000, 70000, 1 << 20][i]
+        B = gpu.gen.er_var(3000, N, 5, 40, 40 + i)
+        A = gpu.gen.er_var(400, 3000, 20, 60, 50 + i)
+        pairs.append((A, B))
+    dev = [(gpu.DeviceCSR.upload(A), gpu.DeviceCSR.upload(B)) for A, B in pairs]
+    C, row_off = gpu.spgemm_batched(dev)
+    prods = gpu.split_products(C, row_off, [B.N for _, B in pairs])
+    for P, (dA, dB), (A, B) in zip(prods, dev, pairs):
+        S = gpu.spgemm_dev(dA, dB).download()
+        assert_csr_equal(P, S)
+        assert_csr_equal(P, oracle.brmerge_precise(as_tuple(A), as_tuple(B), nthreads=8))
+    C.free()
+    for a, b in dev:
+        a.free()
+        b.free()
+
+
 def test_batched_rejects_bad_pair(gpu):
     rng = np.random.default_rng(3)
     A = rand_csr_tuple(rng, 20, 10, 5)

@@ -54,6 +54,36 @@ def test_server_results_match_single_calls(gpu):
         d.free()
 
 
+def test_server_wide_b_columns(gpu):
+    """Resident B operands of up to 200,000 columns (the stacked B is wider
+    than 65,536 columns: the register tier ranks with its two-level bitmap)
+    and requests whose rows have 129 to about 2,400 products: every result
+    equals its single call, and some equal the oracle."""
+    Bs = [gpu.gen.er_var(2000, 200000, 5, 40, 61), gpu.gen.er_var(1500, 90000, 5, 40, 62)]
+    dBs = [gpu.DeviceCSR.upload(B) for B in Bs]
+    rng = np.random.default_rng(17)
+    reqs = []
+    for i in range(12):
+        b = int(rng.integers(0, 2))
+        reqs.append((b, gpu.gen.er_var(int(rng.integers(50, 400)), Bs[b].M, 20, 60, 700 + i)))
+    dAs = [gpu.DeviceCSR.upload(A) for _, A in reqs]
+    with gpu.Server(budget_bytes=8 << 30, max_batch=6, streams=2) as srv:
+        ids = [srv.add_b(d) for d in dBs]
+        rids = [srv.submit(ids[b], dA) for (b, _), dA in zip(reqs, dAs)]
+        for k, rid in enumerate(rids):
+            srv.wait(rid)
+            C = srv.result(rid)
+            b, A = reqs[k]
+            alone = gpu.spgemm_dev(dAs[k], dBs[b])
+            assert_csr_equal(C, alone.download())
+            alone.free()
+            if k % 4 == 0:
+                assert_csr_equal(C, oracle.brmerge_precise(as_tuple(A), as_tuple(Bs[b]), nthreads=8))
+            srv.release(rid)
+    for d in dBs + dAs:
+        d.free()
+
+
 def test_server_admission_budget(gpu):
     B = gpu.gen.er(2000, 2000, 10, 7)
     dB = gpu.DeviceCSR.upload(B)
