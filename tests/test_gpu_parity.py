@@ -34,6 +34,7 @@ TIER_SETTINGS = {
     "wave_medium_hash_count": {"wmed_cap": 1024, "wmed_engine": 1, "wmed_count": 0},
     "register_medium_sort": {"wmed_cap": 1024, "wmed_engine": 1, "wmed_bitmap": 0},
     "register_medium_2048": {"wmed_cap": 2048, "wmed_engine": 1},
+    "register_medium_2048_hash_count": {"wmed_cap": 2048, "wmed_engine": 1, "wmed_count": 0},
 }
 KNOBS = ("force_long", "leaf_cap", "med_cap", "wmed_cap", "group_cfg", "launch_chunk", "wmed_engine", "small_fused",
          "wmed_count", "wmed_bitmap")
