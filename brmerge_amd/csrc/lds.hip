@@ -260,6 +260,8 @@ __global__ __launch_bounds__(BM_NT) void k_sym_bitmap(const int* __restrict__ ro
     for (int64_t c0 = 0; c0 < a.N; c0 += BM_BITS) {
         const int64_t span = min((int64_t)BM_BITS, (int64_t)a.N - c0);
         const int words = (int)((span + 31) / 32);
+        const int c0i = (int)c0;
+        const unsigned span32 = (unsigned)span;
         for (int w = t; w < words; w += BM_NT) bm[w] = 0u;
         __syncthreads();
         for (int cs = ab, nl = 0; cs < ae; cs += nl) {
@@ -293,10 +295,12 @@ __global__ __launch_bounds__(BM_NT) void k_sym_bitmap(const int* __restrict__ ro
                         }
                     }
                 }
+                // 32-bit offsets: columns and c0 are below 2^31; a column
+                // below c0 (or -1, no product) wraps past the span
 #pragma unroll
                 for (int u = 0; u < BM_U; ++u) {
-                    const int64_t rel = (int64_t)col[u] - c0;
-                    if (col[u] >= 0 && rel >= 0 && rel < span) atomicOr(&bm[rel >> 5], 1u << (rel & 31));
+                    const unsigned rel = (unsigned)(col[u] - c0i);
+                    if (rel < span32) atomicOr(&bm[rel >> 5], 1u << (rel & 31));
                 }
             }
             __syncthreads();
