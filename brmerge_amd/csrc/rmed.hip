@@ -1,15 +1,15 @@
 // Register tier for small medium rows: one wavefront per row, the row's
-// products held in registers (R per lane, ip <= 64 R <= 1024, at most 64 lists).
+// products held in registers (R per lane, ip <= 64 R <= 2048, at most 64
+// lists; R = 32 only with bitmap ranks).
 //
 //   1. every lane l < L holds list l's product offset, B row start and A
 //      value; the non-empty lists mark their first product in an LDS bitmap
 //      and describe themselves in an LDS table, so each product finds its
-//      list by a popcount rank, loads its B entry and forms
-//      key = (column, list position, index) and value = a * b (rounded,
-//      BRMerge-SpGEMM/inc/spgemm.h:621).  Keys are 32-bit when the column
-//      fits beside 6 bits of list position and log2(64 R) bits of index
-//      (half the shuffles and compares), else 64-bit.
-//   Sort variant:
+//      list by a popcount rank and loads its B entry; value = a * b (rounded,
+//      BRMerge-SpGEMM/inc/spgemm.h:621).
+//   Sort variant (R <= 16): keys = (column, list position, index), 32-bit
+//   when the column fits beside 6 bits of list position and log2(64 R) bits
+//   of index (half the shuffles and compares), else 64-bit;
 //   2. a bitonic sort over the 64 R keys (cross-lane steps with shuffles,
 //      intra-lane steps between registers) orders the products by column,
 //      and within a column by list position;
@@ -20,13 +20,16 @@
 //      keeps one pending left subtree per height;
 //   4. run heads are compacted with ballots and written to C (Precise), or
 //      to C-bar with the row length (Upper).
-//   Bitmap variant (B of at most RB_SPAN columns, R >= 4): the columns set
-//   bits of an LDS bitmap over the row's column span (in chunks of 2^15
-//   columns), popcount ranks give
-//   each product its output position, products (still in registers) are
-//   bucketed by position into the bitmap's LDS, and each bucket is folded in
-//   list-position order -- no sort.
-// The symbolic count of these rows uses the wavefront hash kernel of lds.hip.
+//   Bitmap variants (B of at most RB2_SPAN columns, R >= 4): the columns set
+//   bits of an LDS bitmap -- flat over the row's column span in chunks of
+//   2^15 columns (B of at most RB_SPAN columns), or in two levels (level 1
+//   numbers the 64-column words in use, level 2 holds those words) -- and
+//   popcount ranks give each product its output position; products (still
+//   in registers) are bucketed by position into the ranks' LDS, and each
+//   bucket is folded in list-position order -- no sort.
+// The symbolic count of these rows (k_rmed_count) uses the same list lookup
+// and bitmaps; B of more than RB2_SPAN columns takes the wavefront hash
+// kernel of lds.hip.
 #include <limits.h>
 
 #include <algorithm>
