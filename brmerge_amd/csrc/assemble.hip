@@ -26,10 +26,17 @@ __global__ __launch_bounds__(CP_NT) void k_compact(const int64_t* __restrict__ c
     const int64_t q1 = min(nnzC, q0 + CP_TILE);
     if (threadIdx.x == 0) r0s = upper_idx(crpt, 0, M, q0);
     __syncthreads();
+    // a thread's outputs ascend: its row's C-bar base and end stay in
+    // registers until an output crosses into a later row
     int r = r0s;
+    int64_t sbase = 0, rend = -1;
     for (int64_t q = q0 + threadIdx.x; q < q1; q += CP_NT) {
-        r = gallop_idx(crpt, r, M, q);
-        const int64_t src = a.ipA[a.arpt[r]] + (q - crpt[r]);
+        if (q >= rend) {
+            r = gallop_idx(crpt, r, M, q);
+            sbase = a.ipA[a.arpt[r]] - crpt[r];
+            rend = crpt[r + 1];
+        }
+        const int64_t src = sbase + q;
         ccol[q] = cbar_col[src];
         cval[q] = cbar_val[src];
     }
