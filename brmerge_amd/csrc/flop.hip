@@ -50,15 +50,35 @@ __global__ __launch_bounds__(NT) void k_validate_all(RowArgs a, int64_t nnzA, in
     if (bad) atomicOr(err, (int)DERR_INVALID);
 }
 
-// Strictly increasing columns within each row; one wavefront per row.
+// Strictly increasing columns within each row.  One wavefront per 64
+// consecutive rows reads their entries 64 at a time: the pair (k, k + 1) is
+// checked unless k + 1 starts a row, and lane i holds the start of row
+// r0 + i, so a window's row starts are one OR over the lanes.
 __global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __restrict__ col, int n, int* err) {
-    const int r = blockIdx.x * (NT / BRM_WAVE) + threadIdx.x / BRM_WAVE;
-    if (r >= n) return;
+    const int r0 = (blockIdx.x * (NT / BRM_WAVE) + threadIdx.x / BRM_WAVE) * BRM_WAVE;
+    if (r0 >= n) return;
     if (*err) return;  // row pointers already flagged (checked by the launches before): not followed
-    const int b = rpt[r], e = rpt[r + 1];
+    const int lane = lane_id();
+    const int nr = min(BRM_WAVE, n - r0);
+    const int e = rpt[r0 + nr];                        // end of the segment's entries
+    const int st = lane < nr ? rpt[r0 + lane] : e;     // start of row r0 + lane
+    const int b = __shfl(st, 0, BRM_WAVE);
     bool bad = false;
-    for (int k = b + lane_id(); k + 1 < e; k += BRM_WAVE) bad |= col[k] >= col[k + 1];
-    if (__ballot(bad) != 0ull && lane_id() == 0) atomicOr(err, (int)DERR_INVALID);
+    for (int kb = b; kb + 1 < e; kb += BRM_WAVE) {     // pairs (k, k + 1), k = kb + lane
+        // bit j: entry kb + 1 + j starts a row (j = 0 .. 63)
+        const unsigned d = (unsigned)(st - kb - 1);
+        unsigned lo = d < 32u ? 1u << d : 0u;
+        unsigned hi = d - 32u < 32u ? 1u << (d - 32u) : 0u;
+#pragma unroll
+        for (int x = BRM_WAVE / 2; x > 0; x >>= 1) {
+            lo |= (unsigned)__shfl_xor((int)lo, x, BRM_WAVE);
+            hi |= (unsigned)__shfl_xor((int)hi, x, BRM_WAVE);
+        }
+        const unsigned starts = lane < 32 ? lo >> lane : hi >> (lane - 32);
+        const int k = kb + lane;
+        if (k + 1 < e && !(starts & 1u)) bad |= col[k] >= col[k + 1];
+    }
+    if (__ballot(bad) != 0ull && lane == 0) atomicOr(err, (int)DERR_INVALID);
 }
 
 // Row tiers.  A block takes RS_IT * NT rows; bin slots are counted in LDS and
@@ -164,7 +184,7 @@ hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* er
         const int64_t ch = launch_chunk(BRM_WAVE);
         for (int64_t b = 0; b < n; b += ch) {
             const int c = (int)std::min<int64_t>(ch, n - b);
-            k_validate_sorted<<<nblk(c, NT / BRM_WAVE), NT, 0, s>>>(rpt + b, col, c, err);
+            k_validate_sorted<<<nblk(nblk(c, BRM_WAVE), NT / BRM_WAVE), NT, 0, s>>>(rpt + b, col, c, err);
         }
     }
     return hipGetLastError();
