@@ -694,7 +694,7 @@ int pattern_prepare(const DevIn& in, hipStream_t s, PhaseClock* clk, Pattern& p)
     a.K = in.K;
     a.N = in.N;
     HIPC(launch_validate(a, in.nnzA, in.b_checked ? -1 : in.nnzB, p.derr, s));
-    if (!in.b_checked) HIPC(launch_validate_sorted(in.brpt, in.bcol, in.K, p.derr, s));
+    if (!in.b_checked) HIPC(launch_validate_sorted(in.brpt, in.bcol, in.K, in.nnzB, p.derr, s));
 
     // The product scan reads no B row pointer outside the array and the
     // row binning skips every row once validation has flagged the input, so

@@ -50,33 +50,65 @@ __global__ __launch_bounds__(NT) void k_validate_all(RowArgs a, int64_t nnzA, in
     if (bad) atomicOr(err, (int)DERR_INVALID);
 }
 
-// Strictly increasing columns within each row.  One wavefront per 64
-// consecutive rows reads their entries 64 at a time: the pair (k, k + 1) is
-// checked unless k + 1 starts a row, and lane i holds the start of row
-// r0 + i, so a window's row starts are one OR over the lanes.
-__global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __restrict__ col, int n, int* err) {
-    const int r0 = (blockIdx.x * (NT / BRM_WAVE) + threadIdx.x / BRM_WAVE) * BRM_WAVE;
-    if (r0 >= n) return;
+// Strictly increasing columns within each row.  Each wavefront checks the
+// pairs (k, k + 1) of a fixed chunk of VS_CHUNK entries, 64 at a time, and
+// skips a pair where k + 1 starts a row, so a hub row costs no more than
+// as many short ones.  The lanes hold 64 consecutive row starts (from the
+// first one after the chunk's first entry, found by a 64-way search); a
+// window's row starts are one OR over the lanes, and the next 64 are loaded
+// only when a window reaches past the last one held.
+constexpr int VS_CHUNK = 8 * BRM_WAVE;
+
+__global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __restrict__ col, int n, int npairs,
+                                  int64_t chunk0, int* err) {
+    const int64_t e0 = (chunk0 + (int64_t)blockIdx.x * (NT / BRM_WAVE) + threadIdx.x / BRM_WAVE) * VS_CHUNK;
+    if (e0 >= npairs) return;
     if (*err) return;  // row pointers already flagged (checked by the launches before): not followed
     const int lane = lane_id();
-    const int nr = min(BRM_WAVE, n - r0);
-    const int e = rpt[r0 + nr];                        // end of the segment's entries
-    const int st = lane < nr ? rpt[r0 + lane] : e;     // start of row r0 + lane
-    const int b = __shfl(st, 0, BRM_WAVE);
-    bool bad = false;
-    for (int kb = b; kb + 1 < e; kb += BRM_WAVE) {     // pairs (k, k + 1), k = kb + lane
-        // bit j: entry kb + 1 + j starts a row (j = 0 .. 63)
-        const unsigned d = (unsigned)(st - kb - 1);
-        unsigned lo = d < 32u ? 1u << d : 0u;
-        unsigned hi = d - 32u < 32u ? 1u << (d - 32u) : 0u;
-#pragma unroll
-        for (int x = BRM_WAVE / 2; x > 0; x >>= 1) {
-            lo |= (unsigned)__shfl_xor((int)lo, x, BRM_WAVE);
-            hi |= (unsigned)__shfl_xor((int)hi, x, BRM_WAVE);
+    const int x = (int)e0;
+    // first row start after entry x: the least i in [0, n] with rpt[i] > x
+    // (rpt[n] = npairs + 1 > x)
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int step = (hi - lo + BRM_WAVE - 1) / BRM_WAVE;
+        const int64_t p = lo + (int64_t)lane * step;
+        const bool gt = p >= hi || rpt[p] > x;
+        const uint64_t m = __ballot(gt);
+        if (m == 0ull) {
+            lo += (BRM_WAVE - 1) * step + 1;
+        } else {
+            const int f = __ffsll((long long)m) - 1;
+            if (f == 0) break;  // rpt[lo] > x
+            const int nlo = lo + (f - 1) * step + 1;
+            hi = min(hi, lo + f * step);
+            lo = nlo;
         }
-        const unsigned starts = lane < 32 ? lo >> lane : hi >> (lane - 32);
-        const int k = kb + lane;
-        if (k + 1 < e && !(starts & 1u)) bad |= col[k] >= col[k + 1];
+    }
+    int rb = lo;
+    constexpr int64_t NONE = (int64_t)1 << 40;  // past every entry
+    int64_t st = rb + lane <= n ? rpt[rb + lane] : NONE;  // start of row rb + lane
+    const int eend = (int)min<int64_t>(e0 + VS_CHUNK, npairs);
+    bool bad = false;
+    for (int64_t kb = x; kb < eend; kb += BRM_WAVE) {  // pairs (k, k + 1), k = kb + lane
+        // bit j: entry kb + 1 + j starts a row (j = 0 .. 63)
+        unsigned blo = 0u, bhi = 0u;
+        while (true) {
+            const int64_t dd = st - kb - 1;
+            const unsigned d = (uint64_t)dd < 64u ? (unsigned)dd : 64u;
+            blo |= d < 32u ? 1u << d : 0u;
+            bhi |= d - 32u < 32u ? 1u << (d - 32u) : 0u;
+            if (__shfl(st, BRM_WAVE - 1, BRM_WAVE) - kb > BRM_WAVE) break;  // the next starts lie past the window
+            rb += BRM_WAVE;
+            st = rb + lane <= n ? rpt[rb + lane] : NONE;
+        }
+#pragma unroll
+        for (int s = BRM_WAVE / 2; s > 0; s >>= 1) {
+            blo |= (unsigned)__shfl_xor((int)blo, s, BRM_WAVE);
+            bhi |= (unsigned)__shfl_xor((int)bhi, s, BRM_WAVE);
+        }
+        const unsigned starts = lane < 32 ? blo >> lane : bhi >> (lane - 32);
+        const int64_t k = kb + lane;
+        if (k < eend && !(starts & 1u)) bad |= col[k] >= col[k + 1];
     }
     if (__ballot(bad) != 0ull && lane == 0) atomicOr(err, (int)DERR_INVALID);
 }
@@ -177,14 +209,15 @@ hipError_t launch_validate(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* er
     return hipGetLastError();
 }
 
-hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* err, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
+hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int64_t nnz, int* err, hipStream_t s) {
+    if (n <= 0 || nnz <= 1) return hipSuccess;
     {
         ProfScope ps_("k_validate_sorted", s);
+        const int64_t chunks = (nnz - 1 + VS_CHUNK - 1) / VS_CHUNK;  // one wavefront each
         const int64_t ch = launch_chunk(BRM_WAVE);
-        for (int64_t b = 0; b < n; b += ch) {
-            const int c = (int)std::min<int64_t>(ch, n - b);
-            k_validate_sorted<<<nblk(nblk(c, BRM_WAVE), NT / BRM_WAVE), NT, 0, s>>>(rpt + b, col, c, err);
+        for (int64_t b = 0; b < chunks; b += ch) {
+            const int64_t c = std::min<int64_t>(ch, chunks - b);
+            k_validate_sorted<<<nblk(c, NT / BRM_WAVE), NT, 0, s>>>(rpt, col, n, (int)(nnz - 1), b, err);
         }
     }
     return hipGetLastError();

@@ -30,7 +30,8 @@ struct OutArgs {
 
 // flop.hip
 hipError_t launch_validate(const RowArgs& a, int64_t nnzA, int64_t nnzB, int* err, hipStream_t s);
-hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int* err, hipStream_t s);
+// nnz = rpt[n] (checked by launch_validate before; not followed when *err is set)
+hipError_t launch_validate_sorted(const int* rpt, const int* col, int n, int64_t nnz, int* err, hipStream_t s);
 // bins: [BIN_COUNT-1][M] row lists (bin b at (b-1)*M); bin_cnt: [BIN_COUNT]
 // Rows are all treated as empty when *err is set (err may be null).
 hipError_t launch_rowstats(const RowArgs& a, int med_cap, int med_lmax, int wmed_cap, int force_long, int hash_cap,

@@ -318,7 +318,7 @@ int brm_server_add_b(brm_server* S, const int* brpt, const int* bcol, const doub
         a.K = N;
         a.brpt = brpt;
         if (st == BRM_OK) st = hip_status(launch_validate(a, nnzB, -1, derr, S->aux), "validate");
-        if (st == BRM_OK) st = hip_status(launch_validate_sorted(brpt, bcol, K, derr, S->aux), "validate");
+        if (st == BRM_OK) st = hip_status(launch_validate_sorted(brpt, bcol, K, nnzB, derr, S->aux), "validate");
         if (st == BRM_OK) st = check_dev_err(derr, S->aux, "resident B");
         if (st == BRM_OK) st = stack_append(S, brpt, bcol, bval, nnzB, K, N, S->aux);
     }

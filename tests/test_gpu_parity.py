@@ -262,6 +262,67 @@ def test_invalid_inputs(gpu):
     assert_csr_equal(bm.brmerge_precise(good, good), R)
 
 
+def _skewed_b(rng):
+    """B with a 3,000-entry first row, runs of empty rows and short rows
+    (valid: columns strictly increase within every row, and often fall
+    from one row to the next)."""
+    lens = [3000]
+    while len(lens) < 2000:
+        lens += [0] * int(rng.integers(0, 100)) if rng.random() < 0.02 else [int(rng.integers(0, 40))]
+    lens = np.array(lens[:2000], np.int64)
+    rpt = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = np.concatenate([np.sort(rng.choice(5000, n, replace=False)) for n in lens]).astype(np.int32)
+    return rpt, col, rng.uniform(-1, 1, len(col)), 5000
+
+
+@pytest.mark.parametrize("launch_chunk", [0, 3], ids=["one_launch", "tiny_launches"])
+def test_unsorted_b_rows_skewed(gpu, launch_chunk):
+    """The sortedness check of B (fixed chunks of 512 entries per wavefront)
+    refuses a descent or a repeated column anywhere inside a row: inside the
+    hub row, across a chunk edge, after a run of empty rows, at the last pair,
+    and in a resident B of the request server; it accepts the valid B."""
+    bm = gpu
+    rng = np.random.default_rng(5)
+    B = _skewed_b(rng)
+    rpt, col = B[0], B[1]
+    nnz = int(rpt[-1])
+    A = rand_csr_tuple(rng, 300, len(rpt) - 1, 6)
+    row_of = np.repeat(np.arange(len(rpt) - 1), np.diff(rpt))
+    same = np.nonzero(row_of[:-1] == row_of[1:])[0]  # pairs (k, k + 1) inside one row
+    after_empty = [k for k in same if rpt[row_of[k]] == k and row_of[k] > 0 and rpt[row_of[k] - 1] == k]
+    at_edge = [k for k in same if k % 512 in (0, 511)]
+    picks = [0, 511, 512, 2998, int(same[-1])] + at_edge[-3:] + after_empty[:2]
+    assert nnz > 4 * 512 and len(after_empty) > 0
+    old = bm.get_param("launch_chunk")
+    bm.set_param("launch_chunk", launch_chunk)
+    try:
+        Bh = csr_from_tuple(B)
+        Ah = csr_from_tuple(A, len(rpt) - 1)
+        assert_csr_equal(bm.brmerge_precise(Ah, Bh), oracle.brmerge_precise(A, B))
+        for i, k in enumerate(picks):
+            c = col.copy()
+            if i % 2:
+                c[k + 1] = c[k]  # repeated column
+            else:
+                c[k], c[k + 1] = c[k + 1], c[k]
+            with pytest.raises(bm.BrmError) as e:
+                bm.brmerge_precise(Ah, csr_from_tuple((rpt, c, B[2], B[3])))
+            assert e.value.status == 1, k
+        c = col.copy()
+        c[picks[1]], c[picks[1] + 1] = c[picks[1] + 1], c[picks[1]]
+        dB = bm.DeviceCSR.upload(csr_from_tuple((rpt, c, B[2], B[3])))
+        try:
+            with bm.Server(budget_bytes=1 << 30) as srv:
+                with pytest.raises(bm.BrmError) as e:
+                    srv.add_b(dB)
+                assert e.value.status == 1
+        finally:
+            dB.free()
+        assert_csr_equal(bm.brmerge_precise(Ah, Bh), oracle.brmerge_precise(A, B))
+    finally:
+        bm.set_param("launch_chunk", old)
+
+
 def test_int32_overflow(gpu):
     """nnz(C) = 65,536 * 64 * 1,024 > INT32_MAX must be refused by the _i32 entry."""
     bm = gpu
