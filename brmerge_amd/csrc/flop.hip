@@ -53,11 +53,13 @@ __global__ __launch_bounds__(NT) void k_validate_all(RowArgs a, int64_t nnzA, in
 // Strictly increasing columns within each row.  Each wavefront checks the
 // pairs (k, k + 1) of a fixed chunk of VS_CHUNK entries, 64 at a time, and
 // skips a pair where k + 1 starts a row, so a hub row costs no more than
-// as many short ones.  The lanes hold 64 consecutive row starts (from the
-// first one after the chunk's first entry, found by a 64-way search); a
-// window's row starts are one OR over the lanes, and the next 64 are loaded
-// only when a window reaches past the last one held.
-constexpr int VS_CHUNK = 8 * BRM_WAVE;
+// as many short ones.  The chunk's columns are loaded first, all windows at
+// once (col[k + 1] comes from the next lane).  The lanes hold 64 consecutive
+// row starts (from the first one after the chunk's first entry, found by a
+// 64-way search); a window's row starts are one OR over the lanes, and the
+// next 64 are loaded only when a window reaches past the last one held.
+constexpr int VS_WIN = 8;
+constexpr int VS_CHUNK = VS_WIN * BRM_WAVE;
 
 __global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __restrict__ col, int n, int npairs,
                                   int64_t chunk0, int* err) {
@@ -66,6 +68,13 @@ __global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __rest
     if (*err) return;  // row pointers already flagged (checked by the launches before): not followed
     const int lane = lane_id();
     const int x = (int)e0;
+    int c[VS_WIN];  // entries 0 .. npairs (= nnz - 1) exist
+#pragma unroll
+    for (int w = 0; w < VS_WIN; ++w) {
+        const int64_t k = e0 + w * BRM_WAVE + lane;
+        c[w] = k <= npairs ? col[k] : 0;
+    }
+    const int cx = e0 + VS_CHUNK <= npairs ? col[e0 + VS_CHUNK] : 0;  // first entry of the next chunk
     // first row start after entry x: the least i in [0, n] with rpt[i] > x
     // (rpt[n] = npairs + 1 > x)
     int lo = 0, hi = n;
@@ -89,7 +98,10 @@ __global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __rest
     int64_t st = rb + lane <= n ? rpt[rb + lane] : NONE;  // start of row rb + lane
     const int eend = (int)min<int64_t>(e0 + VS_CHUNK, npairs);
     bool bad = false;
-    for (int64_t kb = x; kb < eend; kb += BRM_WAVE) {  // pairs (k, k + 1), k = kb + lane
+#pragma unroll
+    for (int w = 0; w < VS_WIN; ++w) {  // pairs (k, k + 1), k = kb + lane
+        const int64_t kb = e0 + w * BRM_WAVE;
+        if (kb >= eend) break;
         // bit j: entry kb + 1 + j starts a row (j = 0 .. 63)
         unsigned blo = 0u, bhi = 0u;
         while (true) {
@@ -107,8 +119,11 @@ __global__ void k_validate_sorted(const int* __restrict__ rpt, const int* __rest
             bhi |= (unsigned)__shfl_xor((int)bhi, s, BRM_WAVE);
         }
         const unsigned starts = lane < 32 ? blo >> lane : bhi >> (lane - 32);
+        const int first_next = w + 1 < VS_WIN ? __shfl(c[w + 1 < VS_WIN ? w + 1 : w], 0, BRM_WAVE) : cx;
+        const int down = __shfl_down(c[w], 1, BRM_WAVE);
+        const int nxt = lane == BRM_WAVE - 1 ? first_next : down;
         const int64_t k = kb + lane;
-        if (k < eend && !(starts & 1u)) bad |= col[k] >= col[k + 1];
+        if (k < eend && !(starts & 1u)) bad |= c[w] >= nxt;
     }
     if (__ballot(bad) != 0ull && lane == 0) atomicOr(err, (int)DERR_INVALID);
 }
