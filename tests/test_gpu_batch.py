@@ -122,6 +122,45 @@ def test_batched_rejects_bad_pair(gpu):
         x.free()
 
 
+def test_batched_skewed_b_and_unsorted_row(gpu):
+    """A pair whose B has a 3,000-entry first row and runs of empty rows,
+    between random pairs: the fused batch matches the oracle per product; a
+    descent inside that hub row (across a 512-entry chunk edge of the fused
+    B's sortedness check) refuses the batch, and the library stays usable."""
+    from tests.test_gpu_parity import _skewed_b
+    rng = np.random.default_rng(8)
+    Bs = _skewed_b(rng)
+    As = rand_csr_tuple(rng, 200, len(Bs[0]) - 1, 6)
+    pairs = _rand_pairs(21, 4)
+    pairs.insert(2, (As, Bs))
+    dev = [(_upload(gpu, A, len(B[0]) - 1), _upload(gpu, B, B[3])) for A, B in pairs]
+    C, ro = gpu.spgemm_batched(dev)
+    try:
+        _check_products(gpu.split_products(C, ro, [B[3] for _, B in pairs]), pairs)
+    finally:
+        C.free()
+    # the fused B holds the pairs' B entries in pair order: put the descent's
+    # second entry first in a 512-entry chunk of the fused B
+    k = (511 - sum(len(B[1]) for _, B in pairs[:2])) % 512
+    col = Bs[1].copy()
+    col[k], col[k + 1] = col[k + 1], col[k]
+    bad = _upload(gpu, (Bs[0], col, Bs[2], Bs[3]), Bs[3])
+    try:
+        with pytest.raises(gpu.BrmError) as e:
+            gpu.spgemm_batched(dev[:2] + [(dev[2][0], bad)] + dev[3:])
+        assert e.value.status == 1
+    finally:
+        bad.free()
+    C, ro = gpu.spgemm_batched(dev)
+    try:
+        _check_products(gpu.split_products(C, ro, [B[3] for _, B in pairs]), pairs)
+    finally:
+        C.free()
+        for a, b in dev:
+            a.free()
+            b.free()
+
+
 def test_batch_plan_graph_replays(gpu):
     """A plan's graph is replayed several times with other library work in
     between; each replay must reproduce the oracle, and value changes of
