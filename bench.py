@@ -40,7 +40,8 @@ def row_stats(A, B):
 
 def tiers(L, ip, med_cap, wmed_cap, force_long=False):
     """Row tiers as the library bins them: 0 empty, 1 short, 2 medium
-    (workgroup LDS), 3 long, 4 medium (wavefront LDS)."""
+    (workgroup LDS), 3 long, 4 medium (one wavefront per row: the register
+    tier, or the wavefront LDS engine)."""
     t = np.full(len(ip), 3, np.int8)
     t[ip == 0] = 0
     if not force_long:
@@ -48,6 +49,22 @@ def tiers(L, ip, med_cap, wmed_cap, force_long=False):
         t[(ip > 64) & (ip <= wmed_cap) & (L <= 64)] = 4
         t[(ip > 0) & (ip <= 64)] = 1
     return t
+
+
+def bin_caps(bm, N):
+    """(med_cap, wmed_cap) as the library's binning resolves them for B of N
+    columns (src api.hip bin_launch): the LDS tier's cap is clipped by its
+    shape (4,096 products for lds_cfg 0, 2,048 otherwise); the register
+    tier takes rows of up to 2,048 products where its bitmap ranks run (B
+    of at most 2^20 columns), else 1,024 while columns fit 16 bits, else
+    512; the LDS wavefront engine stops at 512 by default, 1,024 at most."""
+    med_cap = min(bm.get_param("med_cap"), 4096 if bm.get_param("lds_cfg") == 0 else 2048)
+    reg = bm.get_param("wmed_engine") == 1
+    bitmap = reg and bm.get_param("wmed_bitmap") == 1 and N <= (1 << 20)
+    wcap = bm.get_param("wmed_cap")
+    if wcap < 0:
+        wcap = (2048 if bitmap else 1024 if N <= 65536 else 512) if reg else 512
+    return med_cap, min(wcap, 2048 if bitmap else 1024)
 
 
 def workload_label(w):
@@ -567,10 +584,18 @@ def main():
     last.free()
     nnz_row = np.diff(crpt)
 
-    wcap = bm.get_param("wmed_cap")
-    if wcap < 0:  # the library's default: 1024 products while B has at most 65,536 columns
-        wcap = 1024 if bm.get_param("wmed_engine") == 1 and B.N <= 65536 else 512
-    tier = tiers(L, ip, bm.get_param("med_cap"), wcap)
+    med_cap, wcap = bin_caps(bm, B.N)
+    tier = tiers(L, ip, med_cap, wcap, force_long=bm.get_param("force_long") != 0)
+    # the library's own row counts: empty, short, medium (LDS and register), long
+    lib_rows = [int(x) for x in st["rows_by_tier"]]
+    if lib_rows == [0, A.M, 0, 0]:  # the fused short-row path counts every row short
+        tiers_match = bool(((tier == 0) | (tier == 1)).all())
+    else:
+        tiers_match = lib_rows == [int((tier == 0).sum()), int((tier == 1).sum()),
+                                   int(((tier == 2) | (tier == 4)).sum()), int((tier == 3).sum())]
+    if not tiers_match:
+        print(f"[bench] row tiers differ from the library's {lib_rows}: kernel bytes are approximate",
+              file=sys.stderr, flush=True)
     ms_per_step = 1000.0 * elapsed / args.steps
     gflops = 2.0 * total_ip * args.steps / elapsed / 1e9
     bytes_alg = 4 * (A.M + 1) + 20 * A.nnz + 12 * total_ip + 8 * (A.M + 1) + 12 * nnz_c
@@ -591,7 +616,8 @@ def main():
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "avg_launch_ms": round(ms / n, 4), "alg_bytes_per_launch": int(by_per_launch),
                     "launches_per_step": round(n / args.steps, 2),
-                    "traffic_over_alg": round(traffic / by_per_launch, 3) if traffic else None}
+                    "traffic_over_alg": round(traffic / by_per_launch, 3) if traffic else None,
+                    "tiers_match_library": tiers_match}
             eb = engine_bytes(dom, st)
             if eb is not None:
                 roof["engine_bytes_per_launch"] = int(eb / (n / args.steps))
